@@ -1,0 +1,242 @@
+/*
+ * nanofiller.h - C ABI of the MI355X NanoFiller rod-packing library.
+ *
+ * Plain C types only (no HIP or torch types): callers bind it from C, C++,
+ * Python ctypes or any FFI.  Every entry point returns an int status
+ * (NF_OK = 0, negative NF_E_* on failure) unless stated otherwise; a message
+ * for the last failure is available from nf_last_error().
+ *
+ * Units are micrometres.  `radius` is the rod (capsule) radius a: the
+ * reference stores the same quantity in CNT_t::a (inc/CNT.h:10) and uses
+ * a1+a2 as the hard-core distance (inc/intersection_dev.h:78).
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * pyrlinum/NanoFiller).  The reference has no FFI of its own: its drivers are
+ * in-process C++ calls sharing a simPrms object plus CUDA global state.
+ */
+#ifndef NANOFILLER_H
+#define NANOFILLER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define NF_OK 0
+#define NF_E_INVALID (-1)  /* bad argument or parameter value              */
+#define NF_E_NOMEM (-2)    /* host or device allocation failed             */
+#define NF_E_HIP (-3)      /* HIP runtime error                            */
+#define NF_E_IO (-4)       /* file could not be read or written            */
+#define NF_E_OVERFLOW (-5) /* a fixed-capacity cell or buffer overflowed   */
+#define NF_E_STATE (-6)    /* call out of order (e.g. contacts before rods)*/
+#define NF_E_NODEVICE (-7) /* no HIP device visible                        */
+#define NF_E_CAPACITY (-8) /* caller buffer too small (size reported back) */
+
+/* ---- laws --------------------------------------------------------------- */
+enum { NF_LEN_CONST = 0, NF_LEN_LOGNORMAL = 1 };
+enum { NF_ORT_ISO = 0, NF_ORT_VMF = 1 };
+enum { NF_DEN_UNIFORM = 0, NF_DEN_AGGLOM = 1 };
+enum { NF_PREC_FP64 = 0, NF_PREC_FP32_RECHECK = 1 };
+enum { NF_STATUS_RUNNING = 0, NF_STATUS_TARGET = 1, NF_STATUS_MAX_ROUNDS = 2,
+       NF_STATUS_STALLED = 3, NF_STATUS_OVERFLOW = 4 };
+
+/* Generation parameters of one packing (one instance).
+ * Mirrors the keyword-file state of simPrms (inc/simPrms.h:57-178) with the
+ * additions the MI355X generator needs (seed, length/orientation/density
+ * laws, schedule).  Fill with nf_params_default(), then nf_params_load() or
+ * field assignment, then nf_params_normalize(). */
+typedef struct nf_params {
+  double box[3];        /* periodic box edges; normalised to float-exact   */
+  uint32_t n_rods;      /* target rod count N (0: from vol_frac)           */
+  uint32_t seed;        /* Philox4x32-10 key word 0                        */
+  uint32_t stream;      /* Philox4x32-10 key word 1 (instance id)          */
+  int32_t len_law;      /* NF_LEN_*                                        */
+  double len;           /* constant length, or log-normal median          */
+  double len_sigma;     /* log-normal sigma of ln L                        */
+  double len_min;       /* log-normal clip range                           */
+  double len_max;
+  double radius;        /* rod radius a                                    */
+  double sep;           /* INC_SEPARATION: extra hard-core distance        */
+  double soft_frac;     /* INC_SOFTCORE: hard core = soft_frac*(a1+a2)+sep */
+  int32_t ort_law;      /* NF_ORT_*                                        */
+  double kappa;         /* von Mises-Fisher concentration                  */
+  double axis[3];       /* vMF mean direction (normalised)                 */
+  int32_t den_law;      /* NF_DEN_*                                        */
+  int32_t n_agg;        /* number of Gaussian agglomerates                 */
+  double agg_sigma;     /* agglomerate Gaussian sigma                      */
+  double agg_peak;      /* peak density relative to background (>= 1)      */
+  double vol_frac;      /* VOLUME_PERCENT / 100 (used when n_rods == 0)     */
+  double contact_gap;   /* STAT_DIST_LIM: contact if surface gap <= this   */
+  uint32_t max_rounds;  /* round cap (reference: 1000, cudaMain.cu:113)    */
+  uint32_t stall_rounds;/* stop after this many rounds with no acceptance  */
+  uint32_t min_batch;   /* candidates per round, lower clamp               */
+  uint32_t max_batch;   /* candidates per round, upper clamp (0: derived)  */
+  double cell_size;     /* requested generation cell edge (0: auto)        */
+  int32_t precision;    /* NF_PREC_*                                       */
+  int32_t elec_axis;    /* network electrode axis 0/1/2 (x/y/z)            */
+  double tunnel_lambda; /* tunnelling decay length (um), g = g0 exp(-gap/l)*/
+  double tunnel_g0;     /* contact conductance prefactor                   */
+} nf_params;
+
+/* Generation statistics (reference prints these; cudaMain.cu:98-157). */
+typedef struct nf_gen_stats {
+  uint32_t n_target;      /* N requested                                   */
+  uint32_t n_accepted;    /* rods in the final packing                     */
+  uint32_t rounds;        /* insert/reject rounds run                      */
+  int32_t status;         /* NF_STATUS_*                                   */
+  uint64_t candidates;    /* Philox candidate slots drawn                  */
+  uint64_t proposals;     /* candidates that passed the density test       */
+  uint64_t pair_tests;    /* capsule predicate evaluations (pairs in reach)*/
+  uint64_t rechecks;      /* FP32 results inside the error band, redone FP64*/
+  uint64_t rejected_old;  /* proposals overlapping an accepted rod         */
+  uint64_t rejected_new;  /* proposals overlapping a higher-priority one   */
+  double wall_ms;         /* host clock, first candidate -> packing on HBM */
+  double kern_ms[8];      /* HIP-event time per kernel class, see nf_kern  */
+  double max_abs_d32_d64; /* max |d32-d64| over full tests (diagnostic run)*/
+} nf_gen_stats;
+
+/* Kernel classes reported in nf_gen_stats.kern_ms. */
+enum { NF_K_GEN = 0, NF_K_BIN = 1, NF_K_REJECT_OLD = 2, NF_K_REJECT_NEW = 3,
+       NF_K_COMMIT = 4, NF_K_CONTROL = 5, NF_K_CONTACT = 6, NF_K_OTHER = 7 };
+
+typedef struct nf_handle nf_handle; /* opaque; one HIP stream per handle */
+
+/* ---- parameters --------------------------------------------------------- */
+
+/* Defaults of simPrms::simPrms (src/simPrms.cpp:22-117), with the target
+ * held fixed (SELF_ALLIGN_STEP 0) and VOL_FRAC_PRECISION 0. */
+void nf_params_default(nf_params* p);
+
+/* Keyword parameter file, same keywords as parMap (inc/parameter_input.h:39-69)
+ * and simPrms::set_ParamsFromFile (src/simPrms.cpp:212-327), plus the
+ * keywords listed in the file's header comment. */
+int nf_params_load(const char* path, nf_params* p);
+
+/* Derived values: N from the volume fraction as simPrms::set_numCNT
+ * (src/simPrms.cpp:987-1002), float-exact box edges, batch bounds. */
+int nf_params_normalize(nf_params* p);
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* One packing.  Replaces the simPrms set-up in cudaMain (src/cudaMain.cu:61-97). */
+int nf_create(const nf_params* p, nf_handle** out);
+/* n independent packings generated together (batched persistent rounds). */
+int nf_create_batch(const nf_params* ps, int n, nf_handle** out);
+void nf_destroy(nf_handle* h);
+/* Message of the last failure on h (h == NULL: last failure of nf_create*). */
+const char* nf_last_error(const nf_handle* h);
+/* Number of instances held by h. */
+int nf_num_instances(const nf_handle* h);
+
+/* ---- generation: cudaMain (src/cudaMain.cu:61-161) ---------------------- */
+
+/* Runs the insert/reject rounds for every instance of h until each reaches
+ * its target, its round cap or stalls.  st (may be NULL) receives one record
+ * per instance. */
+int nf_generate(nf_handle* h, nf_gen_stats* st);
+
+/* Rods of instance inst in id order, 8 floats each: x y z cx cy cz l a
+ * (the CNT_t record of inc/CNT.h:6-12 without the diagnostic k).
+ * *n receives the rod count; NF_E_CAPACITY if cap < count. */
+int nf_get_rods(nf_handle* h, int inst, float* rods, uint32_t cap, uint32_t* n);
+
+/* Replaces the packing of instance inst with caller rods (state reload,
+ * read_state_BIN in src/stateSR.cpp:95-134). */
+int nf_set_rods(nf_handle* h, int inst, const float* rods, uint32_t n);
+
+/* ---- contacts: get_contact_counts / get_contact_arrays
+ *      (src/get_contact_arrays.cu:12-140) and get_contact_distance
+ *      (src/get_contact_data.cu:71-128) ----------------------------------- */
+
+/* Finds every pair (I < J, periodic image) whose surface gap is <= gap on
+ * instance 0.  Result stays on the device; *n_contacts receives its size. */
+int nf_contacts(nf_handle* h, double gap, uint64_t* n_contacts);
+
+/* Copies the contact list, sorted by (I, J, image): row_ptr has N+1 entries
+ * (CSR over I), col/gap/image have n_contacts entries.  gap is the surface
+ * gap d - a_I - a_J rounded from FP64; image packs the periodic image of J
+ * as three signed bytes (x | y<<8 | z<<16).  Any pointer may be NULL. */
+int nf_get_contacts(nf_handle* h, uint32_t* row_ptr, uint32_t* col, float* gap,
+                    int32_t* image, uint64_t cap);
+
+/* ---- histograms: statKernel_ANGL / statKernel_MUT / centre profile
+ *      (src/statistics_collection.cu:640-861, inc/statKernel_ANGL.h:13-76,
+ *      inc/statKernel_MUT.h:26-131) ---------------------------------------- */
+
+typedef struct nf_hist_spec {
+  int32_t bins;       /* bins per 1-D histogram                           */
+  int32_t centre_axis;/* centre-density histogram along this axis          */
+  double pol_axis[3]; /* polar angle measured from this direction          */
+  int32_t phi_bins;   /* phi-theta 2-D histogram: phi bins (0: skip)       */
+  int32_t theta_bins; /* phi-theta 2-D histogram: theta bins               */
+} nf_hist_spec;
+
+/* Exact int64 counts.  out_centre[bins], out_polar[bins], out_mutual[bins]
+ * (mutual angle of the pairs of the last nf_contacts call), out_phitheta
+ * [phi_bins*theta_bins] (may be NULL when phi_bins == 0). */
+int nf_histograms(nf_handle* h, const nf_hist_spec* spec, int64_t* out_centre,
+                  int64_t* out_polar, int64_t* out_mutual, int64_t* out_phitheta);
+
+/* ---- network: collect_cunduct_data (src/conductivity.cu:140-423) and
+ *      adjacency_graph (src/adjacency_graph.cu:57-219) --------------------- */
+
+typedef struct nf_network_info {
+  uint32_t n_nodes;        /* N + 2 (rods, source electrode, sink)          */
+  uint32_t n_components;   /* connected components of the rod graph          */
+  uint32_t largest;        /* rods in the largest component                  */
+  uint32_t percolating;    /* components touching both electrodes           */
+  uint64_t nnz;            /* CSR non-zeros of the Kirchhoff matrix          */
+  uint64_t n_edges;        /* rod-rod conductances (contacts not crossing the
+                              electrode axis boundary)                       */
+  uint32_t n_source;       /* rods coupled to the source electrode           */
+  uint32_t n_sink;         /* rods coupled to the sink electrode             */
+} nf_network_info;
+
+/* Builds connected components and the FP64 CSR Kirchhoff (graph Laplacian)
+ * matrix from the contacts of the last nf_contacts call.  The electrode axis
+ * (params.elec_axis) is treated as non-periodic with electrodes at 0 and at
+ * the box edge; rods within the contact gap of a plane couple to it. */
+int nf_network(nf_handle* h, nf_network_info* info);
+
+/* Copies the network: labels[N] (component = smallest rod id in it),
+ * row_ptr[N+3], col[nnz], val[nnz] (FP64).  Any pointer may be NULL. */
+int nf_get_network(nf_handle* h, uint32_t* labels, uint32_t* row_ptr,
+                   uint32_t* col, double* val, uint64_t cap);
+
+/* ---- output writers ---------------------------------------------------- */
+
+/* Legacy-ASCII VTK of instance 0 as write_CNTvtk (src/IO.cpp:38-76). */
+int nf_write_vtk(nf_handle* h, const char* path);
+/* Contact lists as save_coo (src/conductivity.cu:838-879): <prefix>internal.srt.coo,
+ * <prefix>sink.elec.coo, <prefix>source.elec.coo. */
+int nf_write_coo(nf_handle* h, const char* prefix);
+
+/* ---- library ----------------------------------------------------------- */
+
+/* Version string of the library build. */
+const char* nf_version(void);
+
+/* Handle options (no counterpart in the reference, which has no runtime
+ * switches beyond compile-time flags). */
+enum {
+  NF_OPT_GRAPH = 1,        /* 1: replay generation rounds from a HIP graph (default) */
+  NF_OPT_PROFILE = 2,      /* 1: time each kernel class with HIP events (no graph)   */
+  NF_OPT_DIAG = 3,         /* 1: also compute FP64 for FP32-decided pairs and report
+                              max |d32 - d64|                                         */
+  NF_OPT_CONTACT_PREC = 4  /* NF_PREC_* for nf_contacts (-1: the params' precision)   */
+};
+int nf_set_option(nf_handle* h, int key, int64_t value);
+
+/* Statistics of the last nf_contacts call: capsule tests on pairs in reach,
+ * FP64 rechecks, max |d32-d64| (NF_OPT_DIAG), and HIP-event times in ms of
+ * the four contact stages (bin, count, record, row sort; kernel_ms[4]). */
+int nf_contacts_stats(nf_handle* h, uint64_t* pair_tests, uint64_t* rechecks, double* max_dd,
+                      float* kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NANOFILLER_H */

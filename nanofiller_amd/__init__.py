@@ -1,0 +1,362 @@
+"""Python binding of the MI355X NanoFiller library (libnanofiller.so).
+
+A thin ctypes layer over the C ABI in include/nanofiller.h, mirroring the
+reference's host interface: the simPrms parameter state and keyword file
+(inc/simPrms.h:57-178, src/simPrms.cpp:212-327), generation (cudaMain,
+src/cudaMain.cu:61), contact arrays (src/get_contact_arrays.cu:12-140),
+statistics (src/statistics_collection.cu), network extraction
+(src/conductivity.cu:140) and the VTK / COO writers.
+
+The compute path is the HIP library; there is no Python or CPU fallback.
+Loading fails loudly if libnanofiller.so has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnanofiller.so")
+
+NF_OK = 0
+NF_E_INVALID, NF_E_NOMEM, NF_E_HIP, NF_E_IO = -1, -2, -3, -4
+NF_E_OVERFLOW, NF_E_STATE, NF_E_NODEVICE, NF_E_CAPACITY = -5, -6, -7, -8
+NF_LEN_CONST, NF_LEN_LOGNORMAL = 0, 1
+NF_ORT_ISO, NF_ORT_VMF = 0, 1
+NF_DEN_UNIFORM, NF_DEN_AGGLOM = 0, 1
+NF_PREC_FP64, NF_PREC_FP32_RECHECK = 0, 1
+STATUS = {0: "running", 1: "target", 2: "max_rounds", 3: "stalled", 4: "overflow"}
+NF_OPT_GRAPH, NF_OPT_PROFILE, NF_OPT_DIAG, NF_OPT_CONTACT_PREC = 1, 2, 3, 4
+KERNEL_CLASSES = ["gen", "bin", "reject_old", "reject_new", "commit", "control", "contact", "other"]
+
+
+class NFError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"nanofiller error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """nf_params (include/nanofiller.h)."""
+
+    _fields_ = [
+        ("box", C.c_double * 3),
+        ("n_rods", C.c_uint32),
+        ("seed", C.c_uint32),
+        ("stream", C.c_uint32),
+        ("len_law", C.c_int32),
+        ("len", C.c_double),
+        ("len_sigma", C.c_double),
+        ("len_min", C.c_double),
+        ("len_max", C.c_double),
+        ("radius", C.c_double),
+        ("sep", C.c_double),
+        ("soft_frac", C.c_double),
+        ("ort_law", C.c_int32),
+        ("kappa", C.c_double),
+        ("axis", C.c_double * 3),
+        ("den_law", C.c_int32),
+        ("n_agg", C.c_int32),
+        ("agg_sigma", C.c_double),
+        ("agg_peak", C.c_double),
+        ("vol_frac", C.c_double),
+        ("contact_gap", C.c_double),
+        ("max_rounds", C.c_uint32),
+        ("stall_rounds", C.c_uint32),
+        ("min_batch", C.c_uint32),
+        ("max_batch", C.c_uint32),
+        ("cell_size", C.c_double),
+        ("precision", C.c_int32),
+        ("elec_axis", C.c_int32),
+        ("tunnel_lambda", C.c_double),
+        ("tunnel_g0", C.c_double),
+    ]
+
+    def copy(self) -> "Params":
+        p = Params()
+        C.pointer(p)[0] = self
+        return p
+
+    def to_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+class GenStats(C.Structure):
+    """nf_gen_stats (include/nanofiller.h)."""
+
+    _fields_ = [
+        ("n_target", C.c_uint32),
+        ("n_accepted", C.c_uint32),
+        ("rounds", C.c_uint32),
+        ("status", C.c_int32),
+        ("candidates", C.c_uint64),
+        ("proposals", C.c_uint64),
+        ("pair_tests", C.c_uint64),
+        ("rechecks", C.c_uint64),
+        ("rejected_old", C.c_uint64),
+        ("rejected_new", C.c_uint64),
+        ("wall_ms", C.c_double),
+        ("kern_ms", C.c_double * 8),
+        ("max_abs_d32_d64", C.c_double),
+    ]
+
+    def to_dict(self) -> dict:
+        d = {n: getattr(self, n) for n, _ in self._fields_ if n != "kern_ms"}
+        d["kern_ms"] = {k: self.kern_ms[i] for i, k in enumerate(KERNEL_CLASSES)}
+        d["status"] = STATUS.get(self.status, str(self.status))
+        return d
+
+
+class HistSpec(C.Structure):
+    _fields_ = [
+        ("bins", C.c_int32),
+        ("centre_axis", C.c_int32),
+        ("pol_axis", C.c_double * 3),
+        ("phi_bins", C.c_int32),
+        ("theta_bins", C.c_int32),
+    ]
+
+
+class NetworkInfo(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_uint32),
+        ("n_components", C.c_uint32),
+        ("largest", C.c_uint32),
+        ("percolating", C.c_uint32),
+        ("nnz", C.c_uint64),
+        ("n_edges", C.c_uint64),
+        ("n_source", C.c_uint32),
+        ("n_sink", C.c_uint32),
+    ]
+
+    def to_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_lib_handle = None
+
+# (name, restype, argtypes) of every exported entry point
+_SIGS = [
+    ("nf_params_default", None, [C.POINTER(Params)]),
+    ("nf_params_load", C.c_int, [C.c_char_p, C.POINTER(Params)]),
+    ("nf_params_normalize", C.c_int, [C.POINTER(Params)]),
+    ("nf_create", C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    ("nf_create_batch", C.c_int, [C.POINTER(Params), C.c_int, C.POINTER(C.c_void_p)]),
+    ("nf_destroy", None, [C.c_void_p]),
+    ("nf_last_error", C.c_char_p, [C.c_void_p]),
+    ("nf_num_instances", C.c_int, [C.c_void_p]),
+    ("nf_generate", C.c_int, [C.c_void_p, C.POINTER(GenStats)]),
+    ("nf_get_rods", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    ("nf_set_rods", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32]),
+    ("nf_contacts", C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_uint64)]),
+    ("nf_get_contacts", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("nf_get_contact_gap64", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("nf_contacts_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_double), C.c_void_p]),
+    ("nf_histograms", C.c_int, [C.c_void_p, C.POINTER(HistSpec), C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p]),
+    ("nf_network", C.c_int, [C.c_void_p, C.POINTER(NetworkInfo)]),
+    ("nf_get_network", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("nf_write_vtk", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("nf_write_coo", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("nf_version", C.c_char_p, []),
+    ("nf_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
+]
+
+
+def lib() -> C.CDLL:
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m nanofiller_amd.build` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib_handle = L
+    return _lib_handle
+
+
+def exported_symbols() -> list[str]:
+    return [s[0] for s in _SIGS]
+
+
+def _check(rc: int, h=None) -> None:
+    if rc != NF_OK:
+        msg = lib().nf_last_error(h)
+        raise NFError(rc, msg.decode() if msg else "")
+
+
+def default_params(**kw) -> Params:
+    """simPrms defaults (src/simPrms.cpp:22-117) with field overrides."""
+    p = Params()
+    lib().nf_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k in ("box", "axis"):
+            arr = getattr(p, k)
+            for i in range(3):
+                arr[i] = float(v[i])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def load_params(path: str, base: Params | None = None) -> Params:
+    """Keyword parameter file (simPrms::set_ParamsFromFile)."""
+    p = base.copy() if base is not None else default_params()
+    _check(lib().nf_params_load(path.encode(), C.byref(p)))
+    return p
+
+
+def normalize(p: Params) -> Params:
+    q = p.copy()
+    _check(lib().nf_params_normalize(C.byref(q)))
+    return q
+
+
+@dataclass
+class Contacts:
+    row_ptr: np.ndarray  # uint32 [N+1]
+    col: np.ndarray      # uint32 [C]
+    gap: np.ndarray      # float32 [C]
+    image: np.ndarray    # int32 [C] (3 signed bytes)
+
+    @property
+    def I(self) -> np.ndarray:  # noqa: E743
+        return np.repeat(np.arange(len(self.row_ptr) - 1, dtype=np.uint32), np.diff(self.row_ptr))
+
+    @property
+    def n(self) -> int:
+        return int(self.row_ptr[-1])
+
+
+class Packing:
+    """One handle: a single packing or a batch of independent packings."""
+
+    def __init__(self, params, options: dict | None = None):
+        plist = [params] if isinstance(params, Params) else list(params)
+        arr = (Params * len(plist))(*plist)
+        h = C.c_void_p()
+        _check(lib().nf_create_batch(arr, len(plist), C.byref(h)))
+        self._h = h
+        self.n_inst = len(plist)
+        for k, v in (options or {}).items():
+            self.set_option(k, v)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().nf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_option(self, key: int, value: int) -> None:
+        _check(lib().nf_set_option(self._h, int(key), int(value)), self._h)
+
+    def generate(self) -> list[GenStats]:
+        st = (GenStats * self.n_inst)()
+        _check(lib().nf_generate(self._h, st), self._h)
+        return list(st)
+
+    def rods(self, inst: int = 0) -> np.ndarray:
+        n = C.c_uint32()
+        _check(lib().nf_get_rods(self._h, inst, None, 0, C.byref(n)), self._h)
+        out = np.zeros((n.value, 8), dtype=np.float32)
+        _check(lib().nf_get_rods(self._h, inst, out.ctypes.data, n.value, C.byref(n)), self._h)
+        return out
+
+    def set_rods(self, rods: np.ndarray, inst: int = 0) -> None:
+        r = np.ascontiguousarray(rods, dtype=np.float32)
+        assert r.ndim == 2 and r.shape[1] == 8
+        _check(lib().nf_set_rods(self._h, inst, r.ctypes.data, r.shape[0]), self._h)
+
+    def contacts(self, gap: float) -> Contacts:
+        nc = C.c_uint64()
+        _check(lib().nf_contacts(self._h, float(gap), C.byref(nc)), self._h)
+        nrod = C.c_uint32()
+        _check(lib().nf_get_rods(self._h, 0, None, 0, C.byref(nrod)), self._h)
+        row = np.zeros(nrod.value + 1, dtype=np.uint32)
+        col = np.zeros(nc.value, dtype=np.uint32)
+        g = np.zeros(nc.value, dtype=np.float32)
+        im = np.zeros(nc.value, dtype=np.int32)
+        _check(lib().nf_get_contacts(self._h, row.ctypes.data, col.ctypes.data, g.ctypes.data,
+                                     im.ctypes.data, nc.value), self._h)
+        return Contacts(row, col, g, im)
+
+    def contact_gap64(self, n: int) -> np.ndarray:
+        out = np.zeros(n, dtype=np.float64)
+        _check(lib().nf_get_contact_gap64(self._h, out.ctypes.data, n), self._h)
+        return out
+
+    def contact_stats(self) -> dict:
+        t, r, m = C.c_uint64(), C.c_uint64(), C.c_double()
+        ms = (C.c_float * 4)()
+        _check(lib().nf_contacts_stats(self._h, C.byref(t), C.byref(r), C.byref(m), ms), self._h)
+        return {"pair_tests": t.value, "rechecks": r.value, "max_abs_d32_d64": m.value,
+                "kernel_ms": {"bin": ms[0], "count": ms[1], "record": ms[2], "sort": ms[3]}}
+
+    def histograms(self, bins=64, centre_axis=0, pol_axis=(1.0, 0.0, 0.0), mutual=True,
+                   phi_bins=0, theta_bins=0) -> dict:
+        spec = HistSpec()
+        spec.bins = bins
+        spec.centre_axis = centre_axis
+        for i in range(3):
+            spec.pol_axis[i] = pol_axis[i]
+        spec.phi_bins = phi_bins
+        spec.theta_bins = theta_bins
+        c = np.zeros(bins, np.int64)
+        p = np.zeros(bins, np.int64)
+        m = np.zeros(bins, np.int64) if mutual else None
+        pt = np.zeros(max(phi_bins * theta_bins, 1), np.int64) if phi_bins else None
+        _check(lib().nf_histograms(self._h, C.byref(spec), c.ctypes.data, p.ctypes.data,
+                                   m.ctypes.data if m is not None else None,
+                                   pt.ctypes.data if pt is not None else None), self._h)
+        out = {"centre": c, "polar": p}
+        if m is not None:
+            out["mutual"] = m
+        if pt is not None:
+            out["phitheta"] = pt.reshape(theta_bins, phi_bins)
+        return out
+
+    def network(self) -> tuple[dict, dict]:
+        info = NetworkInfo()
+        _check(lib().nf_network(self._h, C.byref(info)), self._h)
+        n = info.n_nodes - 2
+        labels = np.zeros(n, np.uint32)
+        row = np.zeros(info.n_nodes + 1, np.uint32)
+        col = np.zeros(info.nnz, np.uint32)
+        val = np.zeros(info.nnz, np.float64)
+        _check(lib().nf_get_network(self._h, labels.ctypes.data, row.ctypes.data, col.ctypes.data,
+                                    val.ctypes.data, info.nnz), self._h)
+        return info.to_dict(), {"labels": labels, "row_ptr": row, "col": col, "val": val}
+
+    def write_vtk(self, path: str) -> None:
+        _check(lib().nf_write_vtk(self._h, path.encode()), self._h)
+
+    def write_coo(self, prefix: str = "") -> None:
+        _check(lib().nf_write_coo(self._h, prefix.encode()), self._h)
+
+
+def version() -> str:
+    return lib().nf_version().decode()

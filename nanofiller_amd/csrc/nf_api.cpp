@@ -1,0 +1,641 @@
+// nf_api.cpp - C ABI of the library: parameter normalisation, handle
+// lifecycle, batched generation (HIP-graph rounds), contacts, histograms,
+// copies out.  Host orchestration only; all per-rod and per-pair work runs
+// in the HIP kernels of nf_gen.hip, nf_pair.hip and nf_post.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nf_internal.h"
+#include "nf_host.h"
+
+using namespace nf;
+
+std::string g_last_error;
+
+namespace nf {
+
+int fail(nf_handle* h, int code, const std::string& msg) {
+  if (h)
+    h->err = msg;
+  else
+    g_last_error = msg;
+  return code;
+}
+
+int dalloc_bytes(nf_handle* h, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess)
+    return fail(h, NF_E_NOMEM,
+                "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+  h->allocs.push_back(q);
+  *p = q;
+  return NF_OK;
+}
+
+void dfree(nf_handle* h, void* p) {
+  if (!p) return;
+  auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+  if (it != h->allocs.end()) h->allocs.erase(it);
+  (void)hipFree(p);
+}
+
+}  // namespace nf
+
+// ===========================================================================
+// parameters
+// ===========================================================================
+extern "C" void nf_params_default(nf_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  // simPrms::simPrms (src/simPrms.cpp:22-117)
+  p->vol_frac = 0.05 / 100.0;
+  p->len_law = NF_LEN_CONST;
+  p->len = 4.0;
+  p->len_sigma = 0.0;
+  p->len_min = 0.0;
+  p->len_max = 0.0;
+  p->radius = 0.010;
+  p->sep = 0.0;
+  p->soft_frac = 1.0;
+  p->ort_law = NF_ORT_ISO;
+  p->kappa = 0.0;
+  p->axis[0] = 0.0;
+  p->axis[1] = 0.0;
+  p->axis[2] = 1.0;
+  p->den_law = NF_DEN_UNIFORM;
+  p->agg_sigma = 1.0;
+  p->agg_peak = 1.0;
+  p->contact_gap = 0.015;
+  p->max_rounds = 1000;  // cudaMain.cu:113
+  p->stall_rounds = 8;
+  p->min_batch = 64;
+  p->max_batch = 0;
+  p->cell_size = 0.0;
+  p->precision = NF_PREC_FP32_RECHECK;
+  p->elec_axis = 0;
+  p->tunnel_lambda = 1e-4;
+  p->tunnel_g0 = 1.0;
+  p->seed = 1;
+  p->stream = 0;
+}
+
+static double excluded_volume(double L, double a) {
+  const double D = 2.0 * a;
+  const double PI = 3.14159265358979323846;
+  return 0.5 * PI * D * L * L + 2.0 * PI * D * D * L + (4.0 * PI / 3.0) * D * D * D;
+}
+
+extern "C" int nf_params_normalize(nf_params* p) {
+  for (int d = 0; d < 3; ++d) {
+    if (!(p->box[d] > 0.0)) return fail(nullptr, NF_E_INVALID, "box edges must be > 0");
+    p->box[d] = (double)(float)p->box[d];
+  }
+  if (!(p->radius > 0.0)) return fail(nullptr, NF_E_INVALID, "radius must be > 0");
+  if (!(p->len > 0.0)) return fail(nullptr, NF_E_INVALID, "length must be > 0");
+  if (p->len_law == NF_LEN_LOGNORMAL) {
+    if (!(p->len_min > 0.0) || !(p->len_max >= p->len_min) || !(p->len_sigma >= 0.0))
+      return fail(nullptr, NF_E_INVALID, "log-normal length needs 0 < len_min <= len_max, sigma >= 0");
+  } else if (p->len_law != NF_LEN_CONST) {
+    return fail(nullptr, NF_E_INVALID, "unknown length law");
+  }
+  if (p->ort_law == NF_ORT_VMF) {
+    if (!(p->kappa > 0.0)) return fail(nullptr, NF_E_INVALID, "vMF kappa must be > 0");
+    const double n = std::sqrt((p->axis[0] * p->axis[0] + p->axis[1] * p->axis[1]) + p->axis[2] * p->axis[2]);
+    if (!(n > 0.0)) return fail(nullptr, NF_E_INVALID, "vMF axis must be non-zero");
+    for (int d = 0; d < 3; ++d) p->axis[d] = p->axis[d] / n;
+  } else if (p->ort_law != NF_ORT_ISO) {
+    return fail(nullptr, NF_E_INVALID, "unknown orientation law");
+  }
+  if (p->den_law == NF_DEN_AGGLOM) {
+    if (p->n_agg < 1 || !(p->agg_sigma > 0.0) || !(p->agg_peak >= 1.0))
+      return fail(nullptr, NF_E_INVALID, "agglomerates need n_agg >= 1, sigma > 0, peak >= 1");
+  } else if (p->den_law != NF_DEN_UNIFORM) {
+    return fail(nullptr, NF_E_INVALID, "unknown density law");
+  }
+  if (!(p->soft_frac > 0.0) || p->sep < 0.0) return fail(nullptr, NF_E_INVALID, "bad hard core");
+  {
+    // periodic images per axis stay bounded: rod reach at most 16 box edges
+    const double lmax = p->len_law == NF_LEN_LOGNORMAL ? p->len_max : p->len;
+    const double reach = lmax + 2.0 * p->radius * p->soft_frac + p->sep + p->contact_gap;
+    const double bmin = std::min(p->box[0], std::min(p->box[1], p->box[2]));
+    if (reach > 16.0 * bmin) return fail(nullptr, NF_E_INVALID, "rods longer than 16 box edges");
+  }
+  const double V = p->box[0] * p->box[1] * p->box[2];
+  const double PI = 3.14159265358979323846;
+  if (p->n_rods == 0) {
+    // simPrms::set_numCNT (src/simPrms.cpp:987-1002): N = ceil(vf V / (pi a^2 l))
+    const double vcnt = PI * p->radius * p->radius * p->len;
+    const double n = std::ceil(p->vol_frac * V / vcnt);
+    if (!(n >= 1.0) || n > 4.0e9) return fail(nullptr, NF_E_INVALID, "rod count out of range");
+    p->n_rods = (uint32_t)n;
+  }
+  if (p->min_batch == 0) p->min_batch = 64;
+  if (p->max_batch == 0) {
+    const double vex = excluded_volume(p->len, p->radius);
+    double m = 0.25 * V / vex;
+    const double cap = 4.0 * (double)p->n_rods + 1024.0;
+    if (m > cap) m = cap;
+    if (m > (double)(1u << 23)) m = (double)(1u << 23);
+    if (m < 256.0) m = 256.0;
+    p->max_batch = (uint32_t)m;
+  }
+  if (p->min_batch > p->max_batch) p->min_batch = p->max_batch;
+  if (p->max_rounds == 0) p->max_rounds = 1000;
+  if (p->stall_rounds == 0) p->stall_rounds = 8;
+  if (p->precision != NF_PREC_FP64 && p->precision != NF_PREC_FP32_RECHECK)
+    return fail(nullptr, NF_E_INVALID, "unknown precision");
+  if (p->elec_axis < 0 || p->elec_axis > 2) return fail(nullptr, NF_E_INVALID, "elec_axis must be 0..2");
+  return NF_OK;
+}
+
+// ===========================================================================
+// stencils
+// ===========================================================================
+namespace nf {
+std::vector<int4> build_stencil(const int32_t nc[3], const double box[3], double reach, bool half) {
+  std::vector<std::pair<float, int4>> v;
+  const double rm = reach + (double)CELL_MARGIN;
+  int k[3];
+  double E[3];
+  for (int d = 0; d < 3; ++d) {
+    E[d] = box[d] / nc[d];
+    k[d] = (int)std::floor(rm / E[d]) + 1;
+  }
+  for (int dz = -k[2]; dz <= k[2]; ++dz)
+    for (int dy = -k[1]; dy <= k[1]; ++dy)
+      for (int dx = -k[0]; dx <= k[0]; ++dx) {
+        if (half) {
+          const bool pos = dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0)));
+          if (!pos) continue;
+        }
+        const int dd[3] = {dx, dy, dz};
+        double m2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          const int a = std::abs(dd[d]) - 1;
+          if (a > 0) m2 += (a * E[d]) * (a * E[d]);
+        }
+        if (m2 <= rm * rm) {
+          // store a slightly low minimum distance so float rounding never prunes
+          const float md2 = (float)(m2 * (1.0 - 1e-6));
+          int4 e;
+          e.x = dx;
+          e.y = dy;
+          e.z = dz;
+          e.w = 0;
+          std::memcpy(&e.w, &md2, sizeof(float));
+          v.push_back({md2, e});
+        }
+      }
+  std::stable_sort(v.begin(), v.end(), [](const std::pair<float, int4>& a, const std::pair<float, int4>& b) {
+    return a.first < b.first;
+  });
+  std::vector<int4> out;
+  out.reserve(v.size());
+  for (auto& e : v) out.push_back(e.second);
+  return out;
+}
+}  // namespace nf
+
+// ===========================================================================
+// lifecycle
+// ===========================================================================
+static int plan_instance(const nf_params& p, InstDesc& d, std::vector<double>& agg,
+                         std::vector<int4>& stencil) {
+  std::memset(&d, 0, sizeof(d));
+  SampleDesc& s = d.smp;
+  for (int a = 0; a < 3; ++a) s.box[a] = p.box[a];
+  s.len = p.len;
+  s.len_sigma = p.len_sigma;
+  s.len_min = p.len_min;
+  s.len_max = p.len_max;
+  s.radius = p.radius;
+  s.sep = p.sep;
+  s.soft_frac = p.soft_frac;
+  s.kappa = p.kappa;
+  for (int a = 0; a < 3; ++a) s.axis[a] = p.axis[a];
+  if (p.ort_law == NF_ORT_VMF) vmf_basis(s.axis, s.e1, s.e2);
+  s.agg_sigma = p.agg_sigma;
+  s.agg_peak = p.agg_peak;
+  s.seed = p.seed;
+  s.stream = p.stream;
+  s.len_law = p.len_law;
+  s.ort_law = p.ort_law;
+  s.den_law = p.den_law;
+  s.n_agg = p.den_law == NF_DEN_AGGLOM ? p.n_agg : 0;
+  s.agg = nullptr;
+  agg.resize(3 * (size_t)s.n_agg);
+  for (int k = 0; k < s.n_agg; ++k) agg_centre(p.seed, p.stream, p.box, (uint32_t)k, &agg[3 * k]);
+
+  const float lmaxf = (float)(p.len_law == NF_LEN_LOGNORMAL ? p.len_max : p.len);
+  const float af = (float)p.radius;
+  d.hl_max = 0.5f * lmaxf;
+  const double Rmax = p.soft_frac * ((double)af + (double)af) + p.sep;
+  d.R_max = (float)Rmax * 1.000001f;
+  const double reach = ((double)d.hl_max + (double)d.hl_max) * REACH_PAD + Rmax;
+  // cell edge: requested, or about one typical reach
+  double target = p.cell_size;
+  if (!(target > 0.0)) {
+    const double ltyp = p.len;
+    target = ltyp * REACH_PAD + Rmax;
+  }
+  uint64_t ncells = 1;
+  for (int a = 0; a < 3; ++a) {
+    double n = std::floor(p.box[a] / target);
+    if (n < 1.0) n = 1.0;
+    if (n > 1024.0) n = 1024.0;
+    d.ncell[a] = (int32_t)n;
+    d.boxf[a] = (float)p.box[a];
+    d.cell_scale[a] = (double)d.ncell[a] / p.box[a];
+    d.edge[a] = (float)(p.box[a] / d.ncell[a]);
+    ncells *= (uint64_t)d.ncell[a];
+  }
+  if (ncells > (1ull << 26)) return NF_E_INVALID;
+  d.ncells = (uint32_t)ncells;
+  stencil = build_stencil(d.ncell, p.box, reach, false);
+  d.st_n = (uint32_t)stencil.size();
+  // per-cell capacity for accepted rods
+  const double V = p.box[0] * p.box[1] * p.box[2];
+  const double cellV = V / (double)ncells;
+  const double peak = p.den_law == NF_DEN_AGGLOM ? p.agg_peak : 1.0;
+  const double lam = (double)p.n_rods * cellV / V * peak;
+  double cap = std::ceil(lam * 1.25 + 8.0 * std::sqrt(lam) + 16.0);
+  if (cap > (double)p.n_rods + 16.0) cap = (double)p.n_rods + 16.0;
+  d.cap = (uint32_t)cap;
+  d.n_target = p.n_rods;
+  d.min_batch = p.min_batch;
+  d.max_batch = p.max_batch;
+  d.max_rounds = p.max_rounds;
+  d.stall_rounds = p.stall_rounds;
+  d.sf64 = p.soft_frac;
+  d.sep64 = p.sep;
+  d.sf32 = (float)p.soft_frac;
+  d.sep32 = (float)p.sep;
+  const double Lmax = std::max(p.box[0], std::max(p.box[1], p.box[2]));
+  d.ew = (float)(5.9604644775390625e-8 * (Lmax + reach) * 1.01);
+  return NF_OK;
+}
+
+static void release_gen(nf_handle* h) {
+  GenBuffers& b = h->gb;
+  void* ptrs[] = {b.desc, b.st, b.ri, b.slot_base, b.stencil, b.agg, b.cell_inst, b.oP, b.oD,
+                  b.oid, b.ostart, b.ocount, b.ohl, b.bP, b.bD, b.sP, b.sD, b.scell, b.srk,
+                  b.surv, b.srank, b.pP, b.pD, b.pslot, b.palive, b.pcount, b.pstart, b.phl,
+                  b.queues, b.scan_tmp};
+  for (void* p : ptrs) dfree(h, p);
+  b = GenBuffers();
+}
+
+void nf::release_contacts(nf_handle* h) {
+  ContactBuffers& c = h->cb;
+  void* ptrs[] = {c.cP, c.cD, c.cid, c.cstart, c.ccount, c.chl, c.ccur, c.stencil, c.cnt,
+                  c.rowptr, c.fill, c.J, c.gap, c.gap64, c.img, c.labels, c.nrow, c.ncol, c.nval};
+  for (void* p : ptrs) dfree(h, p);
+  c = ContactBuffers();
+}
+
+extern "C" int nf_create_batch(const nf_params* ps, int n, nf_handle** out) {
+  if (!out) return fail(nullptr, NF_E_INVALID, "null output handle");
+  *out = nullptr;
+  if (!ps || n < 1 || n > 65536) return fail(nullptr, NF_E_INVALID, "need 1..65536 instances");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(nullptr, NF_E_NODEVICE, "no HIP device visible");
+  nf_handle* h = new nf_handle();
+  h->params.assign(ps, ps + n);
+  for (int i = 0; i < n; ++i) {
+    int rc = nf_params_normalize(&h->params[i]);
+    if (rc != NF_OK) {
+      std::string m = "instance " + std::to_string(i) + ": " + g_last_error;
+      delete h;
+      return fail(nullptr, rc, m);
+    }
+  }
+  (void)hipGetDevice(&h->device);
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(nullptr, NF_E_HIP, "hipStreamCreate failed");
+  }
+  for (auto& e : h->ev) (void)hipEventCreate(&e);
+
+  // plan all instances
+  h->desc.resize(n);
+  std::vector<std::vector<double>> aggs(n);
+  std::vector<std::vector<int4>> sts(n);
+  GenBuffers& b = h->gb;
+  b.n_inst = n;
+  uint64_t cell_base = 0, old_off = 0, rod_off = 0, T_cap = 0, agg_total = 0, st_total = 0;
+  for (int i = 0; i < n; ++i) {
+    InstDesc& d = h->desc[i];
+    if (plan_instance(h->params[i], d, aggs[i], sts[i]) != NF_OK) {
+      delete h;
+      return fail(nullptr, NF_E_INVALID, "instance " + std::to_string(i) + ": cell grid too large");
+    }
+    d.cell_base = (uint32_t)cell_base;
+    d.old_off = old_off;
+    d.rod_off = (uint32_t)rod_off;
+    d.st_off = (uint32_t)st_total;
+    cell_base += d.ncells;
+    old_off += (uint64_t)d.ncells * d.cap;
+    rod_off += d.n_target;
+    T_cap += d.max_batch;
+    agg_total += aggs[i].size();
+    st_total += sts[i].size();
+    if (cell_base > 0xFFFFFFF0ull || old_off > 0xFFFFFFF0ull || rod_off > 0xFFFFFFF0ull ||
+        T_cap > 0xFFFFFFF0ull) {
+      delete h;
+      return fail(nullptr, NF_E_INVALID, "batch exceeds 32-bit index space");
+    }
+  }
+  b.total_cells = cell_base;
+  b.total_old = old_off;
+  b.total_rods = rod_off;
+  b.T_cap = T_cap;
+  int rc = NF_OK;
+#define ALLOC(p, n_)                      \
+  if (rc == NF_OK) rc = dalloc(h, &(p), (n_));
+  ALLOC(b.desc, n);
+  ALLOC(b.st, n);
+  ALLOC(b.ri, 1);
+  ALLOC(b.slot_base, n + 1);
+  ALLOC(b.stencil, st_total);
+  ALLOC(b.agg, agg_total);
+  ALLOC(b.cell_inst, b.total_cells);
+  ALLOC(b.oP, b.total_old);
+  ALLOC(b.oD, b.total_old);
+  ALLOC(b.oid, b.total_old);
+  ALLOC(b.ostart, b.total_cells);
+  ALLOC(b.ocount, b.total_cells);
+  ALLOC(b.ohl, b.total_cells);
+  ALLOC(b.bP, b.total_rods);
+  ALLOC(b.bD, b.total_rods);
+  ALLOC(b.sP, b.T_cap);
+  ALLOC(b.sD, b.T_cap);
+  ALLOC(b.scell, b.T_cap);
+  ALLOC(b.srk, b.T_cap);
+  ALLOC(b.surv, b.T_cap + 1);
+  ALLOC(b.srank, b.T_cap + 1);
+  ALLOC(b.pP, b.T_cap);
+  ALLOC(b.pD, b.T_cap);
+  ALLOC(b.pslot, b.T_cap);
+  ALLOC(b.palive, b.T_cap);
+  ALLOC(b.pcount, b.total_cells);
+  ALLOC(b.pstart, b.total_cells + 1);
+  ALLOC(b.phl, b.total_cells);
+  ALLOC(b.queues, Q_N * QUEUES_PER_KERNEL);
+  ALLOC(b.scan_tmp, std::max(scan_tmp_words(b.total_cells), scan_tmp_words(b.T_cap)));
+#undef ALLOC
+  if (rc != NF_OK) {
+    std::string m = h->err;
+    nf_destroy(h);
+    return fail(nullptr, rc, m);
+  }
+  // upload descriptors, agglomerates, stencils, per-cell tables
+  std::vector<double> agg_all;
+  std::vector<int4> st_all;
+  std::vector<uint32_t> cinst(b.total_cells), ostart(b.total_cells);
+  for (int i = 0; i < n; ++i) {
+    InstDesc& d = h->desc[i];
+    d.smp.agg = aggs[i].empty() ? nullptr : b.agg + agg_all.size();
+    agg_all.insert(agg_all.end(), aggs[i].begin(), aggs[i].end());
+    st_all.insert(st_all.end(), sts[i].begin(), sts[i].end());
+    for (uint32_t c = 0; c < d.ncells; ++c) {
+      cinst[d.cell_base + c] = (uint32_t)i;
+      ostart[d.cell_base + c] = (uint32_t)(d.old_off + (uint64_t)c * d.cap);
+    }
+  }
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMemcpy(b.desc, h->desc.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !agg_all.empty())
+    e = hipMemcpy(b.agg, agg_all.data(), sizeof(double) * agg_all.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !st_all.empty())
+    e = hipMemcpy(b.stencil, st_all.data(), sizeof(int4) * st_all.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b.cell_inst, cinst.data(), sizeof(uint32_t) * b.total_cells, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b.ostart, ostart.data(), sizeof(uint32_t) * b.total_cells, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    nf_destroy(h);
+    return fail(nullptr, NF_E_HIP, std::string("upload failed: ") + hipGetErrorString(e));
+  }
+  h->n_rods.assign(n, 0);
+  *out = h;
+  return NF_OK;
+}
+
+extern "C" int nf_create(const nf_params* p, nf_handle** out) { return nf_create_batch(p, 1, out); }
+
+extern "C" void nf_destroy(nf_handle* h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->graph) (void)hipGraphExecDestroy(h->graph);
+  release_contacts(h);
+  release_gen(h);
+  for (void* p : h->allocs) (void)hipFree(p);
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+extern "C" const char* nf_last_error(const nf_handle* h) {
+  return h ? h->err.c_str() : g_last_error.c_str();
+}
+
+extern "C" int nf_num_instances(const nf_handle* h) { return h ? (int)h->params.size() : 0; }
+
+extern "C" const char* nf_version(void) { return "nanofiller-mi355x 0.1 (gfx950)"; }
+
+extern "C" int nf_set_option(nf_handle* h, int key, int64_t value) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  switch (key) {
+    case NF_OPT_GRAPH: h->opt_graph = value ? 1 : 0; break;
+    case NF_OPT_PROFILE: h->opt_profile = value ? 1 : 0; break;
+    case NF_OPT_DIAG: h->opt_diag = value ? 1 : 0; break;
+    case NF_OPT_CONTACT_PREC:
+      if (value != -1 && value != NF_PREC_FP64 && value != NF_PREC_FP32_RECHECK)
+        return fail(h, NF_E_INVALID, "bad contact precision");
+      h->opt_contact_prec = (int)value;
+      break;
+    default: return fail(h, NF_E_INVALID, "unknown option");
+  }
+  return NF_OK;
+}
+
+// ===========================================================================
+// generation
+// ===========================================================================
+static int reset_generation(nf_handle* h) {
+  GenBuffers& b = h->gb;
+  HIPCHK(h, hipMemsetAsync(b.st, 0, sizeof(InstState) * b.n_inst, h->stream));
+  HIPCHK(h, hipMemsetAsync(b.ri, 0, sizeof(RoundInfo), h->stream));
+  fill_u32(b.ocount, 0u, b.total_cells, h->stream);
+  fill_u32((uint32_t*)b.ohl, 0u, b.total_cells, h->stream);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return NF_OK;
+}
+
+static int build_graph(nf_handle* h, int prec, int diag) {
+  if (h->graph && h->graph_prec == prec && h->graph_diag == diag) return NF_OK;
+  if (h->graph) {
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  launch_round(h->gb, prec, diag, h->stream, nullptr);
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (e != hipSuccess) return fail(h, NF_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(h, NF_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  h->graph_prec = prec;
+  h->graph_diag = diag;
+  return NF_OK;
+}
+
+extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  h->err.clear();
+  const int prec = h->params[0].precision;
+  for (auto& p : h->params)
+    if (p.precision != prec) return fail(h, NF_E_INVALID, "all instances of a batch share one precision");
+  int rc = reset_generation(h);
+  if (rc) return rc;
+  const bool use_graph = h->opt_graph && !h->opt_profile;
+  if (use_graph) {
+    rc = build_graph(h, prec, h->opt_diag);
+    if (rc) return rc;
+  }
+  double kern[8] = {0};
+  RoundInfo ri{};
+  RoundInfo* ri_host = nullptr;
+  HIPCHK(h, hipHostMalloc((void**)&ri_host, sizeof(RoundInfo), hipHostMallocDefault));
+  uint32_t max_rounds = 0;
+  for (auto& p : h->params) max_rounds = std::max(max_rounds, p.max_rounds);
+  const auto t0 = std::chrono::steady_clock::now();
+  uint32_t launched = 0;
+  int batch = 1;
+  hipError_t e = hipSuccess;
+  for (;;) {
+    for (int k = 0; k < batch && e == hipSuccess; ++k) {
+      if (use_graph) {
+        e = hipGraphLaunch(h->graph, h->stream);
+      } else {
+        launch_round(h->gb, prec, h->opt_diag, h->stream, h->opt_profile ? h->ev : nullptr);
+        e = hipGetLastError();
+        if (e == hipSuccess && h->opt_profile) {
+          e = hipEventSynchronize(h->ev[7]);
+          float ms = 0;
+          const int cls[7] = {NF_K_CONTROL, NF_K_GEN, NF_K_BIN, NF_K_REJECT_OLD, NF_K_REJECT_NEW,
+                              NF_K_COMMIT, NF_K_CONTROL};
+          for (int c = 0; c < 7 && e == hipSuccess; ++c) {
+            if (hipEventElapsedTime(&ms, h->ev[c], h->ev[c + 1]) == hipSuccess) kern[cls[c]] += ms;
+          }
+        }
+      }
+      ++launched;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(ri_host, h->gb.ri, sizeof(RoundInfo), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) break;
+    ri = *ri_host;
+    if (ri.running == 0 || launched > max_rounds + 2) break;
+    if (batch < 8) batch *= 2;
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  (void)hipHostFree(ri_host);
+  if (e != hipSuccess) return fail(h, NF_E_HIP, std::string("generation: ") + hipGetErrorString(e));
+  const double wall = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  std::vector<InstState> st(h->gb.n_inst);
+  HIPCHK(h, hipMemcpy(st.data(), h->gb.st, sizeof(InstState) * st.size(), hipMemcpyDeviceToHost));
+  int status_rc = NF_OK;
+  for (int i = 0; i < h->gb.n_inst; ++i) {
+    h->n_rods[i] = st[i].n_acc;
+    if (st[i].status == NF_STATUS_OVERFLOW) status_rc = NF_E_OVERFLOW;
+    if (stats) {
+      nf_gen_stats& s = stats[i];
+      std::memset(&s, 0, sizeof(s));
+      s.n_target = h->desc[i].n_target;
+      s.n_accepted = st[i].n_acc;
+      s.rounds = st[i].round;
+      s.status = (int32_t)st[i].status;
+      s.candidates = st[i].candidates;
+      s.proposals = st[i].proposals;
+      s.pair_tests = st[i].pair_tests;
+      s.rechecks = st[i].rechecks;
+      s.rejected_old = st[i].rej_old;
+      s.rejected_new = st[i].rej_new;
+      s.wall_ms = wall;
+      for (int k = 0; k < 8; ++k) s.kern_ms[k] = kern[k];
+      s.max_abs_d32_d64 = st[i].max_dd;
+    }
+  }
+  h->generated = true;
+  h->cb.valid = false;
+  h->cb.net_valid = false;
+  if (status_rc != NF_OK) return fail(h, status_rc, "a cell exceeded its capacity during generation");
+  return NF_OK;
+}
+
+extern "C" int nf_get_rods(nf_handle* h, int inst, float* rods, uint32_t cap, uint32_t* n) {
+  if (!h || inst < 0 || inst >= (int)h->params.size()) return fail(h, NF_E_INVALID, "bad instance");
+  const uint32_t cnt = h->n_rods[inst];
+  if (n) *n = cnt;
+  if (!rods) return NF_OK;
+  if (cap < cnt) return fail(h, NF_E_CAPACITY, "rod buffer too small");
+  std::vector<float4> P(cnt), D(cnt);
+  const uint32_t off = h->desc[inst].rod_off;
+  if (cnt) {
+    HIPCHK(h, hipMemcpy(P.data(), h->gb.bP + off, sizeof(float4) * cnt, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(D.data(), h->gb.bD + off, sizeof(float4) * cnt, hipMemcpyDeviceToHost));
+  }
+  for (uint32_t i = 0; i < cnt; ++i) {
+    float* r = rods + 8 * (size_t)i;
+    r[0] = P[i].x;
+    r[1] = P[i].y;
+    r[2] = P[i].z;
+    r[3] = D[i].x;
+    r[4] = D[i].y;
+    r[5] = D[i].z;
+    r[6] = 2.0f * P[i].w;
+    r[7] = D[i].w;
+  }
+  return NF_OK;
+}
+
+extern "C" int nf_set_rods(nf_handle* h, int inst, const float* rods, uint32_t n) {
+  if (!h || inst < 0 || inst >= (int)h->params.size()) return fail(h, NF_E_INVALID, "bad instance");
+  if (n > h->desc[inst].n_target) return fail(h, NF_E_CAPACITY, "more rods than the instance target");
+  if (n && !rods) return fail(h, NF_E_INVALID, "null rods");
+  std::vector<float4> P(n), D(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* r = rods + 8 * (size_t)i;
+    if (!(r[6] > 0.0f) || !(r[7] > 0.0f)) return fail(h, NF_E_INVALID, "rod length and radius must be > 0");
+    for (int a = 0; a < 3; ++a)
+      if (!(r[a] >= 0.0f && r[a] < (float)h->params[inst].box[a]))
+        return fail(h, NF_E_INVALID, "rod centre outside the box");
+    P[i] = make_float4(r[0], r[1], r[2], 0.5f * r[6]);
+    D[i] = make_float4(r[3], r[4], r[5], r[7]);
+  }
+  const uint32_t off = h->desc[inst].rod_off;
+  if (n) {
+    HIPCHK(h, hipMemcpy(h->gb.bP + off, P.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->gb.bD + off, D.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+  }
+  h->n_rods[inst] = n;
+  h->generated = false;
+  h->cb.valid = false;
+  h->cb.net_valid = false;
+  return NF_OK;
+}
+

@@ -1,0 +1,467 @@
+// nf_common.h - arithmetic shared by the HIP kernels and the host side of
+// the library: Philox4x32-10, uniform conversion, exp/log from basic IEEE
+// operations, candidate sampling, and the capsule (segment-segment) distance
+// predicates in FP64 and in FP32 with a rigorous error band.
+//
+// Every double expression is written in the exact operation order the
+// candidate stream and the FP64 decisions are defined by; the library is
+// compiled with -ffp-contract=off so no operation is fused or reordered.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/nanofiller.h"
+
+#define NF_HD __host__ __device__ __forceinline__
+
+namespace nf {
+
+// --------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Counter-based, so a
+// candidate's random numbers depend only on (slot, round, draw, tag) and the
+// key (seed, stream); replaces curand XORWOW seeded from clock()
+// (src/simPrms.cpp:415, inc/cuda_CNTgenerate.h:200).
+// --------------------------------------------------------------------------
+struct u32x4 {
+  uint32_t x, y, z, w;
+};
+
+NF_HD u32x4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+  }
+  return u32x4{c0, c1, c2, c3};
+}
+
+// 52 random bits -> double in (0, 1); exact (m + 0.5 < 2^52 fits 53 bits).
+NF_HD double u01(uint32_t a, uint32_t b) {
+  const uint64_t m = ((uint64_t)(a >> 12) << 32) | (uint64_t)b;
+  return ((double)m + 0.5) * 0x1p-52;
+}
+
+// --------------------------------------------------------------------------
+// exp / log from +, -, *, / only (Cody-Waite reduction; Taylor series for
+// e^r, |r| <= ln2/2; atanh series for log of m in [sqrt(1/2), sqrt(2))).
+// Bit-identical on the CPU and the GPU because each step is one correctly
+// rounded IEEE operation.
+// --------------------------------------------------------------------------
+NF_HD double nf_bits_to_double(uint64_t u) { return __builtin_bit_cast(double, u); }
+
+NF_HD double pow2i(int k) {
+  return nf_bits_to_double((uint64_t)(k + 1023) << 52);
+}
+
+NF_HD double dexp(double x) {
+  if (x != x) return x;
+  if (x > 709.0) return nf_bits_to_double(0x7FF0000000000000ull);
+  if (x < -745.0) return 0.0;
+  const double LN2_HI = 6.93147180369123816490e-01;
+  const double LN2_LO = 1.90821492927058770002e-10;
+  const double INV_LN2 = 1.44269504088896338700e+00;
+  const double kd = floor(x * INV_LN2 + 0.5);
+  const int k = (int)kd;
+  const double r = (x - kd * LN2_HI) - kd * LN2_LO;
+  double p = 1.0 / 6227020800.0;
+  p = p * r + 1.0 / 479001600.0;
+  p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  if (k > 1023) return (p * pow2i(1023)) * pow2i(k - 1023);
+  if (k < -1022) return (p * pow2i(k + 600)) * pow2i(-600);
+  return p * pow2i(k);
+}
+
+NF_HD double dlog(double x) {
+  if (x != x || x < 0.0) return nf_bits_to_double(0x7FF8000000000000ull);
+  if (x == 0.0) return nf_bits_to_double(0xFFF0000000000000ull);
+  if (x == nf_bits_to_double(0x7FF0000000000000ull)) return x;
+  int e = 0;
+  if (x < 0x1p-1022) {
+    x = x * 0x1p54;
+    e = -54;
+  }
+  uint64_t u = __builtin_bit_cast(uint64_t, x);
+  e += (int)((u >> 52) & 0x7FF) - 1023;
+  u = (u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
+  double m = nf_bits_to_double(u);
+  if (m > 1.41421356237309504880) {
+    m = m * 0.5;
+    e += 1;
+  }
+  const double f = (m - 1.0) / (m + 1.0);
+  const double f2 = f * f;
+  double s = 1.0 / 21.0;
+  s = s * f2 + 1.0 / 19.0;
+  s = s * f2 + 1.0 / 17.0;
+  s = s * f2 + 1.0 / 15.0;
+  s = s * f2 + 1.0 / 13.0;
+  s = s * f2 + 1.0 / 11.0;
+  s = s * f2 + 1.0 / 9.0;
+  s = s * f2 + 1.0 / 7.0;
+  s = s * f2 + 1.0 / 5.0;
+  s = s * f2 + 1.0 / 3.0;
+  s = s * f2 + 1.0;
+  const double lm = (2.0 * f) * s;
+  const double ed = (double)e;
+  const double LN2_HI = 6.93147180369123816490e-01;
+  const double LN2_LO = 1.90821492927058770002e-10;
+  return ed * LN2_HI + (lm + ed * LN2_LO);
+}
+
+// --------------------------------------------------------------------------
+// Per-instance sampling description (device-resident, one per packing).
+// --------------------------------------------------------------------------
+struct SampleDesc {
+  double box[3];
+  double len, len_sigma, len_min, len_max;
+  double radius, sep, soft_frac;
+  double kappa;
+  double axis[3], e1[3], e2[3];
+  double agg_sigma, agg_peak;
+  uint32_t seed, stream;
+  int32_t len_law, ort_law, den_law, n_agg;
+  const double* agg;  // 3 doubles per agglomerate centre
+};
+
+// Basis perpendicular to the vMF axis mu: the least-aligned coordinate axis
+// projected out of mu, normalised, and mu x e1.
+NF_HD void vmf_basis(const double mu[3], double e1[3], double e2[3]) {
+  double t[3] = {0.0, 0.0, 0.0};
+  const double ax = fabs(mu[0]), ay = fabs(mu[1]), az = fabs(mu[2]);
+  if (ax <= ay && ax <= az)
+    t[0] = 1.0;
+  else if (ay <= az)
+    t[1] = 1.0;
+  else
+    t[2] = 1.0;
+  const double d = (t[0] * mu[0] + t[1] * mu[1]) + t[2] * mu[2];
+  const double u0 = t[0] - d * mu[0], u1 = t[1] - d * mu[1], u2 = t[2] - d * mu[2];
+  const double nrm = sqrt((u0 * u0 + u1 * u1) + u2 * u2);
+  e1[0] = u0 / nrm;
+  e1[1] = u1 / nrm;
+  e1[2] = u2 / nrm;
+  e2[0] = mu[1] * e1[2] - mu[2] * e1[1];
+  e2[1] = mu[2] * e1[0] - mu[0] * e1[2];
+  e2[2] = mu[0] * e1[1] - mu[1] * e1[0];
+}
+
+// Agglomerate centre k: Philox counter (k, 0, d, 1).
+NF_HD void agg_centre(uint32_t seed, uint32_t stream, const double box[3], uint32_t k,
+                      double out[3]) {
+  const u32x4 o = philox(k, 0u, 0u, 1u, seed, stream);
+  const u32x4 q = philox(k, 0u, 1u, 1u, seed, stream);
+  out[0] = u01(o.x, o.y) * box[0];
+  out[1] = u01(o.z, o.w) * box[1];
+  out[2] = u01(q.x, q.y) * box[2];
+}
+
+// Rod record as the kernels store it: P = (x, y, z, l/2), D = (cx, cy, cz, a).
+struct Rod {
+  float4 P, D;
+};
+
+// --------------------------------------------------------------------------
+// Capsule distance, FP64.  Axis segments P = r1 + s h1, Q = r2 + shift +
+// t h2 with s, t in [-1, 1] and h = (l/2) c; the exact minimiser of |P-Q|^2
+// (clamp s, solve t, clamp t and re-solve s).  Plays the role of
+// distance_cylinder_capped (inc/intersect/intersect_cylinder_capped.h:69-147)
+// and cnt_intersec_shr (inc/intersection_dev.h:10-80), without their l/2+a
+// clamp slack and single-pass clamp.  *within: centre distance within
+// (l1/2 + l2/2) * REACH_PAD + R.
+// --------------------------------------------------------------------------
+constexpr double REACH_PAD = 1.000001;
+
+NF_HD double pair_d2_f64(const float4 P1, const float4 D1, const float4 P2, const float4 D2,
+                         const double sx, const double sy, const double sz, const double R,
+                         bool* within) {
+  const double wx = ((double)P1.x - (double)P2.x) - sx;
+  const double wy = ((double)P1.y - (double)P2.y) - sy;
+  const double wz = ((double)P1.z - (double)P2.z) - sz;
+  const double hli = (double)P1.w, hlj = (double)P2.w;
+  const double rr = (hli + hlj) * REACH_PAD + R;
+  const double w2 = (wx * wx + wy * wy) + wz * wz;
+  *within = (w2 <= rr * rr);
+  const double h1x = hli * (double)D1.x, h1y = hli * (double)D1.y, h1z = hli * (double)D1.z;
+  const double h2x = hlj * (double)D2.x, h2y = hlj * (double)D2.y, h2z = hlj * (double)D2.z;
+  const double a = (h1x * h1x + h1y * h1y) + h1z * h1z;
+  const double e = (h2x * h2x + h2y * h2y) + h2z * h2z;
+  const double b = (h1x * h2x + h1y * h2y) + h1z * h2z;
+  const double c = (h1x * wx + h1y * wy) + h1z * wz;
+  const double f = (h2x * wx + h2y * wy) + h2z * wz;
+  const double denom = a * e - b * b;
+  double s = 0.0;
+  if (denom > 0.0) {
+    s = (b * f - c * e) / denom;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  }
+  double t = (b * s + f) / e;
+  if (t < -1.0) {
+    t = -1.0;
+    s = (-b - c) / a;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  } else if (t > 1.0) {
+    t = 1.0;
+    s = (b - c) / a;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  }
+  const double vx = (wx + s * h1x) - t * h2x;
+  const double vy = (wy + s * h1y) - t * h2y;
+  const double vz = (wz + s * h1z) - t * h2z;
+  return (vx * vx + vy * vy) + vz * vz;
+}
+
+// --------------------------------------------------------------------------
+// Capsule predicate, FP32 with a rigorous error band.
+//
+// Returns 1 (d64^2 below the cut: overlap / contact), 0 (not), or -1
+// (undecided: the caller recomputes with pair_d2_f64).  `strict` selects
+// d^2 < R^2 (hard core) versus d^2 <= R^2 (contact); the result agrees with
+// the FP64 rule `within64 && d2_64 (<|<=) R64^2` whenever it is not -1.
+// *within is the reach result when decided (meaningful only if ret != -1).
+//
+// Band.  Let F(s,t) = |w + s h1 - t h2|^2 with exact inputs and F* its
+// minimum over the box.  The FP32 point (s,t) is feasible, so
+// F* <= F(s,t) <= d2c + E_F, where E_F bounds the evaluation error of v and
+// v.v (input rounding of w and h, gamma_3 per dot product).  F is convex, so
+// F(s,t) - F* <= 2 sum_i (g_i x_i + |g_i|) (Frank-Wolfe gap over the box,
+// g = (h1.v, -h2.v)); the gap is computed in FP32 and inflated by its own
+// error bound.  The FP64 value differs from F* by at most EPS64 * S^2
+// (S = |w|+|h1|+|h2|), a bound with a margin of >100x over the FP64
+// algorithm's worst case.
+// --------------------------------------------------------------------------
+constexpr float U32 = 5.9604644775390625e-8f;  // 2^-24
+constexpr float EPS64_REL = 1e-12f;
+
+struct F32Pair {
+  int decided;  // -1 undecided, else the cut result (0/1)
+  int within;   // reach result when decided
+  float d2;     // FP32 squared distance (valid when within)
+};
+
+NF_HD F32Pair pair_f32(const float4 P1, const float4 D1, const float4 P2, const float4 D2,
+                       const float sx, const float sy, const float sz, const float R,
+                       const float ew, const bool strict) {
+  F32Pair out;
+  const float wx = (P1.x - P2.x) - sx;
+  const float wy = (P1.y - P2.y) - sy;
+  const float wz = (P1.z - P2.z) - sz;
+  const float w2 = __fmaf_rn(wz, wz, __fmaf_rn(wy, wy, wx * wx));
+  const float rr = (P1.w + P2.w) * 1.000001f + R;
+  const float rr2 = rr * rr;
+  // reach band: w components err <= ew; rr differs from the FP64 reach by
+  // < 3.5e-7 relative (5 roundings + the pad constant), so rr^2 by < 8e-7.
+  const float nw = sqrtf(w2) + 2.0f * ew;
+  const float ew2 = 1.7321f * ew;
+  const float e_w2 = 4.0f * U32 * w2 + (2.0f * nw + ew2) * ew2;
+  const float rlo = rr2 * (1.0f - 1.5e-6f), rhi = rr2 * (1.0f + 1.5e-6f);
+  if (w2 - e_w2 > rhi) {
+    out.decided = 0;
+    out.within = 0;
+    out.d2 = w2;
+    return out;
+  }
+  if (!(w2 + e_w2 < rlo)) {
+    out.decided = -1;
+    out.within = 0;
+    out.d2 = 0.0f;
+    return out;
+  }
+  out.within = 1;
+  const float h1x = P1.w * D1.x, h1y = P1.w * D1.y, h1z = P1.w * D1.z;
+  const float h2x = P2.w * D2.x, h2y = P2.w * D2.y, h2z = P2.w * D2.z;
+  const float a = __fmaf_rn(h1z, h1z, __fmaf_rn(h1y, h1y, h1x * h1x));
+  const float e = __fmaf_rn(h2z, h2z, __fmaf_rn(h2y, h2y, h2x * h2x));
+  const float b = __fmaf_rn(h1z, h2z, __fmaf_rn(h1y, h2y, h1x * h2x));
+  const float c = __fmaf_rn(h1z, wz, __fmaf_rn(h1y, wy, h1x * wx));
+  const float f = __fmaf_rn(h2z, wz, __fmaf_rn(h2y, wy, h2x * wx));
+  const float denom = __fmaf_rn(a, e, -(b * b));
+  float s = 0.0f;
+  if (denom > 0.0f) {
+    s = __fmaf_rn(b, f, -(c * e)) / denom;
+    s = fminf(fmaxf(s, -1.0f), 1.0f);
+  }
+  float t = __fmaf_rn(b, s, f) / e;
+  if (t < -1.0f) {
+    t = -1.0f;
+    s = fminf(fmaxf((-b - c) / a, -1.0f), 1.0f);
+  } else if (t > 1.0f) {
+    t = 1.0f;
+    s = fminf(fmaxf((b - c) / a, -1.0f), 1.0f);
+  }
+  const float vx = __fmaf_rn(-t, h2x, __fmaf_rn(s, h1x, wx));
+  const float vy = __fmaf_rn(-t, h2y, __fmaf_rn(s, h1y, wy));
+  const float vz = __fmaf_rn(-t, h2z, __fmaf_rn(s, h1z, wz));
+  const float d2 = __fmaf_rn(vz, vz, __fmaf_rn(vy, vy, vx * vx));
+  out.d2 = d2;
+  // error bounds
+  const float nh1 = P1.w * 1.000001f, nh2 = P2.w * 1.000001f;
+  const float S = nw + nh1 + nh2;
+  const float Ev = ew2 + U32 * (nh1 + nh2) + 4.1f * U32 * S;
+  const float nv = sqrtf(d2) * 1.000001f + Ev;
+  const float EF = 4.0f * U32 * d2 + (2.0f * nv + Ev) * Ev;
+  const float gs = __fmaf_rn(h1z, vz, __fmaf_rn(h1y, vy, h1x * vx));
+  const float gt = -__fmaf_rn(h2z, vz, __fmaf_rn(h2y, vy, h2x * vx));
+  const float Egs = 4.1f * U32 * nh1 * nv + 1.01f * nh1 * Ev;
+  const float Egt = 4.1f * U32 * nh2 * nv + 1.01f * nh2 * Ev;
+  const float ags = fabsf(gs), agt = fabsf(gt);
+  const float gap = 2.0f * (__fmaf_rn(gs, s, ags) + __fmaf_rn(gt, t, agt));
+  const float Egap = 4.0f * (Egs + Egt) + 8.0f * U32 * (ags + agt) * 2.0f;
+  const float e64 = EPS64_REL * S * S;
+  const float band_lo = (EF + gap + Egap + e64) * 1.0001f + 4.0f * U32 * d2;
+  const float band_hi = (EF + e64) * 1.0001f + 4.0f * U32 * d2;
+  const float R2 = R * R;
+  const float R2lo = R2 * (1.0f - 16.0f * U32), R2hi = R2 * (1.0f + 16.0f * U32);
+  const float lo = d2 - band_lo, hi = d2 + band_hi;
+  if (hi < R2lo) {
+    out.decided = 1;
+  } else if (strict ? (lo >= R2hi) : (lo > R2hi)) {
+    out.decided = 0;
+  } else {
+    out.decided = -1;
+  }
+  return out;
+}
+
+// --------------------------------------------------------------------------
+// Candidate sampler.  The job of cuda_CNTgenerate_orient
+// (inc/cuda_CNTgenerate.h:136-260): uniform position, von Neumann accept
+// against the density target (:227-232), orientation from the target law
+// (genCosine/inc_rotation :262-295, here Marsaglia 1972 for isotropic and
+// Wood 1994 for vMF), length from the length law (reference: constant
+// DEF_L, :246).  Returns 1 proposal, 0 density reject, -1 sampler exhausted,
+// -2 overlaps its own periodic image.
+// --------------------------------------------------------------------------
+NF_HD bool self_image_overlap(const SampleDesc& d, const float4 P, const float4 D) {
+  const double R = d.soft_frac * ((double)D.w + (double)D.w) + d.sep;
+  const double rr = ((double)P.w + (double)P.w) * REACH_PAD + R;
+  double lmin = d.box[0];
+  if (d.box[1] < lmin) lmin = d.box[1];
+  if (d.box[2] < lmin) lmin = d.box[2];
+  if (rr < lmin) return false;
+  int K[3];
+  for (int a = 0; a < 3; ++a) K[a] = (int)floor(rr / d.box[a]) + 1;
+  for (int nz = -K[2]; nz <= K[2]; ++nz)
+    for (int ny = -K[1]; ny <= K[1]; ++ny)
+      for (int nx = -K[0]; nx <= K[0]; ++nx) {
+        if (nx == 0 && ny == 0 && nz == 0) continue;
+        bool within;
+        const double d2 = pair_d2_f64(P, D, P, D, (double)nx * d.box[0], (double)ny * d.box[1],
+                                      (double)nz * d.box[2], R, &within);
+        if (within && d2 < R * R) return true;
+      }
+  return false;
+}
+
+NF_HD int sample_candidate(const SampleDesc& d, uint32_t round, uint32_t slot, Rod* out) {
+  const double Lx = d.box[0], Ly = d.box[1], Lz = d.box[2];
+  u32x4 o = philox(slot, round, 0u, 0u, d.seed, d.stream);
+  const double ux = u01(o.x, o.y), uy = u01(o.z, o.w);
+  o = philox(slot, round, 1u, 0u, d.seed, d.stream);
+  const double uz = u01(o.x, o.y), ud = u01(o.z, o.w);
+  float xf = (float)(ux * Lx), yf = (float)(uy * Ly), zf = (float)(uz * Lz);
+  if (!(xf < (float)Lx)) xf = 0.0f;
+  if (!(yf < (float)Ly)) yf = 0.0f;
+  if (!(zf < (float)Lz)) zf = 0.0f;
+
+  if (d.den_law == NF_DEN_AGGLOM) {
+    const double x = (double)xf, y = (double)yf, z = (double)zf;
+    double d2min = nf_bits_to_double(0x7FF0000000000000ull);
+    for (int k = 0; k < d.n_agg; ++k) {
+      double dx = x - d.agg[3 * k + 0], dy = y - d.agg[3 * k + 1], dz = z - d.agg[3 * k + 2];
+      if (dx > 0.5 * Lx) dx -= Lx; else if (dx < -0.5 * Lx) dx += Lx;
+      if (dy > 0.5 * Ly) dy -= Ly; else if (dy < -0.5 * Ly) dy += Ly;
+      if (dz > 0.5 * Lz) dz -= Lz; else if (dz < -0.5 * Lz) dz += Lz;
+      const double dd = (dx * dx + dy * dy) + dz * dz;
+      if (dd < d2min) d2min = dd;
+    }
+    const double inv2s2 = 1.0 / (2.0 * d.agg_sigma * d.agg_sigma);
+    const double rho = 1.0 + (d.agg_peak - 1.0) * dexp(-(d2min * inv2s2));
+    if (!(ud * d.agg_peak < rho)) return 0;
+  }
+
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  bool ok = false;
+  if (d.ort_law == NF_ORT_VMF) {
+    o = philox(slot, round, 8u, 0u, d.seed, d.stream);
+    const double uw = u01(o.x, o.y);
+    const double em2k = dexp(-2.0 * d.kappa);
+    double w = 1.0 + dlog(uw + (1.0 - uw) * em2k) / d.kappa;
+    if (w > 1.0) w = 1.0;
+    if (w < -1.0) w = -1.0;
+    for (uint32_t t = 0; t < 32 && !ok; ++t) {
+      o = philox(slot, round, 9u + t, 0u, d.seed, d.stream);
+      const double v1 = 2.0 * u01(o.x, o.y) - 1.0, v2 = 2.0 * u01(o.z, o.w) - 1.0;
+      const double s = v1 * v1 + v2 * v2;
+      if (s < 1.0 && s > 0.0) {
+        const double rr = sqrt(1.0 - w * w) / sqrt(s);
+        const double a1 = v1 * rr, a2 = v2 * rr;
+        cx = (w * d.axis[0] + a1 * d.e1[0]) + a2 * d.e2[0];
+        cy = (w * d.axis[1] + a1 * d.e1[1]) + a2 * d.e2[1];
+        cz = (w * d.axis[2] + a1 * d.e1[2]) + a2 * d.e2[2];
+        ok = true;
+      }
+    }
+  } else {
+    for (uint32_t t = 0; t < 32 && !ok; ++t) {
+      o = philox(slot, round, 8u + t, 0u, d.seed, d.stream);
+      const double v1 = 2.0 * u01(o.x, o.y) - 1.0, v2 = 2.0 * u01(o.z, o.w) - 1.0;
+      const double s = v1 * v1 + v2 * v2;
+      if (s < 1.0 && s > 0.0) {
+        const double q = 2.0 * sqrt(1.0 - s);
+        cx = v1 * q;
+        cy = v2 * q;
+        cz = 1.0 - 2.0 * s;
+        ok = true;
+      }
+    }
+  }
+  if (!ok) return -1;
+
+  double L = d.len;
+  if (d.len_law == NF_LEN_LOGNORMAL) {
+    ok = false;
+    double z = 0.0;
+    for (uint32_t t = 0; t < 32 && !ok; ++t) {
+      o = philox(slot, round, 48u + t, 0u, d.seed, d.stream);
+      const double v1 = 2.0 * u01(o.x, o.y) - 1.0, v2 = 2.0 * u01(o.z, o.w) - 1.0;
+      const double s = v1 * v1 + v2 * v2;
+      if (s < 1.0 && s > 0.0) {
+        z = v1 * sqrt(-2.0 * dlog(s) / s);
+        ok = true;
+      }
+    }
+    if (!ok) return -1;
+    L = d.len * dexp(d.len_sigma * z);
+    if (L < d.len_min) L = d.len_min;
+    if (L > d.len_max) L = d.len_max;
+  }
+  const float lf = (float)L;
+  out->P = make_float4(xf, yf, zf, 0.5f * lf);
+  out->D = make_float4((float)cx, (float)cy, (float)cz, (float)d.radius);
+  if (self_image_overlap(d, out->P, out->D)) return -2;
+  return 1;
+}
+
+}  // namespace nf

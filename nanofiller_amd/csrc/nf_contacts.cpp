@@ -1,0 +1,322 @@
+// nf_contacts.cpp - host orchestration of contact detection and histograms
+// on the packing of instance 0.
+//
+// Contacts: bin -> count pass -> scan -> record pass -> canonical row sort,
+// the two-pass scheme of get_contact_counts / get_contact_arrays
+// (src/get_contact_arrays.cu:12-140) with the counts kept per rod id so the
+// record pass writes a CSR over I directly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nf_host.h"
+
+using namespace nf;
+
+namespace nf {
+
+int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned long long*) {
+  ContactBuffers& c = h->cb;
+  const nf_params& p = h->params[0];
+  const InstDesc& d = h->desc[0];
+  const uint32_t n = h->n_rods[0];
+  const float4* P = h->gb.bP + d.rod_off;
+  const float4* D = h->gb.bD + d.rod_off;
+  hipStream_t s = h->stream;
+  // release previous results
+  release_contacts(h);
+  c.n_rods = n;
+  c.gap_cut = gap;
+  // geometry: the largest half length / radius any rod of the instance can have
+  float hlmax = d.hl_max, amax = (float)p.radius;
+  if (!h->generated) {
+    // rods came from nf_set_rods: bound from the data (copied once)
+    std::vector<float4> hP(n), hD(n);
+    if (n) {
+      HIPCHK(h, hipMemcpy(hP.data(), P, sizeof(float4) * n, hipMemcpyDeviceToHost));
+      HIPCHK(h, hipMemcpy(hD.data(), D, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    }
+    hlmax = 0.0f;
+    amax = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+      hlmax = std::max(hlmax, hP[i].w);
+      amax = std::max(amax, hD[i].w);
+    }
+  }
+  const double Rmax = ((double)amax + (double)amax) + gap;
+  const double reach = ((double)hlmax + (double)hlmax) * REACH_PAD + Rmax;
+  uint64_t ncells = 1;
+  double scale[3];
+  for (int a = 0; a < 3; ++a) {
+    double nn = std::floor(p.box[a] / reach);
+    if (nn < 1.0) nn = 1.0;
+    if (nn > 1024.0) nn = 1024.0;
+    c.ncell[a] = (int32_t)nn;
+    scale[a] = (double)c.ncell[a] / p.box[a];
+    ncells *= (uint64_t)c.ncell[a];
+  }
+  if (ncells > (1ull << 26)) return fail(h, NF_E_INVALID, "contact grid too large");
+  c.ncells = (uint32_t)ncells;
+  std::vector<int4> st = build_stencil(c.ncell, p.box, reach, true);
+  c.st_n = (uint32_t)st.size();
+  int rc = NF_OK;
+  unsigned long long* ctr = nullptr;
+  uint32_t* tmp = nullptr;
+  uint32_t* queues = nullptr;
+#define ALLOC(ptr, cnt) \
+  if (rc == NF_OK) rc = dalloc(h, &(ptr), (cnt));
+  ALLOC(c.cP, n);
+  ALLOC(c.cD, n);
+  ALLOC(c.cid, n);
+  ALLOC(c.cstart, ncells + 1);
+  ALLOC(c.ccount, ncells);
+  ALLOC(c.chl, ncells);
+  ALLOC(c.ccur, ncells);
+  ALLOC(c.stencil, st.size());
+  ALLOC(c.cnt, (uint64_t)n + 1);
+  ALLOC(c.rowptr, (uint64_t)n + 1);
+  ALLOC(c.fill, n);
+  ALLOC(ctr, 4);
+  ALLOC(tmp, std::max(scan_tmp_words(ncells), scan_tmp_words((uint64_t)n + 1)));
+  ALLOC(queues, 2 * QUEUES_PER_KERNEL);
+#undef ALLOC
+  if (rc != NF_OK) {
+    dfree(h, ctr);
+    dfree(h, tmp);
+    dfree(h, queues);
+    return rc;
+  }
+  HIPCHK(h, hipMemcpyAsync(c.stencil, st.data(), sizeof(int4) * st.size(), hipMemcpyHostToDevice, s));
+  hipEvent_t* ev = h->ev;
+  hipEventRecord(ev[0], s);
+  // bin
+  fill_u32(c.ccount, 0u, ncells, s);
+  fill_u32((uint32_t*)c.chl, 0u, ncells, s);
+  fill_u32(c.ccur, 0u, ncells, s);
+  fill_u32((uint32_t*)ctr, 0u, 8, s);
+  fill_u32(queues, 0u, 2 * QUEUES_PER_KERNEL, s);
+  launch_cell_count(P, n, c.ncell, scale, c.ccount, c.chl, s);
+  scan_u32(c.ccount, c.cstart, ncells, tmp, s);
+  launch_cell_scatter(P, D, n, c.ncell, scale, c.cstart, c.ccur, c.cP, c.cD, c.cid, s);
+  hipEventRecord(ev[1], s);
+  // count
+  PairArgs a{};
+  a.desc = nullptr;
+  a.st = nullptr;
+  a.cell_inst = nullptr;
+  a.ncells_total = c.ncells;
+  a.stencil = c.stencil;
+  a.st_off = 0;
+  a.st_n = c.st_n;
+  a.prP = c.cP;
+  a.prD = c.cD;
+  a.prX = c.cid;
+  a.prStart = c.cstart;
+  a.prCount = c.ccount;
+  a.nbP = c.cP;
+  a.nbD = c.cD;
+  a.nbX = c.cid;
+  a.nbStart = c.cstart;
+  a.nbCount = c.ccount;
+  a.nbHl = c.chl;
+  a.qctr = queues;
+  a.cnt = c.cnt;
+  for (int k = 0; k < 3; ++k) {
+    a.box[k] = p.box[k];
+    a.boxf[k] = (float)p.box[k];
+    a.ncell[k] = c.ncell[k];
+  }
+  a.gap64 = gap;
+  a.gap32 = (float)gap;
+  const double Lmax = std::max(p.box[0], std::max(p.box[1], p.box[2]));
+  a.ew = (float)(5.9604644775390625e-8 * (Lmax + reach) * 1.01);
+  a.hl_max = hlmax;
+  a.R_max = (float)Rmax * 1.000001f;
+  a.ctr = ctr;
+  a.diag = h->opt_diag;
+  fill_u32(c.cnt, 0u, (uint64_t)n + 1, s);
+  launch_pair(PM_CNT_COUNT, prec, a, s);
+  hipEventRecord(ev[2], s);
+  scan_u32(c.cnt, c.rowptr, n, tmp, s);
+  uint32_t total = 0;
+  HIPCHK(h, hipMemcpyAsync(&total, c.rowptr + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  HIPCHK(h, hipGetLastError());
+  c.n_contacts = total;
+  c.cap_contacts = total;
+  rc = NF_OK;
+  if (rc == NF_OK) rc = dalloc(h, &c.J, total);
+  if (rc == NF_OK) rc = dalloc(h, &c.gap, total);
+  if (rc == NF_OK) rc = dalloc(h, &c.gap64, total);
+  if (rc == NF_OK) rc = dalloc(h, &c.img, total);
+  if (rc != NF_OK) {
+    dfree(h, ctr);
+    dfree(h, tmp);
+    dfree(h, queues);
+    return rc;
+  }
+  hipEventRecord(ev[3], s);
+  fill_u32(c.fill, 0u, n, s);
+  fill_u32(queues + QUEUES_PER_KERNEL, 0u, QUEUES_PER_KERNEL, s);
+  fill_u32((uint32_t*)ctr, 0u, 4, s);  // the record pass repeats the count's tests
+  a.qctr = queues + QUEUES_PER_KERNEL;
+  a.rowptr = c.rowptr;
+  a.fill = c.fill;
+  a.outJ = c.J;
+  a.outGap = c.gap;
+  a.outGap64 = c.gap64;
+  a.outImg = c.img;
+  launch_pair(PM_CNT_RECORD, prec, a, s);
+  hipEventRecord(ev[4], s);
+  launch_row_sort(c.rowptr, n, c.J, c.gap, c.gap64, c.img, s);
+  hipEventRecord(ev[5], s);
+  HIPCHK(h, hipMemcpyAsync(h->cb_stats, ctr, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  HIPCHK(h, hipGetLastError());
+  float ms = 0.0f;
+  const int from[4] = {0, 1, 3, 4}, to[4] = {1, 2, 4, 5};
+  for (int k = 0; k < 4; ++k) {
+    (void)hipEventElapsedTime(&ms, ev[from[k]], ev[to[k]]);
+    h->cb_ms[k] = ms;
+  }
+  // verify the record pass filled every counted slot (reference warns the
+  // same way, src/get_contact_arrays.cu:132-134)
+  std::vector<uint32_t> hc(n), hf(n);
+  if (n) {
+    HIPCHK(h, hipMemcpy(hc.data(), c.cnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(hf.data(), c.fill, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  }
+  dfree(h, ctr);
+  dfree(h, tmp);
+  dfree(h, queues);
+  for (uint32_t i = 0; i < n; ++i)
+    if (hc[i] != hf[i]) return fail(h, NF_E_STATE, "contact record pass disagrees with count pass");
+  c.valid = true;
+  if (n_out) *n_out = total;
+  return NF_OK;
+}
+
+}  // namespace nf
+
+extern "C" int nf_contacts(nf_handle* h, double gap, uint64_t* n_contacts) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  h->err.clear();
+  if (!(gap >= 0.0)) return fail(h, NF_E_INVALID, "gap must be >= 0");
+  if (h->params.size() != 1) return fail(h, NF_E_INVALID, "contacts run on single-instance handles");
+  const int prec = h->opt_contact_prec >= 0 ? h->opt_contact_prec : h->params[0].precision;
+  return contacts_run(h, gap, n_contacts, prec, nullptr);
+}
+
+extern "C" int nf_get_contacts(nf_handle* h, uint32_t* row_ptr, uint32_t* col, float* gap,
+                               int32_t* image, uint64_t cap) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  ContactBuffers& c = h->cb;
+  if (!c.valid) return fail(h, NF_E_STATE, "no contacts computed");
+  if ((col || gap || image) && cap < c.n_contacts) return fail(h, NF_E_CAPACITY, "contact buffer too small");
+  if (row_ptr)
+    HIPCHK(h, hipMemcpy(row_ptr, c.rowptr, sizeof(uint32_t) * ((size_t)c.n_rods + 1), hipMemcpyDeviceToHost));
+  if (c.n_contacts) {
+    if (col) HIPCHK(h, hipMemcpy(col, c.J, sizeof(uint32_t) * c.n_contacts, hipMemcpyDeviceToHost));
+    if (gap) HIPCHK(h, hipMemcpy(gap, c.gap, sizeof(float) * c.n_contacts, hipMemcpyDeviceToHost));
+    if (image) HIPCHK(h, hipMemcpy(image, c.img, sizeof(int32_t) * c.n_contacts, hipMemcpyDeviceToHost));
+  }
+  return NF_OK;
+}
+
+extern "C" int nf_get_contact_gap64(nf_handle* h, double* gap64, uint64_t cap) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  ContactBuffers& c = h->cb;
+  if (!c.valid) return fail(h, NF_E_STATE, "no contacts computed");
+  if (cap < c.n_contacts) return fail(h, NF_E_CAPACITY, "buffer too small");
+  if (c.n_contacts && gap64)
+    HIPCHK(h, hipMemcpy(gap64, c.gap64, sizeof(double) * c.n_contacts, hipMemcpyDeviceToHost));
+  return NF_OK;
+}
+
+extern "C" int nf_contacts_stats(nf_handle* h, uint64_t* pair_tests, uint64_t* rechecks,
+                                 double* max_dd, float* kernel_ms) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  if (!h->cb.valid) return fail(h, NF_E_STATE, "no contacts computed");
+  if (pair_tests) *pair_tests = h->cb_stats[0];
+  if (rechecks) *rechecks = h->cb_stats[1];
+  if (max_dd) {
+    const unsigned long long b = h->cb_stats[2];
+    double v;
+    std::memcpy(&v, &b, sizeof(v));
+    *max_dd = v;
+  }
+  if (kernel_ms)
+    for (int k = 0; k < 4; ++k) kernel_ms[k] = h->cb_ms[k];
+  return NF_OK;
+}
+
+// ===========================================================================
+// histograms
+// ===========================================================================
+extern "C" int nf_histograms(nf_handle* h, const nf_hist_spec* spec, int64_t* out_centre,
+                             int64_t* out_polar, int64_t* out_mutual, int64_t* out_phitheta) {
+  if (!h || !spec) return fail(h, NF_E_INVALID, "null argument");
+  const int nb = spec->bins, npb = spec->phi_bins, ntb = spec->theta_bins;
+  if (nb < 1 || nb > 256) return fail(h, NF_E_INVALID, "bins must be 1..256");
+  if (spec->centre_axis < 0 || spec->centre_axis > 2) return fail(h, NF_E_INVALID, "centre_axis must be 0..2");
+  const bool two_d = npb > 0 && out_phitheta;
+  if (two_d && (npb % 2 != 0 || ntb < 1 || npb * ntb > 4096))
+    return fail(h, NF_E_INVALID, "phi-theta needs even phi_bins and phi*theta <= 4096");
+  if (out_mutual && !h->cb.valid) return fail(h, NF_E_STATE, "mutual-angle histogram needs nf_contacts first");
+  const double PI = 3.14159265358979323846;
+  std::vector<double> ep(nb + 1), em(nb + 1), et;
+  for (int k = 0; k <= nb; ++k) {
+    ep[k] = std::cos((double)k * PI / (double)nb);
+    em[k] = std::cos((double)k * (0.5 * PI) / (double)nb);
+  }
+  if (two_d) {
+    const int half = npb / 2;
+    et.resize((size_t)ntb + 1 + half + 1);
+    for (int k = 0; k <= ntb; ++k) et[k] = std::cos((double)k * PI / (double)ntb);
+    for (int k = 0; k <= half; ++k) et[ntb + 1 + k] = std::cos((double)k * PI / (double)half);
+  }
+  double mu[3] = {spec->pol_axis[0], spec->pol_axis[1], spec->pol_axis[2]};
+  const double mn = std::sqrt((mu[0] * mu[0] + mu[1] * mu[1]) + mu[2] * mu[2]);
+  if (!(mn > 0.0)) return fail(h, NF_E_INVALID, "pol_axis must be non-zero");
+  for (int k = 0; k < 3; ++k) mu[k] = mu[k] / mn;
+  const size_t nout = 3 * (size_t)nb + (two_d ? (size_t)npb * ntb : 0);
+  double* dE = nullptr;
+  unsigned long long* dOut = nullptr;
+  int rc = dalloc(h, &dE, 2 * (size_t)(nb + 1) + et.size());
+  if (rc == NF_OK) rc = dalloc(h, &dOut, nout);
+  if (rc != NF_OK) {
+    dfree(h, dE);
+    return rc;
+  }
+  hipStream_t s = h->stream;
+  std::vector<double> all(ep);
+  all.insert(all.end(), em.begin(), em.end());
+  all.insert(all.end(), et.begin(), et.end());
+  hipError_t e = hipMemcpyAsync(dE, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(dOut, 0, sizeof(unsigned long long) * nout, s);
+  const InstDesc& d = h->desc[0];
+  const uint32_t n = h->n_rods[0];
+  if (e == hipSuccess && n) {
+    launch_hist(h->gb.bP + d.rod_off, h->gb.bD + d.rod_off, n,
+                (out_mutual && h->cb.valid) ? h->cb.rowptr : nullptr, h->cb.J, dE, dE + (nb + 1),
+                two_d ? dE + 2 * (nb + 1) : nullptr, nb, two_d ? npb : 0, two_d ? ntb : 0,
+                spec->centre_axis, h->params[0].box[spec->centre_axis], mu, dOut, s);
+    e = hipGetLastError();
+  }
+  std::vector<int64_t> hout(nout);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hout.data(), dOut, sizeof(int64_t) * nout, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(h, dE);
+  dfree(h, dOut);
+  if (e != hipSuccess) return fail(h, NF_E_HIP, std::string("histograms: ") + hipGetErrorString(e));
+  if (out_centre) std::memcpy(out_centre, hout.data(), sizeof(int64_t) * nb);
+  if (out_polar) std::memcpy(out_polar, hout.data() + nb, sizeof(int64_t) * nb);
+  if (out_mutual) std::memcpy(out_mutual, hout.data() + 2 * nb, sizeof(int64_t) * nb);
+  if (two_d) std::memcpy(out_phitheta, hout.data() + 3 * nb, sizeof(int64_t) * npb * ntb);
+  return NF_OK;
+}

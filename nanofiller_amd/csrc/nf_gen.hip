@@ -1,0 +1,338 @@
+// nf_gen.hip - one insert/reject round for a batch of packings:
+//   ctl_begin -> gen -> bin (scan + scatter) -> reject_old -> reject_new
+//   -> rank (scan) -> commit -> ctl_end.
+// Every launch has a fixed shape and reads its work size from device
+// memory, so a round is captured once in a HIP graph and replayed with no
+// host round trip.  Replaces the host-driven round of cudaGenStep
+// (src/cudaMain.cu:219-351) and its per-round device-to-host sync
+// (checkArray, src/cudaMain.cu:201-216).
+#include "nf_internal.h"
+
+namespace nf {
+
+constexpr int CTL_THREADS = 1024;
+constexpr int GEN_THREADS = 256;
+
+// ---------------------------------------------------------------------------
+// Round schedule: candidate slots per round as an integer function of the
+// round history (same formula as the oracle).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t clamp_batch(uint64_t m, const InstDesc& d) {
+  if (m < d.min_batch) m = d.min_batch;
+  if (m > d.max_batch) m = d.max_batch;
+  return (uint32_t)m;
+}
+
+__device__ __forceinline__ uint32_t schedule(const InstDesc& d, const InstState& s) {
+  const uint32_t deficit = d.n_target - s.n_acc;
+  if (s.round == 0) return clamp_batch((uint64_t)deficit + (uint64_t)(deficit / 4u), d);
+  if (s.aprev == 0) return d.max_batch;
+  const uint64_t num = (uint64_t)deficit * (uint64_t)s.mprev * 5ull;
+  const uint64_t den = (uint64_t)s.aprev * 4ull;
+  return clamp_batch((num + den - 1) / den, d);
+}
+
+__device__ uint32_t ctl_block_scan(uint32_t v, uint32_t* sm, uint32_t* total) {
+  // 1024 threads = 16 waves
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  uint32_t inc = v;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o, WAVE);
+    if (lane >= o) inc += t;
+  }
+  if (lane == WAVE - 1) sm[wv] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int w = 0; w < CTL_THREADS / WAVE; ++w) {
+    const uint32_t s = sm[w];
+    if (w < wv) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+// Plans the round: M per running instance, slot prefix, queue reset.
+__global__ void __launch_bounds__(CTL_THREADS) k_ctl_begin(const InstDesc* __restrict__ desc,
+                                                           InstState* __restrict__ st,
+                                                           RoundInfo* __restrict__ ri,
+                                                           uint32_t* __restrict__ slot_base,
+                                                           uint32_t* __restrict__ queues,
+                                                           int n_inst) {
+  __shared__ uint32_t sm[CTL_THREADS / WAVE];
+  uint32_t carry = 0;
+  for (int b = 0; b < n_inst; b += CTL_THREADS) {
+    const int i = b + threadIdx.x;
+    uint32_t M = 0;
+    if (i < n_inst && st[i].status == NF_STATUS_RUNNING) M = schedule(desc[i], st[i]);
+    if (i < n_inst) st[i].M = M;
+    uint32_t tot;
+    const uint32_t ex = ctl_block_scan(M, sm, &tot);
+    if (i < n_inst) slot_base[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    slot_base[n_inst] = carry;
+    ri->T = carry;
+  }
+  if (threadIdx.x < Q_N * QUEUES_PER_KERNEL) queues[threadIdx.x] = 0;
+}
+
+__device__ __forceinline__ int find_inst(const uint32_t* __restrict__ base, int n, uint32_t g) {
+  int lo = 0, hi = n;  // base[lo] <= g < base[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (base[mid] <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t cell_of(const InstDesc& d, const float4 P) {
+  int c[3];
+  const float p[3] = {P.x, P.y, P.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int i = (int)((double)p[a] * d.cell_scale[a]);
+    i = i < 0 ? 0 : (i >= d.ncell[a] ? d.ncell[a] - 1 : i);
+    c[a] = i;
+  }
+  return (uint32_t)(c[0] + d.ncell[0] * (c[1] + d.ncell[1] * c[2]));
+}
+
+// Draws the candidates of every slot [0, T); bins proposals by cell.
+__global__ void __launch_bounds__(GEN_THREADS) k_gen(const InstDesc* __restrict__ desc,
+                                                     const InstState* __restrict__ st,
+                                                     const RoundInfo* __restrict__ ri,
+                                                     const uint32_t* __restrict__ slot_base,
+                                                     int n_inst, uint64_t T_cap,
+                                                     float4* __restrict__ sP, float4* __restrict__ sD,
+                                                     int32_t* __restrict__ scell,
+                                                     uint32_t* __restrict__ srk,
+                                                     uint32_t* __restrict__ surv,
+                                                     uint32_t* __restrict__ pcount,
+                                                     float* __restrict__ phl) {
+  const uint32_t T = ri->T;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g <= T_cap;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    surv[g] = 0;
+    if (g >= T) continue;
+    const int inst = find_inst(slot_base, n_inst, (uint32_t)g);
+    const InstDesc& d = desc[inst];
+    const uint32_t slot = (uint32_t)g - slot_base[inst];
+    Rod r;
+    const int v = sample_candidate(d.smp, st[inst].round, slot, &r);
+    if (v == 1) {
+      const uint32_t gc = d.cell_base + cell_of(d, r.P);
+      srk[g] = atomicAdd(&pcount[gc], 1u);
+      atomicMax((unsigned int*)&phl[gc], __float_as_uint(r.P.w));
+      sP[g] = r.P;
+      sD[g] = r.D;
+      scell[g] = (int32_t)gc;
+    } else {
+      scell[g] = -1;
+    }
+  }
+}
+
+// Moves proposals into their cell ranges.
+__global__ void __launch_bounds__(GEN_THREADS) k_scatter(const RoundInfo* __restrict__ ri,
+                                                         const float4* __restrict__ sP,
+                                                         const float4* __restrict__ sD,
+                                                         const int32_t* __restrict__ scell,
+                                                         const uint32_t* __restrict__ srk,
+                                                         const uint32_t* __restrict__ pstart,
+                                                         float4* __restrict__ pP,
+                                                         float4* __restrict__ pD,
+                                                         uint32_t* __restrict__ pslot,
+                                                         uint32_t* __restrict__ palive) {
+  const uint32_t T = ri->T;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < T; g += gridDim.x * blockDim.x) {
+    const int32_t c = scell[g];
+    if (c < 0) continue;
+    const uint32_t dst = pstart[c] + srk[g];
+    pP[dst] = sP[g];
+    pD[dst] = sD[g];
+    pslot[dst] = g;
+    palive[dst] = 1u;
+  }
+}
+
+// Accepts survivors in slot order up to the target; inserts them by id and
+// into their cell slots.
+__global__ void __launch_bounds__(GEN_THREADS) k_commit(const InstDesc* __restrict__ desc,
+                                                        InstState* __restrict__ st,
+                                                        const RoundInfo* __restrict__ ri,
+                                                        const uint32_t* __restrict__ slot_base,
+                                                        int n_inst,
+                                                        const float4* __restrict__ sP,
+                                                        const float4* __restrict__ sD,
+                                                        const int32_t* __restrict__ scell,
+                                                        const uint32_t* __restrict__ surv,
+                                                        const uint32_t* __restrict__ srank,
+                                                        float4* __restrict__ bP,
+                                                        float4* __restrict__ bD,
+                                                        const uint32_t* __restrict__ ostart,
+                                                        uint32_t* __restrict__ ocount,
+                                                        float* __restrict__ ohl,
+                                                        float4* __restrict__ oP,
+                                                        float4* __restrict__ oD,
+                                                        uint32_t* __restrict__ oid) {
+  const uint32_t T = ri->T;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < T; g += gridDim.x * blockDim.x) {
+    if (!surv[g]) continue;
+    const int inst = find_inst(slot_base, n_inst, g);
+    const InstDesc& d = desc[inst];
+    const uint32_t n_acc = st[inst].n_acc;
+    const uint32_t r = srank[g] - srank[slot_base[inst]];
+    if (r >= d.n_target - n_acc) continue;
+    const uint32_t id = n_acc + r;
+    const float4 P = sP[g], D = sD[g];
+    bP[d.rod_off + id] = P;
+    bD[d.rod_off + id] = D;
+    const int32_t c = scell[g];
+    const uint32_t k = atomicAdd(&ocount[c], 1u);
+    if (k < d.cap) {
+      const uint32_t dst = ostart[c] + k;
+      oP[dst] = P;
+      oD[dst] = D;
+      oid[dst] = id;
+      atomicMax((unsigned int*)&ohl[c], __float_as_uint(P.w));
+    } else {
+      atomicOr(&st[inst].overflow, 1u);
+    }
+  }
+}
+
+// Closes the round: accepted counts, statistics, termination.
+__global__ void __launch_bounds__(CTL_THREADS) k_ctl_end(const InstDesc* __restrict__ desc,
+                                                         InstState* __restrict__ st,
+                                                         RoundInfo* __restrict__ ri,
+                                                         const uint32_t* __restrict__ slot_base,
+                                                         const uint32_t* __restrict__ srank,
+                                                         const uint32_t* __restrict__ pstart,
+                                                         int n_inst) {
+  __shared__ uint32_t running;
+  if (threadIdx.x == 0) running = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_inst; i += CTL_THREADS) {
+    InstState s = st[i];
+    if (s.status != NF_STATUS_RUNNING) continue;
+    const InstDesc& d = desc[i];
+    const uint32_t sb = slot_base[i];
+    const uint32_t surv = srank[sb + s.M] - srank[sb];
+    const uint32_t deficit = d.n_target - s.n_acc;
+    const uint32_t acc = surv < deficit ? surv : deficit;
+    const uint32_t P = pstart[d.cell_base + d.ncells] - pstart[d.cell_base];
+    s.candidates += s.M;
+    s.proposals += P;
+    s.n_acc += acc;
+    s.round += 1;
+    s.mprev = s.M;
+    s.aprev = acc;
+    s.stall = acc == 0 ? s.stall + 1 : 0;
+    if (s.overflow)
+      s.status = NF_STATUS_OVERFLOW;
+    else if (s.n_acc == d.n_target)
+      s.status = NF_STATUS_TARGET;
+    else if (s.stall >= d.stall_rounds)
+      s.status = NF_STATUS_STALLED;
+    else if (s.round >= d.max_rounds)
+      s.status = NF_STATUS_MAX_ROUNDS;
+    st[i].candidates = s.candidates;
+    st[i].proposals = s.proposals;
+    st[i].n_acc = s.n_acc;
+    st[i].round = s.round;
+    st[i].mprev = s.mprev;
+    st[i].aprev = s.aprev;
+    st[i].stall = s.stall;
+    st[i].status = s.status;
+    if (s.status == NF_STATUS_RUNNING) atomicAdd(&running, 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ri->running = running;
+    ri->rounds += 1;
+  }
+}
+
+static int gen_grid(uint64_t n) {
+  uint64_t b = (n + GEN_THREADS - 1) / GEN_THREADS;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// One round.  ev (optional, 9 events) brackets the kernel classes.
+void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* ev) {
+  const int n = b.n_inst;
+  if (ev) (void)hipEventRecord(ev[0], s);
+  hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(CTL_THREADS), 0, s, b.desc, b.st, b.ri,
+                     b.slot_base, b.queues, n);
+  fill_u32(b.pcount, 0u, b.total_cells, s);
+  fill_u32((uint32_t*)b.phl, 0u, b.total_cells, s);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  hipLaunchKernelGGL(k_gen, dim3(gen_grid(b.T_cap + 1)), dim3(GEN_THREADS), 0, s, b.desc, b.st,
+                     b.ri, b.slot_base, n, b.T_cap, b.sP, b.sD, b.scell, b.srk, b.surv, b.pcount,
+                     b.phl);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  scan_u32(b.pcount, b.pstart, b.total_cells, b.scan_tmp, s);
+  hipLaunchKernelGGL(k_scatter, dim3(gen_grid(b.T_cap)), dim3(GEN_THREADS), 0, s, b.ri, b.sP,
+                     b.sD, b.scell, b.srk, b.pstart, b.pP, b.pD, b.pslot, b.palive);
+  if (ev) (void)hipEventRecord(ev[3], s);
+
+  PairArgs a{};
+  a.desc = b.desc;
+  a.st = b.st;
+  a.cell_inst = b.cell_inst;
+  a.ncells_total = (uint32_t)b.total_cells;
+  a.stencil = b.stencil;
+  a.diag = diag;
+  // stage 1: proposals vs accepted rods
+  a.prP = b.pP;
+  a.prD = b.pD;
+  a.prX = b.pslot;
+  a.prStart = b.pstart;
+  a.prCount = b.pcount;
+  a.prAlive = nullptr;
+  a.nbP = b.oP;
+  a.nbD = b.oD;
+  a.nbX = b.oid;
+  a.nbStart = b.ostart;
+  a.nbCount = b.ocount;
+  a.nbHl = b.ohl;
+  a.nbAlive = nullptr;
+  a.qctr = b.queues + Q_OLD * QUEUES_PER_KERNEL;
+  a.alive_out = b.palive;
+  launch_pair(PM_GEN_OLD, prec, a, s);
+  if (ev) (void)hipEventRecord(ev[4], s);
+  // stage 2: survivors vs higher-priority survivors
+  a.prAlive = b.palive;
+  a.nbP = b.pP;
+  a.nbD = b.pD;
+  a.nbX = b.pslot;
+  a.nbStart = b.pstart;
+  a.nbCount = b.pcount;
+  a.nbHl = b.phl;
+  a.nbAlive = b.palive;
+  a.qctr = b.queues + Q_NEW * QUEUES_PER_KERNEL;
+  a.alive_out = nullptr;
+  a.surv = b.surv;
+  launch_pair(PM_GEN_NEW, prec, a, s);
+  if (ev) (void)hipEventRecord(ev[5], s);
+  scan_u32(b.surv, b.srank, b.T_cap, b.scan_tmp, s);
+  hipLaunchKernelGGL(k_commit, dim3(gen_grid(b.T_cap)), dim3(GEN_THREADS), 0, s, b.desc, b.st,
+                     b.ri, b.slot_base, n, b.sP, b.sD, b.scell, b.surv, b.srank, b.bP, b.bD,
+                     b.ostart, b.ocount, b.ohl, b.oP, b.oD, b.oid);
+  if (ev) (void)hipEventRecord(ev[6], s);
+  hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(CTL_THREADS), 0, s, b.desc, b.st, b.ri,
+                     b.slot_base, b.srank, b.pstart, n);
+  if (ev) (void)hipEventRecord(ev[7], s);
+}
+
+}  // namespace nf

@@ -1,0 +1,58 @@
+// nf_host.h - host-side handle layout and helpers shared by the library's
+// C++ translation units (nf_api.cpp, nf_contacts.cpp, nf_io.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "nf_internal.h"
+
+// Options of nf_set_option() (declared in nanofiller.h).
+struct nf_handle {
+  std::vector<nf_params> params;
+  std::vector<nf::InstDesc> desc;  // host copy of the device descriptors
+  nf::GenBuffers gb;
+  nf::ContactBuffers cb;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_prec = -1, graph_diag = -1;
+  hipEvent_t ev[9] = {};
+  std::string err;
+  int opt_graph = 1, opt_profile = 0, opt_diag = 0, opt_contact_prec = -1;
+  int device = 0;
+  std::vector<uint32_t> n_rods;  // current rod count per instance
+  bool generated = false;
+  std::vector<void*> allocs;
+  unsigned long long cb_stats[4] = {0, 0, 0, 0};  // contact pair tests, rechecks, max dd bits
+  float cb_ms[4] = {0, 0, 0, 0};                  // contact kernel times: bin, count, record, sort
+  nf_network_info net_info = {};
+};
+
+extern std::string g_last_error;
+
+namespace nf {
+
+int fail(nf_handle* h, int code, const std::string& msg);
+int dalloc_bytes(nf_handle* h, void** p, size_t bytes);
+void dfree(nf_handle* h, void* p);
+void release_contacts(nf_handle* h);
+std::vector<int4> build_stencil(const int32_t nc[3], const double box[3], double reach, bool half);
+
+template <class T>
+int dalloc(nf_handle* h, T** p, size_t count) {
+  void* q = nullptr;
+  const int rc = dalloc_bytes(h, &q, count * sizeof(T));
+  *p = (T*)q;
+  return rc;
+}
+
+}  // namespace nf
+
+#define HIPCHK(h, call)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return nf::fail((h), NF_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));  \
+  } while (0)

@@ -1,0 +1,211 @@
+// nf_internal.h - device descriptors, per-instance state and launcher
+// declarations shared by the HIP translation units and the host library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "nf_common.h"
+
+namespace nf {
+
+constexpr int WAVE = 64;
+constexpr float CELL_MARGIN = 1e-4f;  // rods may sit this far outside their cell
+
+// Per-instance constant descriptor (device array, read by every kernel).
+struct InstDesc {
+  SampleDesc smp;
+  float boxf[3];
+  int32_t ncell[3];
+  double cell_scale[3];  // ncell / box, for cell assignment
+  float edge[3];         // cell edge
+  uint32_t cell_base;    // first global cell of this instance
+  uint32_t ncells;
+  uint32_t cap;          // accepted-rod capacity per cell
+  uint32_t rod_off;      // offset of this instance in the by-id rod arrays
+  uint64_t old_off;      // offset of this instance in the cell-slot arrays
+  uint32_t n_target, min_batch, max_batch, max_rounds, stall_rounds;
+  uint32_t st_off, st_n;  // generation stencil (int4 entries)
+  double sf64, sep64;     // hard core = sf*(a1+a2)+sep
+  float sf32, sep32;
+  float ew;               // FP32 bound on a centre-difference component error
+  float hl_max;           // max half length of any rod of this instance
+  float R_max;            // max hard-core distance
+  uint32_t pad;
+};
+
+// Per-instance dynamic state (device), updated by the round controller.
+struct InstState {
+  uint32_t n_acc, round, M, status;
+  uint32_t mprev, aprev, stall, overflow;
+  unsigned long long candidates, proposals, pair_tests, rechecks, rej_old, rej_new;
+  double max_dd;  // diagnostic max |d32 - d64|
+};
+
+struct RoundInfo {
+  uint32_t T;         // candidate slots this round over all instances
+  uint32_t running;   // instances still running after the last round
+  uint32_t rounds;    // rounds executed by the controller
+  uint32_t pad;
+};
+
+// Queue counters: 8 per pair kernel launch (one per XCD chunk).
+constexpr int QUEUES_PER_KERNEL = 8;
+enum { Q_OLD = 0, Q_NEW = 1, Q_CNT = 2, Q_REC = 3, Q_N = 4 };
+
+// Device buffers of a generation batch.
+struct GenBuffers {
+  InstDesc* desc = nullptr;
+  InstState* st = nullptr;
+  RoundInfo* ri = nullptr;
+  uint32_t* slot_base = nullptr;  // n_inst+1, exclusive prefix of M
+  int4* stencil = nullptr;
+  double* agg = nullptr;
+  uint32_t* cell_inst = nullptr;  // instance of every global cell
+  // accepted rods in fixed-capacity cell slots
+  float4* oP = nullptr;
+  float4* oD = nullptr;
+  uint32_t* oid = nullptr;
+  uint32_t* ostart = nullptr;  // start slot of every cell
+  uint32_t* ocount = nullptr;
+  float* ohl = nullptr;        // max half length in the cell
+  // accepted rods by id
+  float4* bP = nullptr;
+  float4* bD = nullptr;
+  // per-round candidate staging, indexed by global slot
+  float4* sP = nullptr;
+  float4* sD = nullptr;
+  int32_t* scell = nullptr;
+  uint32_t* srk = nullptr;
+  uint32_t* surv = nullptr;   // T_cap + 1
+  uint32_t* srank = nullptr;  // T_cap + 1
+  // proposals sorted by cell
+  float4* pP = nullptr;
+  float4* pD = nullptr;
+  uint32_t* pslot = nullptr;
+  uint32_t* palive = nullptr;
+  uint32_t* pcount = nullptr;  // total_cells
+  uint32_t* pstart = nullptr;  // total_cells + 1
+  float* phl = nullptr;
+  uint32_t* queues = nullptr;  // Q_N * 8
+  uint32_t* scan_tmp = nullptr;
+  uint64_t total_cells = 0, total_old = 0, T_cap = 0, total_rods = 0;
+  int n_inst = 0;
+};
+
+// Contact / network buffers of a single packing.
+struct ContactBuffers {
+  // compact cell list
+  float4* cP = nullptr;
+  float4* cD = nullptr;
+  uint32_t* cid = nullptr;
+  uint32_t* cstart = nullptr;  // ncells + 1
+  uint32_t* ccount = nullptr;
+  float* chl = nullptr;
+  uint32_t* ccur = nullptr;
+  int4* stencil = nullptr;
+  uint32_t st_n = 0;
+  int32_t ncell[3] = {0, 0, 0};
+  uint32_t ncells = 0;
+  // contacts
+  uint32_t* cnt = nullptr;     // N
+  uint32_t* rowptr = nullptr;  // N + 1
+  uint32_t* fill = nullptr;    // N
+  uint32_t* J = nullptr;
+  float* gap = nullptr;
+  double* gap64 = nullptr;
+  int32_t* img = nullptr;
+  uint64_t n_contacts = 0, cap_contacts = 0;
+  uint32_t n_rods = 0;
+  double gap_cut = 0.0;
+  bool valid = false;
+  // network
+  uint32_t* labels = nullptr;
+  uint32_t* nrow = nullptr;
+  uint32_t* ncol = nullptr;
+  double* nval = nullptr;
+  uint64_t nnz = 0;
+  bool net_valid = false;
+};
+
+// Arguments of the cell-pair engine.
+struct PairArgs {
+  const InstDesc* desc;
+  InstState* st;
+  const uint32_t* cell_inst;
+  uint32_t ncells_total;
+  const int4* stencil;
+  uint32_t st_off, st_n;  // used when desc == nullptr (contacts)
+  // probes
+  const float4* prP;
+  const float4* prD;
+  const uint32_t* prX;
+  const uint32_t* prStart;
+  const uint32_t* prCount;
+  const uint32_t* prAlive;
+  // neighbours
+  const float4* nbP;
+  const float4* nbD;
+  const uint32_t* nbX;
+  const uint32_t* nbStart;
+  const uint32_t* nbCount;
+  const float* nbHl;
+  const uint32_t* nbAlive;
+  uint32_t* qctr;
+  // outputs
+  uint32_t* alive_out;
+  uint32_t* surv;
+  uint32_t* cnt;
+  const uint32_t* rowptr;
+  uint32_t* fill;
+  uint32_t* outJ;
+  float* outGap;
+  double* outGap64;
+  int32_t* outImg;
+  // contact geometry (single instance)
+  double box[3];
+  float boxf[3];
+  int32_t ncell[3];
+  double gap64;
+  float gap32;
+  float ew;
+  float hl_max, R_max;
+  unsigned long long* ctr;  // [0] pair tests, [1] rechecks
+  int diag;                 // record max |d32-d64| over full tests
+};
+
+enum PairMode { PM_GEN_OLD = 0, PM_GEN_NEW = 1, PM_CNT_COUNT = 2, PM_CNT_RECORD = 3 };
+
+// ---- launchers (nf_scan.hip) ----
+void scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
+size_t scan_tmp_words(uint64_t n);
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+
+// ---- launchers (nf_gen.hip) ----
+void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* ev);
+void launch_ctl_init(GenBuffers& b, hipStream_t s);
+
+// ---- launchers (nf_pair.hip) ----
+void launch_pair(int mode, int prec, const PairArgs& a, hipStream_t s);
+int pair_grid();
+
+// ---- launchers (nf_post.hip) ----
+void launch_cell_count(const float4* P, uint32_t n, const int32_t ncell[3], const double scale[3],
+                       uint32_t* count, float* hl, hipStream_t s);
+void launch_cell_scatter(const float4* P, const float4* D, uint32_t n, const int32_t ncell[3],
+                         const double scale[3], const uint32_t* start, uint32_t* cur, float4* oP,
+                         float4* oD, uint32_t* oid, hipStream_t s);
+void launch_row_sort(const uint32_t* rowptr, uint32_t n, uint32_t* J, float* gap, double* gap64,
+                     int32_t* img, hipStream_t s);
+void launch_hist(const float4* P, const float4* D, uint32_t n, const uint32_t* rowptr,
+                 const uint32_t* J, const double* edges_p, const double* edges_m,
+                 const double* edges_t, int nb, int npb, int ntb, int axis, double L,
+                 const double* mu, unsigned long long* out, hipStream_t s);
+void launch_network(ContactBuffers& c, const float4* P, const float4* D, uint32_t n, int axis,
+                    double L, double gcut, double lam, double g0, uint32_t* info_dev,
+                    hipStream_t s, std::string* err);
+
+}  // namespace nf

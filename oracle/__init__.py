@@ -1,0 +1,185 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/nf_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
+baseline leg of bench.py, as the checker.  The shipped library never loads
+it.  Build with ``make -C oracle``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnforacle.so")
+_lib = None
+
+
+class GenStats(C.Structure):
+    _fields_ = [
+        ("n_accepted", C.c_uint32),
+        ("rounds", C.c_uint32),
+        ("status", C.c_int32),
+        ("pad", C.c_uint32),
+        ("candidates", C.c_uint64),
+        ("proposals", C.c_uint64),
+        ("pair_tests", C.c_uint64),
+        ("rejected_old", C.c_uint64),
+        ("rejected_new", C.c_uint64),
+    ]
+
+    def to_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        from nanofiller_amd import HistSpec, NetworkInfo, Params
+
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        sigs = [
+            ("orc_philox4x32_10", None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
+            ("orc_u01", C.c_double, [C.c_uint32, C.c_uint32]),
+            ("orc_exp", C.c_double, [C.c_double]),
+            ("orc_log", C.c_double, [C.c_double]),
+            ("orc_agg_centres", None, [P(Params), C.c_void_p]),
+            ("orc_vmf_basis", None, [C.c_void_p, C.c_void_p, C.c_void_p]),
+            ("orc_candidate", C.c_int, [P(Params), C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+            ("orc_pair_d2", C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, P(C.c_int)]),
+            ("orc_generate", C.c_int, [P(Params), C.c_void_p, P(GenStats), C.c_void_p, C.c_int]),
+            ("orc_contacts", C.c_int64, [P(Params), C.c_void_p, C.c_uint32, C.c_double, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                         P(C.c_uint64), C.c_int]),
+            ("orc_histograms", C.c_int, [P(Params), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                         C.c_int64, P(HistSpec), C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]),
+            ("orc_network", C.c_int64, [P(Params), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_int64, P(NetworkInfo)]),
+        ]
+        for name, res, args in sigs:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def philox(ctr, key) -> list[int]:
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def pair_d2(ri, rj, shift, R) -> tuple[float, bool]:
+    a = np.ascontiguousarray(ri, np.float32)
+    b = np.ascontiguousarray(rj, np.float32)
+    s = np.ascontiguousarray(shift, np.float64)
+    w = C.c_int()
+    d2 = lib().orc_pair_d2(a.ctypes.data, b.ctypes.data, s.ctypes.data, float(R), C.byref(w))
+    return d2, bool(w.value)
+
+
+def candidate(p, rnd: int, slot: int, agg=None):
+    out = np.zeros(8, np.float32)
+    a = np.ascontiguousarray(agg if agg is not None else np.zeros(3), np.float64)
+    v = lib().orc_candidate(C.byref(p), a.ctypes.data, rnd, slot, out.ctypes.data)
+    return v, out
+
+
+def agg_centres(p) -> np.ndarray:
+    out = np.zeros((max(p.n_agg, 1), 3), np.float64)
+    lib().orc_agg_centres(C.byref(p), out.ctypes.data)
+    return out[: p.n_agg]
+
+
+def generate(p, threads: int = 1):
+    rods = np.zeros((p.n_rods, 8), np.float32)
+    st = GenStats()
+    acc = np.zeros(max(p.max_rounds, 1), np.uint32)
+    rc = lib().orc_generate(C.byref(p), rods.ctypes.data, C.byref(st), acc.ctypes.data, threads)
+    if rc != 0:
+        raise RuntimeError(f"oracle generate failed {rc}")
+    return rods[: st.n_accepted], st.to_dict(), acc[: st.rounds]
+
+
+def contacts(p, rods: np.ndarray, gap: float, threads: int = 1):
+    r = np.ascontiguousarray(rods, np.float32)
+    n = r.shape[0]
+    tests = C.c_uint64()
+    cnt = lib().orc_contacts(C.byref(p), r.ctypes.data, n, gap, None, None, None, None, None, 0,
+                             C.byref(tests), threads)
+    I = np.zeros(cnt, np.uint32)
+    J = np.zeros(cnt, np.uint32)
+    g32 = np.zeros(cnt, np.float32)
+    g64 = np.zeros(cnt, np.float64)
+    im = np.zeros(cnt, np.int32)
+    cnt2 = lib().orc_contacts(C.byref(p), r.ctypes.data, n, gap, I.ctypes.data, J.ctypes.data,
+                              g32.ctypes.data, g64.ctypes.data, im.ctypes.data, cnt, C.byref(tests),
+                              threads)
+    assert cnt2 == cnt
+    return {"I": I, "J": J, "gap": g32, "gap64": g64, "image": im, "pair_tests": tests.value}
+
+
+def histograms(p, rods, I, J, bins=64, centre_axis=0, pol_axis=(1.0, 0.0, 0.0), phi_bins=0,
+               theta_bins=0):
+    from nanofiller_amd import HistSpec
+
+    spec = HistSpec()
+    spec.bins = bins
+    spec.centre_axis = centre_axis
+    for i in range(3):
+        spec.pol_axis[i] = pol_axis[i]
+    spec.phi_bins = phi_bins
+    spec.theta_bins = theta_bins
+    r = np.ascontiguousarray(rods, np.float32)
+    I = np.ascontiguousarray(I, np.uint32)
+    J = np.ascontiguousarray(J, np.uint32)
+    c = np.zeros(bins, np.int64)
+    pp = np.zeros(bins, np.int64)
+    m = np.zeros(bins, np.int64)
+    pt = np.zeros(max(phi_bins * theta_bins, 1), np.int64)
+    rc = lib().orc_histograms(C.byref(p), r.ctypes.data, r.shape[0], I.ctypes.data, J.ctypes.data,
+                              len(I), C.byref(spec), c.ctypes.data, pp.ctypes.data, m.ctypes.data,
+                              pt.ctypes.data if phi_bins else None)
+    if rc != 0:
+        raise RuntimeError("oracle histograms failed")
+    out = {"centre": c, "polar": pp, "mutual": m}
+    if phi_bins:
+        out["phitheta"] = pt.reshape(theta_bins, phi_bins)
+    return out
+
+
+def network(p, rods, I, J, gap64, image):
+    from nanofiller_amd import NetworkInfo
+
+    r = np.ascontiguousarray(rods, np.float32)
+    n = r.shape[0]
+    I = np.ascontiguousarray(I, np.uint32)
+    J = np.ascontiguousarray(J, np.uint32)
+    g = np.ascontiguousarray(gap64, np.float64)
+    im = np.ascontiguousarray(image, np.int32)
+    info = NetworkInfo()
+    nnz = lib().orc_network(C.byref(p), r.ctypes.data, n, I.ctypes.data, J.ctypes.data, g.ctypes.data,
+                            im.ctypes.data, len(I), None, None, None, None, 0, C.byref(info))
+    labels = np.zeros(n, np.uint32)
+    row = np.zeros(n + 3, np.uint32)
+    col = np.zeros(nnz, np.uint32)
+    val = np.zeros(nnz, np.float64)
+    lib().orc_network(C.byref(p), r.ctypes.data, n, I.ctypes.data, J.ctypes.data, g.ctypes.data,
+                      im.ctypes.data, len(I), labels.ctypes.data, row.ctypes.data, col.ctypes.data,
+                      val.ctypes.data, nnz, C.byref(info))
+    return info.to_dict(), {"labels": labels, "row_ptr": row, "col": col, "val": val}
