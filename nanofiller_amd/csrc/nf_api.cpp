@@ -7,6 +7,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -40,6 +43,47 @@ int dalloc_bytes(nf_handle* h, void** p, size_t bytes) {
   h->allocs.push_back(q);
   *p = q;
   return NF_OK;
+}
+
+static bool trace_on() {
+  static int on = -1;
+  if (on < 0) {
+    const char* e = std::getenv("NF_TRACE");
+    on = (e && *e && *e != '0') ? 1 : 0;
+  }
+  return on == 1;
+}
+
+void trace(const char* fmt, ...) {
+  if (!trace_on()) return;
+  va_list ap;
+  va_start(ap, fmt);
+  std::fprintf(stderr, "[nf] ");
+  std::vfprintf(stderr, fmt, ap);
+  std::fprintf(stderr, "\n");
+  std::fflush(stderr);
+  va_end(ap);
+}
+
+hipError_t sync_stream(hipStream_t s, const char* what) {
+  static double limit = -1.0;
+  if (limit < 0.0) {
+    const char* e = std::getenv("NF_SYNC_TIMEOUT_S");
+    limit = e ? std::atof(e) : 120.0;
+    if (!(limit > 0.0)) limit = 120.0;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const hipError_t e = hipStreamQuery(s);
+    if (e != hipErrorNotReady) return e;
+    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (el > limit) {
+      std::fprintf(stderr, "[nf] %s: stream not done after %.0f s\n", what, el);
+      std::fflush(stderr);
+      return hipErrorLaunchTimeOut;
+    }
+    std::this_thread::sleep_for(std::chrono::microseconds(el < 0.01 ? 2 : 200));
+  }
 }
 
 void dfree(nf_handle* h, void* p) {
@@ -480,7 +524,7 @@ static int reset_generation(nf_handle* h) {
   fill_u32(b.ocount, 0u, b.total_cells, h->stream);
   fill_u32((uint32_t*)b.ohl, 0u, b.total_cells, h->stream);
   HIPCHK(h, hipGetLastError());
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, sync_stream(h->stream, "generation reset"));
   return NF_OK;
 }
 
@@ -512,9 +556,12 @@ extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
   int rc = reset_generation(h);
   if (rc) return rc;
   const bool use_graph = h->opt_graph && !h->opt_profile;
+  trace("generate: %d instance(s), precision %d, graph %d, profile %d", h->gb.n_inst, prec,
+        (int)use_graph, h->opt_profile);
   if (use_graph) {
     rc = build_graph(h, prec, h->opt_diag);
     if (rc) return rc;
+    trace("generate: graph built");
   }
   double kern[8] = {0};
   RoundInfo ri{};
@@ -546,9 +593,11 @@ extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
       ++launched;
     }
     if (e == hipSuccess) e = hipMemcpyAsync(ri_host, h->gb.ri, sizeof(RoundInfo), hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = sync_stream(h->stream, "generation round");
     if (e != hipSuccess) break;
     ri = *ri_host;
+    trace("generate: %u rounds launched, controller rounds %u, running %u, last T %u", launched,
+          ri.rounds, ri.running, ri.T);
     if (ri.running == 0 || launched > max_rounds + 2) break;
     if (batch < 8) batch *= 2;
   }

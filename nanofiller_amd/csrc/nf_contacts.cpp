@@ -143,8 +143,10 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   hipEventRecord(ev[2], s);
   scan_u32(c.cnt, c.rowptr, n, tmp, s);
   uint32_t total = 0;
+  trace("contacts: %u rods, %u cells, %u stencil offsets, count pass launched", n, c.ncells, c.st_n);
   HIPCHK(h, hipMemcpyAsync(&total, c.rowptr + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(h, hipStreamSynchronize(s));
+  HIPCHK(h, sync_stream(s, "contact count"));
+  trace("contacts: %u pairs counted", total);
   HIPCHK(h, hipGetLastError());
   c.n_contacts = total;
   c.cap_contacts = total;
@@ -175,7 +177,8 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   launch_row_sort(c.rowptr, n, c.J, c.gap, c.gap64, c.img, s);
   hipEventRecord(ev[5], s);
   HIPCHK(h, hipMemcpyAsync(h->cb_stats, ctr, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, s));
-  HIPCHK(h, hipStreamSynchronize(s));
+  HIPCHK(h, sync_stream(s, "contact record"));
+  trace("contacts: recorded and sorted");
   HIPCHK(h, hipGetLastError());
   float ms = 0.0f;
   const int from[4] = {0, 1, 3, 4}, to[4] = {1, 2, 4, 5};

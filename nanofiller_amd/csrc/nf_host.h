@@ -20,7 +20,7 @@ struct nf_handle {
   int graph_prec = -1, graph_diag = -1;
   hipEvent_t ev[9] = {};
   std::string err;
-  int opt_graph = 1, opt_profile = 0, opt_diag = 0, opt_contact_prec = -1;
+  int opt_graph = 0, opt_profile = 0, opt_diag = 0, opt_contact_prec = -1;
   int device = 0;
   std::vector<uint32_t> n_rods;  // current rod count per instance
   bool generated = false;
@@ -35,6 +35,10 @@ extern std::string g_last_error;
 namespace nf {
 
 int fail(nf_handle* h, int code, const std::string& msg);
+// Waits for the stream with a wall-clock limit (NF_SYNC_TIMEOUT_S, default
+// 120 s); prints progress when NF_TRACE is set.
+hipError_t sync_stream(hipStream_t s, const char* what);
+void trace(const char* fmt, ...);
 int dalloc_bytes(nf_handle* h, void** p, size_t bytes);
 void dfree(nf_handle* h, void* p);
 void release_contacts(nf_handle* h);
