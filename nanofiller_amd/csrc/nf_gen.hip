@@ -10,7 +10,6 @@
 
 namespace nf {
 
-constexpr int CTL_THREADS = 1024;
 constexpr int GEN_THREADS = 256;
 
 // ---------------------------------------------------------------------------
@@ -32,53 +31,46 @@ __device__ __forceinline__ uint32_t schedule(const InstDesc& d, const InstState&
   return clamp_batch((num + den - 1) / den, d);
 }
 
-__device__ uint32_t ctl_block_scan(uint32_t v, uint32_t* sm, uint32_t* total) {
-  // 1024 threads = 16 waves
+// Exclusive scan over the 64 lanes of one wave (all lanes active).
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
   uint32_t inc = v;
 #pragma unroll
   for (int o = 1; o < WAVE; o <<= 1) {
     const uint32_t t = __shfl_up(inc, o, WAVE);
-    if (lane >= o) inc += t;
+    inc += (lane >= o) ? t : 0u;
   }
-  if (lane == WAVE - 1) sm[wv] = inc;
-  __syncthreads();
-  uint32_t off = 0, tot = 0;
-  for (int w = 0; w < CTL_THREADS / WAVE; ++w) {
-    const uint32_t s = sm[w];
-    if (w < wv) off += s;
-    tot += s;
-  }
-  __syncthreads();
-  *total = tot;
-  return off + inc - v;
+  *total = (uint32_t)__builtin_amdgcn_readlane((int)inc, WAVE - 1);
+  return inc - v;
 }
 
-// Plans the round: M per running instance, slot prefix, queue reset.
-__global__ void __launch_bounds__(CTL_THREADS) k_ctl_begin(const InstDesc* __restrict__ desc,
-                                                           InstState* __restrict__ st,
-                                                           RoundInfo* __restrict__ ri,
-                                                           uint32_t* __restrict__ slot_base,
-                                                           uint32_t* __restrict__ queues,
-                                                           int n_inst) {
-  __shared__ uint32_t sm[CTL_THREADS / WAVE];
-  uint32_t carry = 0;
-  for (int b = 0; b < n_inst; b += CTL_THREADS) {
-    const int i = b + threadIdx.x;
+// Plans the round: M per running instance, slot prefix, queue reset.  One
+// wave; lane l owns a contiguous chunk of instances.
+__global__ void __launch_bounds__(WAVE) k_ctl_begin(const InstDesc* __restrict__ desc,
+                                                    InstState* __restrict__ st,
+                                                    RoundInfo* __restrict__ ri,
+                                                    uint32_t* __restrict__ slot_base,
+                                                    uint32_t* __restrict__ queues, int n_inst) {
+  const int lane = threadIdx.x;
+  const int chunk = (n_inst + WAVE - 1) / WAVE;
+  const int b = lane * chunk;
+  const int e = min(b + chunk, n_inst);
+  uint32_t sum = 0;
+  for (int i = b; i < e; ++i) {
     uint32_t M = 0;
-    if (i < n_inst && st[i].status == NF_STATUS_RUNNING) M = schedule(desc[i], st[i]);
-    if (i < n_inst) st[i].M = M;
-    uint32_t tot;
-    const uint32_t ex = ctl_block_scan(M, sm, &tot);
-    if (i < n_inst) slot_base[i] = carry + ex;
-    carry += tot;
+    if (st[i].status == NF_STATUS_RUNNING) M = schedule(desc[i], st[i]);
+    st[i].M = M;
+    slot_base[i] = sum;
+    sum += M;
   }
-  if (threadIdx.x == 0) {
-    slot_base[n_inst] = carry;
-    ri->T = carry;
+  uint32_t total;
+  const uint32_t off = wave_excl_scan(sum, &total);
+  for (int i = b; i < e; ++i) slot_base[i] += off;
+  if (lane == 0) {
+    slot_base[n_inst] = total;
+    ri->T = total;
   }
-  if (threadIdx.x < Q_N * QUEUES_PER_KERNEL) queues[threadIdx.x] = 0;
+  if (lane < Q_N * QUEUES_PER_KERNEL) queues[lane] = 0;
 }
 
 __device__ __forceinline__ int find_inst(const uint32_t* __restrict__ base, int n, uint32_t g) {
@@ -209,18 +201,16 @@ __global__ void __launch_bounds__(GEN_THREADS) k_commit(const InstDesc* __restri
   }
 }
 
-// Closes the round: accepted counts, statistics, termination.
-__global__ void __launch_bounds__(CTL_THREADS) k_ctl_end(const InstDesc* __restrict__ desc,
-                                                         InstState* __restrict__ st,
-                                                         RoundInfo* __restrict__ ri,
-                                                         const uint32_t* __restrict__ slot_base,
-                                                         const uint32_t* __restrict__ srank,
-                                                         const uint32_t* __restrict__ pstart,
-                                                         int n_inst) {
-  __shared__ uint32_t running;
-  if (threadIdx.x == 0) running = 0;
-  __syncthreads();
-  for (int i = threadIdx.x; i < n_inst; i += CTL_THREADS) {
+// Closes the round: accepted counts, statistics, termination.  One wave.
+__global__ void __launch_bounds__(WAVE) k_ctl_end(const InstDesc* __restrict__ desc,
+                                                  InstState* __restrict__ st,
+                                                  RoundInfo* __restrict__ ri,
+                                                  const uint32_t* __restrict__ slot_base,
+                                                  const uint32_t* __restrict__ srank,
+                                                  const uint32_t* __restrict__ pstart, int n_inst) {
+  const int lane = threadIdx.x;
+  uint32_t running = 0;
+  for (int i = lane; i < n_inst; i += WAVE) {
     InstState s = st[i];
     if (s.status != NF_STATUS_RUNNING) continue;
     const InstDesc& d = desc[i];
@@ -252,11 +242,13 @@ __global__ void __launch_bounds__(CTL_THREADS) k_ctl_end(const InstDesc* __restr
     st[i].aprev = s.aprev;
     st[i].stall = s.stall;
     st[i].status = s.status;
-    if (s.status == NF_STATUS_RUNNING) atomicAdd(&running, 1u);
+    running += (s.status == NF_STATUS_RUNNING) ? 1u : 0u;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    ri->running = running;
+  uint32_t v = running;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  if (lane == 0) {
+    ri->running = v;
     ri->rounds += 1;
   }
 }
@@ -272,7 +264,7 @@ static int gen_grid(uint64_t n) {
 void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* ev) {
   const int n = b.n_inst;
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(CTL_THREADS), 0, s, b.desc, b.st, b.ri,
+  hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(WAVE), 0, s, b.desc, b.st, b.ri,
                      b.slot_base, b.queues, n);
   fill_u32(b.pcount, 0u, b.total_cells, s);
   fill_u32((uint32_t*)b.phl, 0u, b.total_cells, s);
@@ -330,7 +322,7 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
                      b.ri, b.slot_base, n, b.sP, b.sD, b.scell, b.surv, b.srank, b.bP, b.bD,
                      b.ostart, b.ocount, b.ohl, b.oP, b.oD, b.oid);
   if (ev) (void)hipEventRecord(ev[6], s);
-  hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(CTL_THREADS), 0, s, b.desc, b.st, b.ri,
+  hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(WAVE), 0, s, b.desc, b.st, b.ri,
                      b.slot_base, b.srank, b.pstart, n);
   if (ev) (void)hipEventRecord(ev[7], s);
 }

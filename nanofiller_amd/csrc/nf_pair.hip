@@ -53,21 +53,70 @@ struct CellGeom {
   InstState* state;
 };
 
+// Per-lane statistics, flushed to the instance (or the contact counters)
+// by every lane together; lanes other than 0 add zero.
+struct Acc {
+  uint32_t tests, rech, kills;
+  double maxdd;
+};
+
+// Values the hardware must treat as wave-uniform: read from lane 0 into an
+// SGPR so that every branch and loop bound that depends on them is scalar.
+// The pair engine keeps all control flow outside the innermost tile loop
+// uniform in this way, and performs cross-lane operations (shuffles,
+// ballots, barriers) only where every lane of the wave is active.
+__device__ __forceinline__ uint32_t uni(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ int unii(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float unif(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+template <bool GEN>
+__device__ __forceinline__ void flush(const PairArgs& a, const CellGeom& g, Acc& acc, int mode) {
+  const uint32_t wt = wave_sum(acc.tests), wr = wave_sum(acc.rech), wk = wave_sum(acc.kills);
+  double v = acc.maxdd;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+  const bool l0 = threadIdx.x == 0;
+  unsigned long long* c0;
+  unsigned long long* c1;
+  unsigned long long* c2;
+  unsigned long long* c3;
+  if (GEN) {
+    c0 = &g.state->pair_tests;
+    c1 = &g.state->rechecks;
+    c2 = mode == PM_GEN_OLD ? &g.state->rej_old : &g.state->rej_new;
+    c3 = (unsigned long long*)&g.state->max_dd;
+  } else {
+    c0 = &a.ctr[0];
+    c1 = &a.ctr[1];
+    c2 = &a.ctr[3];
+    c3 = &a.ctr[2];
+  }
+  // every lane issues the atomics (lane 0 carries the value) so no branch
+  // precedes the next cross-lane operation of the caller
+  atomicAdd(c0, (unsigned long long)(l0 ? wt : 0u));
+  atomicAdd(c1, (unsigned long long)(l0 ? wr : 0u));
+  atomicAdd(c2, (unsigned long long)(l0 ? wk : 0u));
+  atomicMax(c3, (unsigned long long)(l0 ? __double_as_longlong(v) : 0ll));
+  acc.tests = acc.rech = acc.kills = 0;
+  acc.maxdd = 0.0;
+}
+
 template <int MODE, int PREC>
 __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, float4* tP,
-                             float4* tD, uint32_t* tX) {
+                             float4* tD, uint32_t* tX, uint32_t* tDummy, Acc& acc) {
   constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
   const int lane = threadIdx.x;
-  const uint32_t pc = a.prCount[gc];
+  const uint32_t pc = uni(a.prCount[gc]);
   if (pc == 0) return;
-  const uint32_t ps = a.prStart[gc];
+  const uint32_t ps = uni(a.prStart[gc]);
   const uint32_t lc = gc - g.cell_base;
   const int c0 = (int)(lc % (uint32_t)g.nc[0]);
   const int c1 = (int)((lc / (uint32_t)g.nc[0]) % (uint32_t)g.nc[1]);
   const int c2 = (int)(lc / ((uint32_t)g.nc[0] * (uint32_t)g.nc[1]));
-
-  uint32_t tests = 0, rech = 0, kills = 0;
-  double maxdd = 0.0;
 
   for (uint32_t pb = 0; pb < pc; pb += WAVE) {
     const uint32_t G = min((uint32_t)WAVE, pc - pb);
@@ -77,26 +126,24 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
     const uint32_t lk = __builtin_ctz(K);
     const uint32_t p = (uint32_t)lane >> lk;
     const uint32_t sub = (uint32_t)lane & (K - 1);
-    const uint32_t pidx = ps + pb + p;
-    bool have = p < G;
-    float4 P1 = make_float4(0.f, 0.f, 0.f, 0.f), D1 = P1;
-    uint32_t X1 = 0;
-    if (have) {
-      P1 = a.prP[pidx];
-      D1 = a.prD[pidx];
-      X1 = a.prX[pidx];
-      if (MODE == PM_GEN_NEW) have = a.prAlive[pidx] != 0u;
-    }
-    const float chunk_hl = wave_max(have ? P1.w : 0.0f);
+    const bool inrange = p < G;
+    const uint32_t pidx = ps + pb + (inrange ? p : 0u);  // clamped: every lane loads
+    const float4 P1 = a.prP[pidx];
+    const float4 D1 = a.prD[pidx];
+    const uint32_t X1 = a.prX[pidx];
+    bool have = inrange;
+    if (MODE == PM_GEN_NEW) have = have && (a.prAlive[pidx] != 0u);
+    const float chunk_hl = unif(wave_max(have ? P1.w : 0.0f));
     bool hit = false;
     const float rr_chunk = (chunk_hl + g.hl_max) * 1.000001f + g.R_max + CELL_MARGIN;
     const float rr_chunk2 = rr_chunk * rr_chunk;
 
     for (uint32_t si = 0; si < g.st_n; ++si) {
       const int4 so = g.st[si];
-      const float md2 = __int_as_float(so.w);
+      const int ox = unii(so.x), oy = unii(so.y), oz = unii(so.z);
+      const float md2 = unif(__int_as_float(so.w));
       if (md2 > rr_chunk2) break;
-      int raw[3] = {c0 + so.x, c1 + so.y, c2 + so.z};
+      const int raw[3] = {c0 + ox, c1 + oy, c2 + oz};
       int img[3], m[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -106,26 +153,28 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
         m[d] = mm;
       }
       const uint32_t nc = g.cell_base + (uint32_t)(m[0] + g.nc[0] * (m[1] + g.nc[1] * m[2]));
-      uint32_t n = a.nbCount[nc];
+      uint32_t n = uni(a.nbCount[nc]);
       if (MODE == PM_GEN_OLD) n = min(n, g.cap);
       if (n == 0) continue;
-      const float rr_c = (chunk_hl + a.nbHl[nc]) * 1.000001f + g.R_max + CELL_MARGIN;
+      const float rr_c = (chunk_hl + unif(a.nbHl[nc])) * 1.000001f + g.R_max + CELL_MARGIN;
       if (md2 > rr_c * rr_c) continue;
       const double sx = (double)img[0] * g.box[0], sy = (double)img[1] * g.box[1],
                    sz = (double)img[2] * g.box[2];
       const float fx = (float)sx, fy = (float)sy, fz = (float)sz;
-      const bool same = (so.x == 0 && so.y == 0 && so.z == 0);
-      const uint32_t nstart = a.nbStart[nc];
+      const bool same = (ox == 0 && oy == 0 && oz == 0);
+      const uint32_t nstart = uni(a.nbStart[nc]);
       for (uint32_t tb = 0; tb < n; tb += WAVE) {
         const uint32_t tn = min((uint32_t)WAVE, n - tb);
-        if ((uint32_t)lane < tn) {
-          const uint32_t q = nstart + tb + lane;
-          tP[lane] = a.nbP[q];
-          tD[lane] = a.nbD[q];
-          uint32_t x = a.nbX[q];
-          if (MODE == PM_GEN_NEW && a.nbAlive[q] == 0u) x = 0xFFFFFFFFu;
-          tX[lane] = x;
-        }
+        // every lane loads (index clamped to the tile) and writes its slot;
+        // slots >= tn are never read
+        const uint32_t q = nstart + tb + min((uint32_t)lane, tn - 1u);
+        const float4 lp = a.nbP[q];
+        const float4 ld = a.nbD[q];
+        uint32_t x = a.nbX[q];
+        if (MODE == PM_GEN_NEW) x = (a.nbAlive[q] != 0u) ? x : 0xFFFFFFFFu;
+        tP[lane] = lp;
+        tD[lane] = ld;
+        tX[lane] = x;
         __syncthreads();
         if (have) {
           for (uint32_t j = sub; j < tn; j += K) {
@@ -151,7 +200,7 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
               const float R32 = GEN ? (g.sf32 * (D1.w + D2.w) + g.sep32) : ((D1.w + D2.w) + a.gap32);
               const F32Pair r = pair_f32(P1, D1, P2, D2, fx, fy, fz, R32, g.ew, GEN);
               if (r.decided < 0) {
-                ++rech;
+                ++acc.rech;
                 d2_64 = pair_d2_f64(P1, D1, P2, D2, sx, sy, sz, R64, &within);
                 cut = GEN ? (d2_64 < R64 * R64) : (d2_64 <= R64 * R64);
               } else {
@@ -161,12 +210,12 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
                   bool w2;
                   const double e64 = pair_d2_f64(P1, D1, P2, D2, sx, sy, sz, R64, &w2);
                   const double dd = fabs(sqrt((double)r.d2) - sqrt(e64));
-                  if (dd > maxdd) maxdd = dd;
+                  acc.maxdd = dd > acc.maxdd ? dd : acc.maxdd;
                 }
               }
             }
             if (!within) continue;
-            ++tests;
+            ++acc.tests;
             if (!cut) continue;
             if (GEN) {
               hit = true;
@@ -197,35 +246,20 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
       }
     }
     if (GEN) {
-      const unsigned long long m = __ballot(hit);
+      const unsigned long long mk = __ballot(hit);
       const unsigned long long gm = (K == 64) ? ~0ull : ((1ull << K) - 1ull);
-      const bool hitp = ((m >> (p * K)) & gm) != 0ull;
+      const bool hitp = ((mk >> (p * K)) & gm) != 0ull;
       const bool lead = (sub == 0) && have;
-      if (lead && hitp) ++kills;
-      if (MODE == PM_GEN_OLD && lead && hitp) a.alive_out[pidx] = 0u;
-      if (MODE == PM_GEN_NEW && lead && !hitp) a.surv[X1] = 1u;
-    }
-  }
-  const uint32_t wt = wave_sum(tests), wr = wave_sum(rech), wk = wave_sum(kills);
-  if (a.diag) {
-    double v = maxdd;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
-    maxdd = v;
-  }
-  if (lane == 0) {
-    if (GEN) {
-      InstState* s = g.state;
-      if (wt) atomicAdd(&s->pair_tests, (unsigned long long)wt);
-      if (wr) atomicAdd(&s->rechecks, (unsigned long long)wr);
-      if (wk) atomicAdd(MODE == PM_GEN_OLD ? &s->rej_old : &s->rej_new, (unsigned long long)wk);
-      if (a.diag && maxdd > 0.0)
-        atomicMax((unsigned long long*)&s->max_dd, (unsigned long long)__double_as_longlong(maxdd));
-    } else {
-      if (wt) atomicAdd(&a.ctr[0], (unsigned long long)wt);
-      if (wr) atomicAdd(&a.ctr[1], (unsigned long long)wr);
-      if (a.diag && maxdd > 0.0)
-        atomicMax(&a.ctr[2], (unsigned long long)__double_as_longlong(maxdd));
+      acc.kills += (lead && hitp) ? 1u : 0u;
+      // branch-free result store: lanes without a result write a private
+      // LDS slot instead
+      if (MODE == PM_GEN_OLD) {
+        uint32_t* dst = (lead && hitp) ? &a.alive_out[pidx] : &tDummy[lane];
+        *dst = 0u;
+      } else {
+        uint32_t* dst = (have && !hitp) ? &a.surv[X1] : &tDummy[lane];
+        *dst = 1u;
+      }
     }
   }
 }
@@ -235,11 +269,14 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_pair(PairArgs a) {
   __shared__ float4 tP[WAVE];
   __shared__ float4 tD[WAVE];
   __shared__ uint32_t tX[WAVE];
+  __shared__ uint32_t tDummy[WAVE];
   constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
   const uint32_t W = a.ncells_total;
   const int xcd = blockIdx.x & 7;
+  const bool l0 = threadIdx.x == 0;
   CellGeom g;
-  int cur_inst = -1;
+  Acc acc = {0u, 0u, 0u, 0.0};
+  uint32_t cur_inst = 0xFFFFFFFFu;
   if (!GEN) {
     for (int d = 0; d < 3; ++d) {
       g.box[d] = a.box[d];
@@ -264,14 +301,16 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_pair(PairArgs a) {
     const uint32_t qb = (uint32_t)(((uint64_t)W * (uint32_t)q) / 8u);
     const uint32_t qe = (uint32_t)(((uint64_t)W * (uint32_t)(q + 1)) / 8u);
     for (;;) {
-      uint32_t item = 0;
-      if (threadIdx.x == 0) item = atomicAdd(&a.qctr[q], 1u);
-      item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, WAVE));
+      // all lanes take part in the atomic (lane 0 adds 1), lane 0's old
+      // value is the work item
+      const uint32_t old = atomicAdd(&a.qctr[q], l0 ? 1u : 0u);
+      const uint32_t item = uni(old);
       if (item >= qe - qb) break;
       const uint32_t gc = qb + item;
       if (GEN) {
-        const int inst = (int)a.cell_inst[gc];
+        const uint32_t inst = uni(a.cell_inst[gc]);
         if (inst != cur_inst) {
+          if (cur_inst != 0xFFFFFFFFu) flush<GEN>(a, g, acc, MODE);
           const InstDesc& d = a.desc[inst];
           for (int dd = 0; dd < 3; ++dd) {
             g.box[dd] = d.smp.box[dd];
@@ -293,9 +332,10 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_pair(PairArgs a) {
           cur_inst = inst;
         }
       }
-      process_cell<MODE, PREC>(a, g, gc, tP, tD, tX);
+      process_cell<MODE, PREC>(a, g, gc, tP, tD, tX, tDummy, acc);
     }
   }
+  if (!GEN || cur_inst != 0xFFFFFFFFu) flush<GEN>(a, g, acc, MODE);
 }
 
 static int g_pair_grid = 0;

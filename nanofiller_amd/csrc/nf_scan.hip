@@ -19,7 +19,7 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 #pragma unroll
   for (int o = 1; o < WAVE; o <<= 1) {
     const uint32_t t = __shfl_up(v, o, WAVE);
-    if (lane >= o) v += t;
+    v += (lane >= o) ? t : 0u;
   }
   return v;
 }
@@ -51,7 +51,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const uint32_t* __r
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     const uint64_t i = base + k;
-    if (i < n) s += in[i];
+    const uint32_t v = in[i < n ? i : n - 1];  // clamped load, masked below
+    s += i < n ? v : 0u;
   }
   uint32_t tot;
   block_excl_scan(s, sm, &tot);
@@ -62,13 +63,16 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_sums(uint32_t* __restrict
                                                             uint32_t nt, uint32_t* __restrict__ out,
                                                             uint64_t n) {
   __shared__ uint32_t sm[SCAN_THREADS / WAVE];
+  __shared__ uint32_t dummy[SCAN_THREADS];
   uint32_t carry = 0;
   for (uint32_t b = 0; b < nt; b += SCAN_THREADS) {
     const uint32_t i = b + threadIdx.x;
-    const uint32_t v = i < nt ? sums[i] : 0u;
+    const uint32_t v = sums[i < nt ? i : nt - 1];
     uint32_t tot;
-    const uint32_t ex = block_excl_scan(v, sm, &tot);
-    if (i < nt) sums[i] = carry + ex;
+    const uint32_t ex = block_excl_scan(i < nt ? v : 0u, sm, &tot);
+    // branch-free store: out-of-range threads write a private LDS slot
+    uint32_t* dst = i < nt ? &sums[i] : &dummy[threadIdx.x];
+    *dst = carry + ex;
     carry += tot;
   }
   if (threadIdx.x == 0) out[n] = carry;
@@ -85,7 +89,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __r
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     const uint64_t i = base + k;
-    v[k] = i < n ? in[i] : 0u;
+    const uint32_t x = in[i < n ? i : n - 1];
+    v[k] = i < n ? x : 0u;
     s += v[k];
   }
   uint32_t tot;
