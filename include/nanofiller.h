@@ -236,6 +236,13 @@ int nf_set_option(nf_handle* h, int key, int64_t value);
 int nf_contacts_stats(nf_handle* h, uint64_t* pair_tests, uint64_t* rechecks, double* max_dd,
                       float* kernel_ms);
 
+/* Diagnostics (no counterpart in the reference): evaluates one FP64 device
+ * building block element-wise on the GPU so callers can check it against
+ * the CPU.  which: 0 correctly rounded sqrt, 1 exp, 2 log, 3 division
+ * (in holds 2n values, out[i] = in[2i]/in[2i+1]), 4 Philox-word-to-uniform
+ * conversion (bits of in[i] = hi:lo words), 5 the device's own sqrt. */
+int nf_selftest(int which, const double* in, double* out, uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif

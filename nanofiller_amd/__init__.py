@@ -168,6 +168,7 @@ _SIGS = [
     ("nf_write_coo", C.c_int, [C.c_void_p, C.c_char_p]),
     ("nf_version", C.c_char_p, []),
     ("nf_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
+    ("nf_selftest", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint64]),
 ]
 
 
@@ -356,6 +357,19 @@ class Packing:
 
     def write_coo(self, prefix: str = "") -> None:
         _check(lib().nf_write_coo(self._h, prefix.encode()), self._h)
+
+
+SELFTEST = {"sqrt": 0, "exp": 1, "log": 2, "div": 3, "u01": 4, "hw_sqrt": 5}
+
+
+def selftest(which: str, x: np.ndarray) -> np.ndarray:
+    """Evaluate one FP64 device building block element-wise on the GPU."""
+    k = SELFTEST[which]
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    n = a.size // 2 if k == 3 else a.size
+    out = np.zeros(n, np.float64)
+    _check(lib().nf_selftest(k, a.ctypes.data, out.ctypes.data, n))
+    return out
 
 
 def version() -> str:

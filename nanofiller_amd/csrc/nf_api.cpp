@@ -688,3 +688,31 @@ extern "C" int nf_set_rods(nf_handle* h, int inst, const float* rods, uint32_t n
   return NF_OK;
 }
 
+
+// ===========================================================================
+// diagnostics
+// ===========================================================================
+extern "C" int nf_selftest(int which, const double* in, double* out, uint64_t n) {
+  if (which < 0 || which > 5 || (n && (!in || !out))) return fail(nullptr, NF_E_INVALID, "bad selftest args");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, NF_E_NODEVICE, "no HIP device");
+  const uint64_t nin = which == 3 ? 2 * n : n;
+  double *din = nullptr, *dout = nullptr;
+  if (hipMalloc((void**)&din, sizeof(double) * (nin ? nin : 1)) != hipSuccess)
+    return fail(nullptr, NF_E_NOMEM, "hipMalloc failed");
+  if (hipMalloc((void**)&dout, sizeof(double) * (n ? n : 1)) != hipSuccess) {
+    (void)hipFree(din);
+    return fail(nullptr, NF_E_NOMEM, "hipMalloc failed");
+  }
+  hipError_t e = hipMemcpy(din, in, sizeof(double) * nin, hipMemcpyHostToDevice);
+  if (e == hipSuccess && n) {
+    launch_selftest(which, din, dout, n, nullptr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(nullptr, NF_E_HIP, std::string("selftest: ") + hipGetErrorString(e));
+  return NF_OK;
+}

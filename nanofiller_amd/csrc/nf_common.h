@@ -60,6 +60,36 @@ NF_HD double u01(uint32_t a, uint32_t b) {
 // --------------------------------------------------------------------------
 NF_HD double nf_bits_to_double(uint64_t u) { return __builtin_bit_cast(double, u); }
 
+// Correctly rounded square root.  The hardware / library sqrt of the GPU is
+// not guaranteed to round correctly in FP64, so: estimate, one Newton step
+// with fused residual, then an exact residual test picks between the
+// neighbouring doubles (for y = k*ulp the residual x - y^2 is an integer
+// multiple of ulp^2, so "x above the upper midpoint" is exactly
+// r > y*ulp and "x below the lower midpoint" is r <= -y*ulp_below).
+NF_HD double sqrt_cr(double x) {
+  if (!(x > 0.0) || !(x < 0x1p1023)) return sqrt(x);
+  double scale = 1.0;
+  if (x < 0x1p-900) {
+    x = x * 0x1p1000;
+    scale = 0x1p-500;
+  } else if (x > 0x1p900) {
+    x = x * 0x1p-1000;
+    scale = 0x1p500;
+  }
+  double y = sqrt(x);
+  double r = fma(-y, y, x);
+  y = fma(r, 0.5 / y, y);
+  r = fma(-y, y, x);
+  const uint64_t b = __builtin_bit_cast(uint64_t, y);
+  const double u_up = __builtin_bit_cast(double, ((b >> 52) - 52ull) << 52);
+  const double u_dn = (b & 0x000FFFFFFFFFFFFFull) == 0 ? u_up * 0.5 : u_up;
+  if (r > y * u_up)
+    y = y + u_up;
+  else if (r <= -(y * u_dn))
+    y = y - u_dn;
+  return y * scale;
+}
+
 NF_HD double pow2i(int k) {
   return nf_bits_to_double((uint64_t)(k + 1023) << 52);
 }
@@ -158,7 +188,7 @@ NF_HD void vmf_basis(const double mu[3], double e1[3], double e2[3]) {
     t[2] = 1.0;
   const double d = (t[0] * mu[0] + t[1] * mu[1]) + t[2] * mu[2];
   const double u0 = t[0] - d * mu[0], u1 = t[1] - d * mu[1], u2 = t[2] - d * mu[2];
-  const double nrm = sqrt((u0 * u0 + u1 * u1) + u2 * u2);
+  const double nrm = sqrt_cr((u0 * u0 + u1 * u1) + u2 * u2);
   e1[0] = u0 / nrm;
   e1[1] = u1 / nrm;
   e1[2] = u2 / nrm;
@@ -415,7 +445,7 @@ NF_HD int sample_candidate(const SampleDesc& d, uint32_t round, uint32_t slot, R
       const double v1 = 2.0 * u01(o.x, o.y) - 1.0, v2 = 2.0 * u01(o.z, o.w) - 1.0;
       const double s = v1 * v1 + v2 * v2;
       if (s < 1.0 && s > 0.0) {
-        const double rr = sqrt(1.0 - w * w) / sqrt(s);
+        const double rr = sqrt_cr(1.0 - w * w) / sqrt_cr(s);
         const double a1 = v1 * rr, a2 = v2 * rr;
         cx = (w * d.axis[0] + a1 * d.e1[0]) + a2 * d.e2[0];
         cy = (w * d.axis[1] + a1 * d.e1[1]) + a2 * d.e2[1];
@@ -429,7 +459,7 @@ NF_HD int sample_candidate(const SampleDesc& d, uint32_t round, uint32_t slot, R
       const double v1 = 2.0 * u01(o.x, o.y) - 1.0, v2 = 2.0 * u01(o.z, o.w) - 1.0;
       const double s = v1 * v1 + v2 * v2;
       if (s < 1.0 && s > 0.0) {
-        const double q = 2.0 * sqrt(1.0 - s);
+        const double q = 2.0 * sqrt_cr(1.0 - s);
         cx = v1 * q;
         cy = v2 * q;
         cz = 1.0 - 2.0 * s;
@@ -448,7 +478,7 @@ NF_HD int sample_candidate(const SampleDesc& d, uint32_t round, uint32_t slot, R
       const double v1 = 2.0 * u01(o.x, o.y) - 1.0, v2 = 2.0 * u01(o.z, o.w) - 1.0;
       const double s = v1 * v1 + v2 * v2;
       if (s < 1.0 && s > 0.0) {
-        z = v1 * sqrt(-2.0 * dlog(s) / s);
+        z = v1 * sqrt_cr(-2.0 * dlog(s) / s);
         ok = true;
       }
     }

@@ -204,6 +204,7 @@ void launch_hist(const float4* P, const float4* D, uint32_t n, const uint32_t* r
                  const uint32_t* J, const double* edges_p, const double* edges_m,
                  const double* edges_t, int nb, int npb, int ntb, int axis, double L,
                  const double* mu, unsigned long long* out, hipStream_t s);
+void launch_selftest(int which, const double* in, double* out, uint64_t n, hipStream_t s);
 void launch_network(ContactBuffers& c, const float4* P, const float4* D, uint32_t n, int axis,
                     double L, double gcut, double lam, double g0, uint32_t* info_dev,
                     hipStream_t s, std::string* err);

@@ -233,7 +233,7 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
                 const uint32_t k = atomicAdd(&a.fill[I], 1u);
                 if (k >= a.rowptr[I + 1] - a.rowptr[I]) continue;  // host reports the mismatch
                 const uint32_t pos = a.rowptr[I] + k;
-                const double gv = sqrt(d2_64) - ((double)D1.w + (double)D2.w);
+                const double gv = sqrt_cr(d2_64) - ((double)D1.w + (double)D2.w);
                 a.outJ[pos] = J;
                 a.outGap[pos] = (float)gv;
                 a.outGap64[pos] = gv;

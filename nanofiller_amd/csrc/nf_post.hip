@@ -176,13 +176,13 @@ __global__ void __launch_bounds__(HIST_THREADS) k_hist_rods(const float4* __rest
     b = b < 0 ? 0 : (b >= h.nb ? h.nb - 1 : b);
     atomicAdd(&hc[b], 1u);
     const double cx = d.x, cy = d.y, cz = d.z;
-    const double cn = sqrt((cx * cx + cy * cy) + cz * cz);
+    const double cn = sqrt_cr((cx * cx + cy * cy) + cz * cz);
     const double cosv = ((cx * h.mu[0] + cy * h.mu[1]) + cz * h.mu[2]) / cn;
     atomicAdd(&hp[edge_bin(cosv, h.ep, h.nb)], 1u);
     if (n2 > 0) {
       const double ct = cz / cn;
       const int tb = edge_bin(ct, h.et, h.ntb);
-      const double rxy = sqrt(cx * cx + cy * cy);
+      const double rxy = sqrt_cr(cx * cx + cy * cy);
       int pb = 0;
       if (rxy > 0.0) {
         const double cp = cx / rxy;
@@ -215,11 +215,11 @@ __global__ void __launch_bounds__(HIST_THREADS) k_hist_mut(const float4* __restr
     if (b == e) continue;
     const float4 a = D[i];
     const double ax = a.x, ay = a.y, az = a.z;
-    const double na = sqrt((ax * ax + ay * ay) + az * az);
+    const double na = sqrt_cr((ax * ax + ay * ay) + az * az);
     for (uint32_t k = b; k < e; ++k) {
       const float4 c = D[J[k]];
       const double bx = c.x, by = c.y, bz = c.z;
-      const double nbv = sqrt((bx * bx + by * by) + bz * bz);
+      const double nbv = sqrt_cr((bx * bx + by * by) + bz * bz);
       const double dv = fabs((ax * bx + ay * by) + az * bz) / (na * nbv);
       atomicAdd(&hm[edge_bin(dv, h.em, h.nb)], 1u);
     }
@@ -250,6 +250,36 @@ void launch_hist(const float4* P, const float4* D, uint32_t n, const uint32_t* r
   if (rowptr)
     hipLaunchKernelGGL(k_hist_mut, dim3(blocks), dim3(HIST_THREADS), 0, s, D, n, rowptr, J, h,
                        out);
+}
+
+// ---------------------------------------------------------------------------
+// Device arithmetic self-test: the FP64 building blocks whose results must be
+// bit-identical to the CPU (correctly rounded sqrt, IEEE division, the
+// library's exp/log, uniform conversion).
+// ---------------------------------------------------------------------------
+__global__ void k_selftest(int which, const double* __restrict__ in, double* __restrict__ out,
+                           uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    double r = 0.0;
+    switch (which) {
+      case 0: r = sqrt_cr(in[i]); break;
+      case 1: r = dexp(in[i]); break;
+      case 2: r = dlog(in[i]); break;
+      case 3: r = in[2 * i] / in[2 * i + 1]; break;
+      case 4: {
+        const uint64_t b = __builtin_bit_cast(uint64_t, in[i]);
+        r = u01((uint32_t)(b >> 32), (uint32_t)b);
+        break;
+      }
+      default: r = sqrt(in[i]); break;
+    }
+    out[i] = r;
+  }
+}
+
+void launch_selftest(int which, const double* in, double* out, uint64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_selftest, dim3(grid_for(n, 256)), dim3(256), 0, s, which, in, out, n);
 }
 
 }  // namespace nf
