@@ -96,6 +96,10 @@ typedef struct nf_gen_stats {
   double wall_ms;         /* host clock, first candidate -> packing on HBM */
   double kern_ms[8];      /* HIP-event time per kernel class, see nf_kern  */
   double max_abs_d32_d64; /* max |d32-d64| over full tests (diagnostic run)*/
+  uint64_t tile_bytes;    /* bytes the reject-vs-accepted kernel streamed
+                             (probes and neighbour tiles; roofline figure) */
+  uint64_t tile_bytes_new;/* same for the reject-vs-same-round kernel      */
+  uint64_t rounds_launched; /* rounds launched (incl. no-op tail rounds)   */
 } nf_gen_stats;
 
 /* Kernel classes reported in nf_gen_stats.kern_ms. */
@@ -231,10 +235,11 @@ enum {
 int nf_set_option(nf_handle* h, int key, int64_t value);
 
 /* Statistics of the last nf_contacts call: capsule tests on pairs in reach,
- * FP64 rechecks, max |d32-d64| (NF_OPT_DIAG), and HIP-event times in ms of
- * the four contact stages (bin, count, record, row sort; kernel_ms[4]). */
+ * FP64 rechecks, max |d32-d64| (NF_OPT_DIAG), HIP-event times in ms of the
+ * four contact stages (bin, count, record, row sort; kernel_ms[4]) and the
+ * probe + tile bytes the record pass streamed. */
 int nf_contacts_stats(nf_handle* h, uint64_t* pair_tests, uint64_t* rechecks, double* max_dd,
-                      float* kernel_ms);
+                      float* kernel_ms, uint64_t* tile_bytes);
 
 /* Diagnostics (no counterpart in the reference): evaluates one FP64 device
  * building block element-wise on the GPU so callers can check it against

@@ -105,6 +105,9 @@ class GenStats(C.Structure):
         ("wall_ms", C.c_double),
         ("kern_ms", C.c_double * 8),
         ("max_abs_d32_d64", C.c_double),
+        ("tile_bytes", C.c_uint64),
+        ("tile_bytes_new", C.c_uint64),
+        ("rounds_launched", C.c_uint64),
     ]
 
     def to_dict(self) -> dict:
@@ -159,7 +162,7 @@ _SIGS = [
     ("nf_get_contacts", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_get_contact_gap64", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_contacts_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
-                                    C.POINTER(C.c_double), C.c_void_p]),
+                                    C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_uint64)]),
     ("nf_histograms", C.c_int, [C.c_void_p, C.POINTER(HistSpec), C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p]),
     ("nf_network", C.c_int, [C.c_void_p, C.POINTER(NetworkInfo)]),
@@ -311,10 +314,12 @@ class Packing:
         return out
 
     def contact_stats(self) -> dict:
-        t, r, m = C.c_uint64(), C.c_uint64(), C.c_double()
+        t, r, m, tb = C.c_uint64(), C.c_uint64(), C.c_double(), C.c_uint64()
         ms = (C.c_float * 4)()
-        _check(lib().nf_contacts_stats(self._h, C.byref(t), C.byref(r), C.byref(m), ms), self._h)
+        _check(lib().nf_contacts_stats(self._h, C.byref(t), C.byref(r), C.byref(m), ms, C.byref(tb)),
+               self._h)
         return {"pair_tests": t.value, "rechecks": r.value, "max_abs_d32_d64": m.value,
+                "tile_bytes": tb.value,
                 "kernel_ms": {"bin": ms[0], "count": ms[1], "record": ms[2], "sort": ms[3]}}
 
     def histograms(self, bins=64, centre_axis=0, pol_axis=(1.0, 0.0, 0.0), mutual=True,

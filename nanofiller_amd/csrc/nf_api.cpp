@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -572,23 +573,25 @@ extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
   const auto t0 = std::chrono::steady_clock::now();
   uint32_t launched = 0;
   int batch = 1;
+  std::vector<std::array<hipEvent_t, 8>> prof_ev;
+  size_t prof_used = 0;
   hipError_t e = hipSuccess;
   for (;;) {
     for (int k = 0; k < batch && e == hipSuccess; ++k) {
       if (use_graph) {
         e = hipGraphLaunch(h->graph, h->stream);
       } else {
-        launch_round(h->gb, prec, h->opt_diag, h->stream, h->opt_profile ? h->ev : nullptr);
-        e = hipGetLastError();
-        if (e == hipSuccess && h->opt_profile) {
-          e = hipEventSynchronize(h->ev[7]);
-          float ms = 0;
-          const int cls[7] = {NF_K_CONTROL, NF_K_GEN, NF_K_BIN, NF_K_REJECT_OLD, NF_K_REJECT_NEW,
-                              NF_K_COMMIT, NF_K_CONTROL};
-          for (int c = 0; c < 7 && e == hipSuccess; ++c) {
-            if (hipEventElapsedTime(&ms, h->ev[c], h->ev[c + 1]) == hipSuccess) kern[cls[c]] += ms;
+        hipEvent_t* ev = nullptr;
+        if (h->opt_profile) {
+          // one event set per round, read back after the final sync
+          if (prof_used == prof_ev.size()) {
+            prof_ev.emplace_back();
+            for (auto& x : prof_ev.back()) (void)hipEventCreate(&x);
           }
+          ev = prof_ev[prof_used++].data();
         }
+        launch_round(h->gb, prec, h->opt_diag, h->stream, ev);
+        e = hipGetLastError();
       }
       ++launched;
     }
@@ -603,6 +606,17 @@ extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
   }
   const auto t1 = std::chrono::steady_clock::now();
   (void)hipHostFree(ri_host);
+  if (e == hipSuccess && h->opt_profile) {
+    const int cls[7] = {NF_K_CONTROL, NF_K_GEN, NF_K_BIN, NF_K_REJECT_OLD, NF_K_REJECT_NEW,
+                        NF_K_COMMIT, NF_K_CONTROL};
+    for (size_t r = 0; r < prof_used; ++r)
+      for (int c = 0; c < 7; ++c) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, prof_ev[r][c], prof_ev[r][c + 1]) == hipSuccess) kern[cls[c]] += ms;
+      }
+  }
+  for (auto& a : prof_ev)
+    for (auto& x : a) (void)hipEventDestroy(x);
   if (e != hipSuccess) return fail(h, NF_E_HIP, std::string("generation: ") + hipGetErrorString(e));
   const double wall = std::chrono::duration<double, std::milli>(t1 - t0).count();
   std::vector<InstState> st(h->gb.n_inst);
@@ -627,6 +641,9 @@ extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
       s.wall_ms = wall;
       for (int k = 0; k < 8; ++k) s.kern_ms[k] = kern[k];
       s.max_abs_d32_d64 = st[i].max_dd;
+      s.tile_bytes = st[i].tile_bytes;
+      s.tile_bytes_new = st[i].tile_bytes_new;
+      s.rounds_launched = launched;
     }
   }
   h->generated = true;

@@ -164,7 +164,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   hipEventRecord(ev[3], s);
   fill_u32(c.fill, 0u, n, s);
   fill_u32(queues + QUEUES_PER_KERNEL, 0u, QUEUES_PER_KERNEL, s);
-  fill_u32((uint32_t*)ctr, 0u, 4, s);  // the record pass repeats the count's tests
+  fill_u32((uint32_t*)ctr, 0u, 8, s);  // the record pass repeats the count's tests
   a.qctr = queues + QUEUES_PER_KERNEL;
   a.rowptr = c.rowptr;
   a.fill = c.fill;
@@ -176,7 +176,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   hipEventRecord(ev[4], s);
   launch_row_sort(c.rowptr, n, c.J, c.gap, c.gap64, c.img, s);
   hipEventRecord(ev[5], s);
-  HIPCHK(h, hipMemcpyAsync(h->cb_stats, ctr, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipMemcpyAsync(h->cb_stats, ctr, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s));
   HIPCHK(h, sync_stream(s, "contact record"));
   trace("contacts: recorded and sorted");
   HIPCHK(h, hipGetLastError());
@@ -241,7 +241,7 @@ extern "C" int nf_get_contact_gap64(nf_handle* h, double* gap64, uint64_t cap) {
 }
 
 extern "C" int nf_contacts_stats(nf_handle* h, uint64_t* pair_tests, uint64_t* rechecks,
-                                 double* max_dd, float* kernel_ms) {
+                                 double* max_dd, float* kernel_ms, uint64_t* tile_bytes) {
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   if (!h->cb.valid) return fail(h, NF_E_STATE, "no contacts computed");
   if (pair_tests) *pair_tests = h->cb_stats[0];
@@ -254,6 +254,7 @@ extern "C" int nf_contacts_stats(nf_handle* h, uint64_t* pair_tests, uint64_t* r
   }
   if (kernel_ms)
     for (int k = 0; k < 4; ++k) kernel_ms[k] = h->cb_ms[k];
+  if (tile_bytes) *tile_bytes = h->cb_stats[3];
   return NF_OK;
 }
 

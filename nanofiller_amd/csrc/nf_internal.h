@@ -43,6 +43,8 @@ struct InstState {
   uint32_t mprev, aprev, stall, overflow;
   unsigned long long candidates, proposals, pair_tests, rechecks, rej_old, rej_new;
   double max_dd;  // diagnostic max |d32 - d64|
+  unsigned long long tile_bytes;      // probe + tile bytes streamed by the reject-vs-accepted kernel
+  unsigned long long tile_bytes_new;  // same, reject-vs-same-round kernel
 };
 
 struct RoundInfo {
@@ -173,7 +175,7 @@ struct PairArgs {
   float gap32;
   float ew;
   float hl_max, R_max;
-  unsigned long long* ctr;  // [0] pair tests, [1] rechecks
+  unsigned long long* ctr;  // [0] pair tests, [1] rechecks, [2] max dd bits, [3] tile bytes
   int diag;                 // record max |d32-d64| over full tests
 };
 

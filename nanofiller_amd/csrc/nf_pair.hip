@@ -58,6 +58,7 @@ struct CellGeom {
 struct Acc {
   uint32_t tests, rech, kills;
   double maxdd;
+  unsigned long long bytes;  // wave-uniform: probe and tile bytes streamed
 };
 
 // Values the hardware must treat as wave-uniform: read from lane 0 into an
@@ -92,17 +93,21 @@ __device__ __forceinline__ void flush(const PairArgs& a, const CellGeom& g, Acc&
   } else {
     c0 = &a.ctr[0];
     c1 = &a.ctr[1];
-    c2 = &a.ctr[3];
+    c2 = &a.ctr[3];  // unused for contacts
     c3 = &a.ctr[2];
   }
   // every lane issues the atomics (lane 0 carries the value) so no branch
   // precedes the next cross-lane operation of the caller
   atomicAdd(c0, (unsigned long long)(l0 ? wt : 0u));
   atomicAdd(c1, (unsigned long long)(l0 ? wr : 0u));
-  atomicAdd(c2, (unsigned long long)(l0 ? wk : 0u));
+  if (GEN) atomicAdd(c2, (unsigned long long)(l0 ? wk : 0u));
   atomicMax(c3, (unsigned long long)(l0 ? __double_as_longlong(v) : 0ll));
+  unsigned long long* cb =
+      GEN ? (mode == PM_GEN_OLD ? &g.state->tile_bytes : &g.state->tile_bytes_new) : &a.ctr[3];
+  atomicAdd(cb, l0 ? acc.bytes : 0ull);
   acc.tests = acc.rech = acc.kills = 0;
   acc.maxdd = 0.0;
+  acc.bytes = 0;
 }
 
 template <int MODE, int PREC>
@@ -134,6 +139,7 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
     bool have = inrange;
     if (MODE == PM_GEN_NEW) have = have && (a.prAlive[pidx] != 0u);
     const float chunk_hl = unif(wave_max(have ? P1.w : 0.0f));
+    acc.bytes += (unsigned long long)G * 36ull;
     bool hit = false;
     const float rr_chunk = (chunk_hl + g.hl_max) * 1.000001f + g.R_max + CELL_MARGIN;
     const float rr_chunk2 = rr_chunk * rr_chunk;
@@ -165,6 +171,7 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
       const uint32_t nstart = uni(a.nbStart[nc]);
       for (uint32_t tb = 0; tb < n; tb += WAVE) {
         const uint32_t tn = min((uint32_t)WAVE, n - tb);
+        acc.bytes += (unsigned long long)tn * 36ull;
         // every lane loads (index clamped to the tile) and writes its slot;
         // slots >= tn are never read
         const uint32_t q = nstart + tb + min((uint32_t)lane, tn - 1u);
@@ -185,6 +192,15 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
               if (xj == X1) continue;
             }
             const float4 P2 = tP[j], D2 = tD[j];
+            // FP64 evaluation in the canonical orientation the decisions are
+            // defined by: generation puts the probe (proposal, or the later
+            // slot) first; contacts put the lower rod id first, with the
+            // periodic image applied to the higher id.  The closest-point
+            // solve is not bitwise symmetric under swapping the segments.
+            const bool swap = !GEN && (X1 > xj);
+            const float4 Pa = swap ? P2 : P1, Da = swap ? D2 : D1;
+            const float4 Pb = swap ? P1 : P2, Db = swap ? D1 : D2;
+            const double ux = swap ? -sx : sx, uy = swap ? -sy : sy, uz = swap ? -sz : sz;
             bool within;
             bool cut;
             double d2_64 = -1.0;
@@ -192,23 +208,23 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
             if (GEN)
               R64 = g.sf64 * ((double)D1.w + (double)D2.w) + g.sep64;
             else
-              R64 = ((double)D1.w + (double)D2.w) + a.gap64;
+              R64 = ((double)Da.w + (double)Db.w) + a.gap64;
             if (PREC == NF_PREC_FP64) {
-              d2_64 = pair_d2_f64(P1, D1, P2, D2, sx, sy, sz, R64, &within);
+              d2_64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &within);
               cut = GEN ? (d2_64 < R64 * R64) : (d2_64 <= R64 * R64);
             } else {
               const float R32 = GEN ? (g.sf32 * (D1.w + D2.w) + g.sep32) : ((D1.w + D2.w) + a.gap32);
               const F32Pair r = pair_f32(P1, D1, P2, D2, fx, fy, fz, R32, g.ew, GEN);
               if (r.decided < 0) {
                 ++acc.rech;
-                d2_64 = pair_d2_f64(P1, D1, P2, D2, sx, sy, sz, R64, &within);
+                d2_64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &within);
                 cut = GEN ? (d2_64 < R64 * R64) : (d2_64 <= R64 * R64);
               } else {
                 within = r.within != 0;
                 cut = r.decided != 0;
                 if (a.diag && within) {
                   bool w2;
-                  const double e64 = pair_d2_f64(P1, D1, P2, D2, sx, sy, sz, R64, &w2);
+                  const double e64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &w2);
                   const double dd = fabs(sqrt((double)r.d2) - sqrt(e64));
                   acc.maxdd = dd > acc.maxdd ? dd : acc.maxdd;
                 }
@@ -222,7 +238,7 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
             } else {
               if (d2_64 < 0.0) {
                 bool w2;
-                d2_64 = pair_d2_f64(P1, D1, P2, D2, sx, sy, sz, R64, &w2);
+                d2_64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &w2);
               }
               const uint32_t I = X1 < xj ? X1 : xj;
               const uint32_t J = X1 < xj ? xj : X1;
@@ -233,7 +249,7 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
                 const uint32_t k = atomicAdd(&a.fill[I], 1u);
                 if (k >= a.rowptr[I + 1] - a.rowptr[I]) continue;  // host reports the mismatch
                 const uint32_t pos = a.rowptr[I] + k;
-                const double gv = sqrt_cr(d2_64) - ((double)D1.w + (double)D2.w);
+                const double gv = sqrt_cr(d2_64) - ((double)Da.w + (double)Db.w);
                 a.outJ[pos] = J;
                 a.outGap[pos] = (float)gv;
                 a.outGap64[pos] = gv;
@@ -275,7 +291,7 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_pair(PairArgs a) {
   const int xcd = blockIdx.x & 7;
   const bool l0 = threadIdx.x == 0;
   CellGeom g;
-  Acc acc = {0u, 0u, 0u, 0.0};
+  Acc acc = {0u, 0u, 0u, 0.0, 0ull};
   uint32_t cur_inst = 0xFFFFFFFFu;
   if (!GEN) {
     for (int d = 0; d < 3; ++d) {
