@@ -245,7 +245,8 @@ int nf_contacts_stats(nf_handle* h, uint64_t* pair_tests, uint64_t* rechecks, do
  * building block element-wise on the GPU so callers can check it against
  * the CPU.  which: 0 correctly rounded sqrt, 1 exp, 2 log, 3 division
  * (in holds 2n values, out[i] = in[2i]/in[2i+1]), 4 Philox-word-to-uniform
- * conversion (bits of in[i] = hi:lo words), 5 the device's own sqrt. */
+ * conversion (bits of in[i] = hi:lo words), 5 the device's own sqrt, 6 a
+ * plain copy (out[i] = in[i]; a known-size kernel for counter calibration). */
 int nf_selftest(int which, const double* in, double* out, uint64_t n);
 
 #ifdef __cplusplus

@@ -364,7 +364,7 @@ class Packing:
         _check(lib().nf_write_coo(self._h, prefix.encode()), self._h)
 
 
-SELFTEST = {"sqrt": 0, "exp": 1, "log": 2, "div": 3, "u01": 4, "hw_sqrt": 5}
+SELFTEST = {"sqrt": 0, "exp": 1, "log": 2, "div": 3, "u01": 4, "hw_sqrt": 5, "copy": 6}
 
 
 def selftest(which: str, x: np.ndarray) -> np.ndarray:

@@ -710,7 +710,7 @@ extern "C" int nf_set_rods(nf_handle* h, int inst, const float* rods, uint32_t n
 // diagnostics
 // ===========================================================================
 extern "C" int nf_selftest(int which, const double* in, double* out, uint64_t n) {
-  if (which < 0 || which > 5 || (n && (!in || !out))) return fail(nullptr, NF_E_INVALID, "bad selftest args");
+  if (which < 0 || which > 6 || (n && (!in || !out))) return fail(nullptr, NF_E_INVALID, "bad selftest args");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, NF_E_NODEVICE, "no HIP device");
   const uint64_t nin = which == 3 ? 2 * n : n;

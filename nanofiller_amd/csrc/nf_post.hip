@@ -272,6 +272,7 @@ __global__ void k_selftest(int which, const double* __restrict__ in, double* __r
         r = u01((uint32_t)(b >> 32), (uint32_t)b);
         break;
       }
+      case 6: r = in[i]; break;
       default: r = sqrt(in[i]); break;
     }
     out[i] = r;
