@@ -28,6 +28,7 @@ NF_LEN_CONST, NF_LEN_LOGNORMAL = 0, 1
 NF_ORT_ISO, NF_ORT_VMF = 0, 1
 NF_DEN_UNIFORM, NF_DEN_AGGLOM = 0, 1
 NF_PREC_FP64, NF_PREC_FP32_RECHECK = 0, 1
+NF_STATUS_RUNNING, NF_STATUS_TARGET, NF_STATUS_MAX_ROUNDS, NF_STATUS_STALLED, NF_STATUS_OVERFLOW = 0, 1, 2, 3, 4
 STATUS = {0: "running", 1: "target", 2: "max_rounds", 3: "stalled", 4: "overflow"}
 NF_OPT_GRAPH, NF_OPT_PROFILE, NF_OPT_DIAG, NF_OPT_CONTACT_PREC = 1, 2, 3, 4
 KERNEL_CLASSES = ["gen", "bin", "reject_old", "reject_new", "commit", "control", "contact", "other"]

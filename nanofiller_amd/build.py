@@ -95,6 +95,15 @@ def build(force: bool = False, jobs: int = 8) -> str:
         if r.returncode != 0:
             raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
         os.replace(tmp, LIB)
+    # host build of the pair predicates for the CPU test suite (test only)
+    harness = os.path.join(ROOT, "tests", "native", "band_harness.cpp")
+    hlib = os.path.join(ROOT, "tests", "native", "libnf_band.so")
+    if os.path.exists(harness) and (force or _newer(hlib, [harness, os.path.join(CSRC, "nf_common.h")])):
+        cmd = [hipcc(), "--offload-host-only", "-x", "hip", "-O2", "-fPIC", "-shared", "-std=c++17",
+               "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), harness, "-o", hlib]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"band harness build failed\n{' '.join(cmd)}\n{r.stderr}")
     cli_src = os.path.join(CSRC, "nf_cli.cpp")
     if os.path.exists(cli_src) and (force or _newer(CLI, [cli_src, LIB])):
         cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), cli_src,

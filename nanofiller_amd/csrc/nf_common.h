@@ -297,7 +297,7 @@ NF_HD F32Pair pair_f32(const float4 P1, const float4 D1, const float4 P2, const 
   const float wx = (P1.x - P2.x) - sx;
   const float wy = (P1.y - P2.y) - sy;
   const float wz = (P1.z - P2.z) - sz;
-  const float w2 = __fmaf_rn(wz, wz, __fmaf_rn(wy, wy, wx * wx));
+  const float w2 = fmaf(wz, wz, fmaf(wy, wy, wx * wx));
   const float rr = (P1.w + P2.w) * 1.000001f + R;
   const float rr2 = rr * rr;
   // reach band: w components err <= ew; rr differs from the FP64 reach by
@@ -321,18 +321,18 @@ NF_HD F32Pair pair_f32(const float4 P1, const float4 D1, const float4 P2, const 
   out.within = 1;
   const float h1x = P1.w * D1.x, h1y = P1.w * D1.y, h1z = P1.w * D1.z;
   const float h2x = P2.w * D2.x, h2y = P2.w * D2.y, h2z = P2.w * D2.z;
-  const float a = __fmaf_rn(h1z, h1z, __fmaf_rn(h1y, h1y, h1x * h1x));
-  const float e = __fmaf_rn(h2z, h2z, __fmaf_rn(h2y, h2y, h2x * h2x));
-  const float b = __fmaf_rn(h1z, h2z, __fmaf_rn(h1y, h2y, h1x * h2x));
-  const float c = __fmaf_rn(h1z, wz, __fmaf_rn(h1y, wy, h1x * wx));
-  const float f = __fmaf_rn(h2z, wz, __fmaf_rn(h2y, wy, h2x * wx));
-  const float denom = __fmaf_rn(a, e, -(b * b));
+  const float a = fmaf(h1z, h1z, fmaf(h1y, h1y, h1x * h1x));
+  const float e = fmaf(h2z, h2z, fmaf(h2y, h2y, h2x * h2x));
+  const float b = fmaf(h1z, h2z, fmaf(h1y, h2y, h1x * h2x));
+  const float c = fmaf(h1z, wz, fmaf(h1y, wy, h1x * wx));
+  const float f = fmaf(h2z, wz, fmaf(h2y, wy, h2x * wx));
+  const float denom = fmaf(a, e, -(b * b));
   float s = 0.0f;
   if (denom > 0.0f) {
-    s = __fmaf_rn(b, f, -(c * e)) / denom;
+    s = fmaf(b, f, -(c * e)) / denom;
     s = fminf(fmaxf(s, -1.0f), 1.0f);
   }
-  float t = __fmaf_rn(b, s, f) / e;
+  float t = fmaf(b, s, f) / e;
   if (t < -1.0f) {
     t = -1.0f;
     s = fminf(fmaxf((-b - c) / a, -1.0f), 1.0f);
@@ -340,10 +340,10 @@ NF_HD F32Pair pair_f32(const float4 P1, const float4 D1, const float4 P2, const 
     t = 1.0f;
     s = fminf(fmaxf((b - c) / a, -1.0f), 1.0f);
   }
-  const float vx = __fmaf_rn(-t, h2x, __fmaf_rn(s, h1x, wx));
-  const float vy = __fmaf_rn(-t, h2y, __fmaf_rn(s, h1y, wy));
-  const float vz = __fmaf_rn(-t, h2z, __fmaf_rn(s, h1z, wz));
-  const float d2 = __fmaf_rn(vz, vz, __fmaf_rn(vy, vy, vx * vx));
+  const float vx = fmaf(-t, h2x, fmaf(s, h1x, wx));
+  const float vy = fmaf(-t, h2y, fmaf(s, h1y, wy));
+  const float vz = fmaf(-t, h2z, fmaf(s, h1z, wz));
+  const float d2 = fmaf(vz, vz, fmaf(vy, vy, vx * vx));
   out.d2 = d2;
   // error bounds
   const float nh1 = P1.w * 1.000001f, nh2 = P2.w * 1.000001f;
@@ -351,12 +351,12 @@ NF_HD F32Pair pair_f32(const float4 P1, const float4 D1, const float4 P2, const 
   const float Ev = ew2 + U32 * (nh1 + nh2) + 4.1f * U32 * S;
   const float nv = sqrtf(d2) * 1.000001f + Ev;
   const float EF = 4.0f * U32 * d2 + (2.0f * nv + Ev) * Ev;
-  const float gs = __fmaf_rn(h1z, vz, __fmaf_rn(h1y, vy, h1x * vx));
-  const float gt = -__fmaf_rn(h2z, vz, __fmaf_rn(h2y, vy, h2x * vx));
+  const float gs = fmaf(h1z, vz, fmaf(h1y, vy, h1x * vx));
+  const float gt = -fmaf(h2z, vz, fmaf(h2y, vy, h2x * vx));
   const float Egs = 4.1f * U32 * nh1 * nv + 1.01f * nh1 * Ev;
   const float Egt = 4.1f * U32 * nh2 * nv + 1.01f * nh2 * Ev;
   const float ags = fabsf(gs), agt = fabsf(gt);
-  const float gap = 2.0f * (__fmaf_rn(gs, s, ags) + __fmaf_rn(gt, t, agt));
+  const float gap = 2.0f * (fmaf(gs, s, ags) + fmaf(gt, t, agt));
   const float Egap = 4.0f * (Egs + Egt) + 8.0f * U32 * (ags + agt) * 2.0f;
   const float e64 = EPS64_REL * S * S;
   const float band_lo = (EF + gap + Egap + e64) * 1.0001f + 4.0f * U32 * d2;

@@ -49,10 +49,14 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   }
   const double Rmax = ((double)amax + (double)amax) + gap;
   const double reach = ((double)hlmax + (double)hlmax) * REACH_PAD + Rmax;
+  // cell edge about one typical pair reach; the sorted stencil reaches the
+  // longest rods and per-cell length bounds prune the rest
+  double edge = (p.len_law == NF_LEN_LOGNORMAL ? p.len : (double)hlmax * 2.0) * REACH_PAD + Rmax;
+  if (edge > reach) edge = reach;
   uint64_t ncells = 1;
   double scale[3];
   for (int a = 0; a < 3; ++a) {
-    double nn = std::floor(p.box[a] / reach);
+    double nn = std::floor(p.box[a] / edge);
     if (nn < 1.0) nn = 1.0;
     if (nn > 1024.0) nn = 1024.0;
     c.ncell[a] = (int32_t)nn;

@@ -280,6 +280,7 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
 
   PairArgs a{};
   a.desc = b.desc;
+  a.ri = b.ri;
   a.st = b.st;
   a.cell_inst = b.cell_inst;
   a.ncells_total = (uint32_t)b.total_cells;

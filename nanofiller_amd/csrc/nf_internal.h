@@ -136,6 +136,7 @@ struct ContactBuffers {
 // Arguments of the cell-pair engine.
 struct PairArgs {
   const InstDesc* desc;
+  const RoundInfo* ri;  // generation: this round's plan (T == 0: no work)
   InstState* st;
   const uint32_t* cell_inst;
   uint32_t ncells_total;

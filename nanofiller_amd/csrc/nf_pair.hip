@@ -289,6 +289,8 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_pair(PairArgs a) {
   constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
   const uint32_t W = a.ncells_total;
   const int xcd = blockIdx.x & 7;
+  // rounds after every instance finished plan no candidates: nothing to do
+  if (GEN && uni(a.ri->T) == 0u) return;
   const bool l0 = threadIdx.x == 0;
   CellGeom g;
   Acc acc = {0u, 0u, 0u, 0.0, 0ull};
@@ -316,6 +318,9 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_pair(PairArgs a) {
     const int q = (xcd + k) & 7;
     const uint32_t qb = (uint32_t)(((uint64_t)W * (uint32_t)q) / 8u);
     const uint32_t qe = (uint32_t)(((uint64_t)W * (uint32_t)(q + 1)) / 8u);
+    // a plain read first: an exhausted queue costs no atomic
+    if (uni(__hip_atomic_load(&a.qctr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= qe - qb)
+      continue;
     for (;;) {
       // all lanes take part in the atomic (lane 0 adds 1), lane 0's old
       // value is the work item

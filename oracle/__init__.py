@@ -57,6 +57,8 @@ def lib() -> C.CDLL:
             ("orc_vmf_basis", None, [C.c_void_p, C.c_void_p, C.c_void_p]),
             ("orc_candidate", C.c_int, [P(Params), C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
             ("orc_pair_d2", C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, P(C.c_int)]),
+            ("orc_pair_batch", None, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                      C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
             ("orc_generate", C.c_int, [P(Params), C.c_void_p, P(GenStats), C.c_void_p, C.c_int]),
             ("orc_contacts", C.c_int64, [P(Params), C.c_void_p, C.c_uint32, C.c_double, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
@@ -91,6 +93,20 @@ def pair_d2(ri, rj, shift, R) -> tuple[float, bool]:
     w = C.c_int()
     d2 = lib().orc_pair_d2(a.ctypes.data, b.ctypes.data, s.ctypes.data, float(R), C.byref(w))
     return d2, bool(w.value)
+
+
+def pair_batch(ra, rb, shifts, contact: bool, sep_or_gap: float, soft_frac: float = 1.0):
+    ra = np.ascontiguousarray(ra, np.float32)
+    rb = np.ascontiguousarray(rb, np.float32)
+    sh = np.ascontiguousarray(shifts, np.float64)
+    n = ra.shape[0]
+    d2 = np.zeros(n, np.float64)
+    w = np.zeros(n, np.int32)
+    cut = np.zeros(n, np.int32)
+    lib().orc_pair_batch(n, ra.ctypes.data, rb.ctypes.data, sh.ctypes.data, int(contact),
+                         float(sep_or_gap), float(soft_frac), d2.ctypes.data, w.ctypes.data,
+                         cut.ctypes.data)
+    return d2, w.astype(bool), cut.astype(bool)
 
 
 def candidate(p, rnd: int, slot: int, agg=None):

@@ -233,6 +233,23 @@ double orc_pair_d2(const float* ri, const float* rj, const double shift[3], doub
   return (vx * vx + vy * vy) + vz * vz;
 }
 
+/* Batch form for the tests: cut[i] = within && (contact ? d2 <= R^2 : d2 < R^2). */
+void orc_pair_batch(int n, const float* ra, const float* rb, const double* sh, int contact,
+                    double sep_or_gap, double soft_frac, double* d2out, int* within_out,
+                    int* cut_out) {
+  for (int i = 0; i < n; ++i) {
+    const float* a = ra + 8 * (size_t)i;
+    const float* b = rb + 8 * (size_t)i;
+    double R = contact ? ((double)a[7] + (double)b[7]) + sep_or_gap
+                       : soft_frac * ((double)a[7] + (double)b[7]) + sep_or_gap;
+    int w;
+    double d2 = orc_pair_d2(a, b, &sh[3 * (size_t)i], R, &w);
+    d2out[i] = d2;
+    within_out[i] = w;
+    cut_out[i] = w && (contact ? (d2 <= R * R) : (d2 < R * R));
+  }
+}
+
 /* Hard-core distance: soft_frac*(a1+a2)+sep (inc/intersection_dev.h:78,
  * inc/intersect/intersect_cylinder_capped.h:157-159 without the +1e-6). */
 static double hard_core(const nf_params* p, const float* ri, const float* rj) {

@@ -51,6 +51,11 @@ int orc_candidate(const nf_params* p, const double* agg, uint32_t round, uint32_
 double orc_pair_d2(const float* ri, const float* rj, const double shift[3], double R,
                    int* within);
 
+/* Batch of independent pair evaluations (tests). */
+void orc_pair_batch(int n, const float* ra, const float* rb, const double* sh, int contact,
+                    double sep_or_gap, double soft_frac, double* d2out, int* within_out,
+                    int* cut_out);
+
 typedef struct orc_gen_stats {
   uint32_t n_accepted, rounds;
   int32_t status;
