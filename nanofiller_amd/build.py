@@ -28,6 +28,7 @@ SOURCES = [
     "nf_gen.hip",
     "nf_pair.hip",
     "nf_post.hip",
+    "nf_net.hip",
     "nf_api.cpp",
     "nf_contacts.cpp",
     "nf_network.cpp",

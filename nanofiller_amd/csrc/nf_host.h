@@ -28,6 +28,7 @@ struct nf_handle {
   unsigned long long cb_stats[4] = {0, 0, 0, 0};  // contact pair tests, rechecks, max dd bits
   float cb_ms[4] = {0, 0, 0, 0};                  // contact kernel times: bin, count, record, sort
   nf_network_info net_info = {};
+  float net_ms = 0.0f;  // device time of the last nf_network call
 };
 
 extern std::string g_last_error;

@@ -208,8 +208,60 @@ void launch_hist(const float4* P, const float4* D, uint32_t n, const uint32_t* r
                  const double* edges_t, int nb, int npb, int ntb, int axis, double L,
                  const double* mu, unsigned long long* out, hipStream_t s);
 void launch_selftest(int which, const double* in, double* out, uint64_t n, hipStream_t s);
-void launch_network(ContactBuffers& c, const float4* P, const float4* D, uint32_t n, int axis,
-                    double L, double gcut, double lam, double g0, uint32_t* info_dev,
-                    hipStream_t s, std::string* err);
+
+// ---- network extraction (nf_net.hip) ----
+enum NetSummary {
+  SUM_EDGES = 0,
+  SUM_COMPONENTS = 1,
+  SUM_LARGEST = 2,
+  SUM_PERCOLATING = 3,
+  SUM_NNZ = 4,
+  SUM_SOURCE = 5,
+  SUM_SINK = 6,
+  NET_SUMMARY = 8
+};
+
+struct NetArgs {
+  // contacts: CSR over the lower id, J > row, image applies to J
+  const uint32_t* rowptr;
+  const uint32_t* J;
+  const int32_t* img;
+  const double* gap64;
+  const float4* P;
+  const float4* D;
+  uint32_t n;
+  int axis;
+  double L, gcut, lam, g0;
+  // work arrays
+  uint32_t* par;       // n: union-find parents, final component labels
+  uint32_t* lowcnt;    // n + 1
+  uint32_t* lowstart;  // n + 1
+  uint32_t* lowfill;   // n
+  uint32_t* lcol;      // lower-triangle entries (<= contacts)
+  int32_t* limg;
+  double* lg;
+  uint32_t* fsrc;      // n + 1: electrode coupling flags
+  uint32_t* fsnk;
+  uint32_t* psrc;      // n + 1: their exclusive scans ([n] = count)
+  uint32_t* psnk;
+  double* gsrc;        // n: electrode conductances
+  double* gsnk;
+  uint32_t* csize;     // n: component sizes (at the root)
+  uint32_t* cflag;     // n: component electrode flags (at the root)
+  uint32_t* rowcnt;    // n + 2
+  uint32_t* nrow;      // n + 3: CSR row pointers of the N + 2 node matrix
+  unsigned long long* summary;  // NET_SUMMARY entries
+  uint32_t* tmp;       // scan scratch
+  // outputs of the fill phase
+  uint32_t* ncol;
+  double* nval;
+  double* cg;    // n: compacted electrode conductances
+  double* bsum;  // n / 1024 + 1: their block sums
+};
+
+// Components, couplings, lower lists, row counts, row pointers, summary.
+void launch_network_build(const NetArgs& a, hipStream_t s);
+// Row entries (ncol/nval sized from summary[SUM_NNZ]).
+void launch_network_fill(const NetArgs& a, uint32_t n_src, uint32_t n_snk, hipStream_t s);
 
 }  // namespace nf
