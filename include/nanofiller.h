@@ -166,6 +166,11 @@ int nf_contacts(nf_handle* h, double gap, uint64_t* n_contacts);
 int nf_get_contacts(nf_handle* h, uint32_t* row_ptr, uint32_t* col, float* gap,
                     int32_t* image, uint64_t cap);
 
+/* FP64 surface gaps of the same contacts, in the same order (the value the
+ * FP32 gap is rounded from; the conductances of nf_network use it).  The
+ * reference keeps only an FP32 distance (src/get_contact_data.cu:28-68). */
+int nf_get_contact_gap64(nf_handle* h, double* gap64, uint64_t cap);
+
 /* ---- histograms: statKernel_ANGL / statKernel_MUT / centre profile
  *      (src/statistics_collection.cu:640-861, inc/statKernel_ANGL.h:13-76,
  *      inc/statKernel_MUT.h:26-131) ---------------------------------------- */

@@ -1,0 +1,49 @@
+"""The oracle reproduces the committed golden fixtures bit for bit."""
+import numpy as np
+import pytest
+
+import oracle
+from tests import golden_cases as G
+
+
+@pytest.mark.parametrize("name", G.NAMES)
+def test_oracle_matches_golden(name):
+    p, meta, g = G.load(name)
+    rods, st, acc = oracle.generate(p, threads=2)
+    np.testing.assert_array_equal(rods.view(np.uint32), g["rods_bits"])
+    np.testing.assert_array_equal(acc, g["round_accepts"])
+    assert st == meta["stats"]
+    c = oracle.contacts(p, rods, p.contact_gap)
+    for k in ("I", "J", "image"):
+        np.testing.assert_array_equal(c[k], g[k])
+    np.testing.assert_array_equal(c["gap"].view(np.uint32), g["gap_bits"])
+    np.testing.assert_array_equal(c["gap64"].view(np.uint64), g["gap64_bits"])
+    assert c["pair_tests"] == meta["pair_tests"]
+    hs = meta["hist"]
+    h = oracle.histograms(p, rods, c["I"], c["J"], bins=hs["bins"], centre_axis=hs["centre_axis"],
+                          pol_axis=tuple(hs["pol_axis"]), phi_bins=hs["phi_bins"],
+                          theta_bins=hs["theta_bins"])
+    for k in ("centre", "polar", "mutual", "phitheta"):
+        np.testing.assert_array_equal(h[k], g["h_" + k])
+    info, net = oracle.network(p, rods, c["I"], c["J"], c["gap64"], c["image"])
+    assert info == meta["network"]
+    for k in ("labels", "row_ptr", "col"):
+        np.testing.assert_array_equal(net[k], g[k])
+    np.testing.assert_array_equal(net["val"].view(np.uint64), g["val_bits"])
+
+
+def test_golden_fixtures_are_consistent():
+    """Independent sanity of the stored data: counts, ordering, invariants."""
+    for name in G.NAMES:
+        p, meta, g = G.load(name)
+        n = len(g["rods_bits"])
+        assert n == meta["stats"]["n_accepted"] == p.n_rods
+        assert g["round_accepts"].sum() == n
+        I, J = g["I"].astype(np.int64), g["J"].astype(np.int64)
+        assert np.all(I < J)
+        key = (I * n + J) * (1 << 24) + (g["image"].astype(np.int64) & 0xFFFFFF)
+        assert np.all(np.diff(I) >= 0)
+        assert len(np.unique(key)) == len(key)
+        assert g["h_centre"].sum() == n and g["h_polar"].sum() == n
+        assert g["h_mutual"].sum() == len(I)
+        assert g["row_ptr"][-1] == meta["network"]["nnz"] == len(g["col"])
