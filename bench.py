@@ -1,17 +1,28 @@
 """Benchmark: accepted rods/s and pair-tests/s of the MI355X generator.
 
-Workload (one GPU): 500,000 rods in a 20 um periodic cube, log-normal
-lengths (median 1 um, sigma 0.3, clipped to [0.3, 3] um), radius 5 nm,
-density = background + 8 Gaussian agglomerates (sigma 1.5 um, peak 3x),
+Headline workload (one GPU): 500,000 rods in a 20 um periodic cube,
+log-normal lengths (median 1 um, sigma 0.3, clipped to [0.3, 3] um), radius
+5 nm, density = background + 8 Gaussian agglomerates (sigma 1.5 um, peak 3x),
 vMF(+z, kappa 5) orientations, hard core 10 nm, Philox seed 2.  A step is one
 complete generation of that packing (every insert/reject round, from the
 first candidate to the final packing resident in HBM) by the FP32 path with
-FP64 re-check band.  The same packing generated by the plain FP64 path, the
-identity of the two packings and of their contact lists, the roofline of the
-dominant kernel and the CPU oracle's rate on a bounded sample are reported
-on the same JSON line.
+FP64 re-check band; `value` is accepted rods/s over the timed steps.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+The same JSON line carries:
+  * fp64_plain: the same packing by the plain FP64 path; identity of the two
+    packings and of their contact lists;
+  * roofline: the reject-vs-accepted kernel, algorithmic tile bytes per launch
+    over its HIP-event time per launch (one extra, profiled generation);
+  * batch: 256 independent packings of uneven size (2k-200k rods) in one
+    batched FP32 + recheck generation, against the same 256 packings made one
+    at a time by the plain FP64 path, with identical packings;
+  * contacts: 4,000,000 rods in a 50 um cube: contact list (FP32 + recheck,
+    and FP64) and three 64-bin int64 histograms, contact-kernel roofline;
+  * network: 8,000,000 rods in a 40 um cube: contacts, components and the
+    FP64 Kirchhoff matrix;
+  * cpu_baseline: the C oracle on a bounded sample of the headline workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs main,batch,contacts,network]
 """
 from __future__ import annotations
 
@@ -27,40 +38,199 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 import nanofiller_amd as nf  # noqa: E402
+from nanofiller_amd import configs  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "accepted rods/s & pair-tests/s, FP32+recheck vs FP64, identical contact lists"
-
-
-def cfg2(precision, **kw):
-    base = dict(box=(20, 20, 20), n_rods=500000, len=1.0, len_law=nf.NF_LEN_LOGNORMAL,
-                len_sigma=0.3, len_min=0.3, len_max=3.0, radius=0.005, seed=2,
-                den_law=nf.NF_DEN_AGGLOM, n_agg=8, agg_sigma=1.5, agg_peak=3.0,
-                ort_law=nf.NF_ORT_VMF, kappa=5.0, axis=(0, 0, 1), contact_gap=0.002,
-                precision=precision)
-    base.update(kw)
-    return nf.normalize(nf.default_params(**base))
+K_OLD = nf.KERNEL_CLASSES.index("reject_old")
+K_NEW = nf.KERNEL_CLASSES.index("reject_new")
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def timed_generation(p, steps: int, warmup: int, profile: bool):
-    opts = {nf.NF_OPT_PROFILE: 1} if profile else {}
-    with nf.Packing(p, opts) as pk:
+def timed_generation(p, steps: int, warmup: int, options=None):
+    with nf.Packing(p, options or {}) as pk:
         for i in range(warmup):
             pk.generate()
-            log(f"warmup {i + 1}/{warmup} done")
-        t0 = time.perf_counter()
         sts = []
+        t0 = time.perf_counter()
         for i in range(steps):
             sts.append(pk.generate()[0])
-        t1 = time.perf_counter()
+        dt = time.perf_counter() - t0
         rods = pk.rods()
         cont = pk.contacts(p.contact_gap)
-        cst = pk.contact_stats()
-    return t1 - t0, sts, rods, cont, cst
+    return dt, sts, rods, cont
+
+
+def same_contacts(a, b) -> bool:
+    return (np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col, b.col)
+            and np.array_equal(a.image, b.image) and np.array_equal(a.gap.view(np.uint32),
+                                                                      b.gap.view(np.uint32)))
+
+
+def leg_main(a, out: dict) -> None:
+    p32 = configs.cfg2(nf.NF_PREC_FP32_RECHECK)
+    log("headline: FP32 + FP64 re-check")
+    dt, sts, rods32, c32 = timed_generation(p32, a.steps, a.warmup)
+    st = sts[-1]
+    rods_s = st.n_accepted * a.steps / dt
+    pair_s = sum(s.pair_tests for s in sts) / dt
+    ms_step = dt / a.steps * 1e3
+    log(f"FP32: {rods_s:.4e} rods/s, {ms_step:.2f} ms/step, {st.rounds} rounds")
+    # one profiled generation for the per-kernel times and the roofline
+    with nf.Packing(p32, {nf.NF_OPT_PROFILE: 1}) as pk:
+        sp = pk.generate()[0]
+    launches = max(sp.rounds_launched, 1)
+    k_old = sp.kern_ms[K_OLD]
+    achieved = (sp.tile_bytes / launches) / (k_old / launches * 1e-3) / 1e9 if k_old > 0 else 0.0
+    log(f"profiled: reject_old {k_old:.3f} ms over {launches} launches, "
+        f"reject_new {sp.kern_ms[K_NEW]:.3f} ms")
+    out.update(value=rods_s, ms_per_step=ms_step, pair_s=pair_s, n_rods=p32.n_rods)
+    out["roofline"] = {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "kernel": "k_pair<reject vs accepted, FP32+recheck>",
+        "bytes_per_launch": sp.tile_bytes / launches, "avg_launch_ms": k_old / launches,
+        "launches": launches,
+    }
+    out["extra"]["fp32_recheck"] = {
+        "accepted_rods_per_s": rods_s, "pair_tests_per_s": pair_s, "ms_per_step": ms_step,
+        "rounds": st.rounds, "candidates": st.candidates, "proposals": st.proposals,
+        "pair_tests": st.pair_tests, "rechecks": st.rechecks,
+        "recheck_fraction": st.rechecks / max(st.pair_tests, 1), "status": nf.STATUS[st.status],
+        "profiled_kernel_ms": {k: sp.kern_ms[i] for i, k in enumerate(nf.KERNEL_CLASSES)},
+        "profiled_wall_ms": sp.wall_ms,
+    }
+    if a.no_fp64:
+        return
+    p64 = configs.cfg2(nf.NF_PREC_FP64)
+    log("headline: plain FP64")
+    steps64 = max(1, min(a.steps, 3))
+    dt64, sts64, rods64, c64 = timed_generation(p64, steps64, 1)
+    rods64_s = sts64[-1].n_accepted * steps64 / dt64
+    same_rods = rods64.shape == rods32.shape and np.array_equal(rods64.view(np.uint32),
+                                                                  rods32.view(np.uint32))
+    same_c = same_contacts(c64, c32)
+    out["extra"]["fp64_plain"] = {
+        "accepted_rods_per_s": rods64_s,
+        "pair_tests_per_s": sum(s.pair_tests for s in sts64) / dt64,
+        "ms_per_step": dt64 / steps64 * 1e3,
+        "speedup_fp32_recheck_over_fp64": rods_s / rods64_s,
+        "identical_packing": bool(same_rods), "identical_contact_lists": bool(same_c),
+        "contacts": int(c32.n),
+    }
+    log(f"FP64: {rods64_s:.4e} rods/s; identical packing {same_rods}, contacts {same_c}")
+
+
+def leg_batch(a, out: dict) -> None:
+    ps32 = configs.cfg3(nf.NF_PREC_FP32_RECHECK)
+    ps64 = configs.cfg3(nf.NF_PREC_FP64)
+    total = sum(p.n_rods for p in ps32)
+    log(f"batch: {len(ps32)} packings, {total} rods, batched FP32 + re-check")
+    with nf.Packing(ps32) as pk:
+        pk.generate()  # warm-up
+        t0 = time.perf_counter()
+        sts = pk.generate()
+        dt = time.perf_counter() - t0
+        rods32 = [pk.rods(i) for i in range(len(ps32))]
+    acc = sum(s.n_accepted for s in sts)
+    b_rate = acc / dt
+    log(f"batched: {b_rate:.4e} rods/s ({dt * 1e3:.1f} ms, max rounds {max(s.rounds for s in sts)})")
+    # the plain FP64 single-instance path, one packing at a time
+    with nf.Packing(ps64[0]) as pk:
+        pk.generate()  # warm-up
+    t_seq = 0.0
+    acc64 = 0
+    same = True
+    for i, p in enumerate(ps64):
+        with nf.Packing(p) as pk:
+            t0 = time.perf_counter()
+            s = pk.generate()[0]
+            t_seq += time.perf_counter() - t0
+            r = pk.rods()
+        acc64 += s.n_accepted
+        same = same and r.shape == rods32[i].shape and np.array_equal(r.view(np.uint32),
+                                                                      rods32[i].view(np.uint32))
+    s_rate = acc64 / t_seq
+    log(f"sequential FP64: {s_rate:.4e} rods/s ({t_seq * 1e3:.1f} ms); identical packings {same}")
+    out["extra"]["batch"] = {
+        "workload": f"{len(ps32)} packings, N log-uniform 2k-200k (sum {total}), aspect 20-500, "
+                    "vf 0.1-2% (capped at 1.5/aspect), half isotropic / half vMF kappa 1-20",
+        "batched_fp32_recheck_rods_per_s": b_rate, "batched_ms": dt * 1e3,
+        "sequential_fp64_rods_per_s": s_rate, "sequential_fp64_ms": t_seq * 1e3,
+        "speedup_batched_over_fp64_single": b_rate / s_rate,
+        "identical_packings": bool(same), "accepted": acc, "target": total,
+        "pair_tests_per_s": sum(s.pair_tests for s in sts) / dt,
+        "max_rounds": max(s.rounds for s in sts),
+        "statuses": sorted({nf.STATUS[s.status] for s in sts}),
+    }
+
+
+def leg_contacts(a, out: dict) -> None:
+    p = configs.cfg4(nf.NF_PREC_FP32_RECHECK)
+    log("contacts: 4M rods, generate")
+    with nf.Packing(p) as pk:
+        t0 = time.perf_counter()
+        st = pk.generate()[0]
+        t_gen = time.perf_counter() - t0
+        pk.contacts(p.contact_gap)  # warm-up
+        t0 = time.perf_counter()
+        c32 = pk.contacts(p.contact_gap)
+        t_c = time.perf_counter() - t0
+        cs = pk.contact_stats()
+        t0 = time.perf_counter()
+        h = pk.histograms(bins=64, centre_axis=0, pol_axis=(1, 0, 0))
+        t_h = time.perf_counter() - t0
+        pk.set_option(nf.NF_OPT_CONTACT_PREC, nf.NF_PREC_FP64)
+        t0 = time.perf_counter()
+        c64 = pk.contacts(p.contact_gap)
+        t_c64 = time.perf_counter() - t0
+        cs64 = pk.contact_stats()
+    km = cs["kernel_ms"]
+    bytes_pass = cs["tile_bytes"]  # probe and neighbour bytes streamed by the pair pass
+    achieved = bytes_pass / (km["pairs"] * 1e-3) / 1e9 if km["pairs"] > 0 else 0.0
+    log(f"contacts: {c32.n} pairs, pair pass {km['pairs']:.3f} ms, CSR {km['csr']:.3f} ms, "
+        f"{achieved:.0f} GB/s; FP64 pair pass {cs64['kernel_ms']['pairs']:.3f} ms")
+    out["extra"]["contacts"] = {
+        "workload": "4,000,000 rods, 50 um cube, l 1 um, d 10 nm, uniform, vMF(+x, 2), gap <= 2 nm",
+        "generate_ms": t_gen * 1e3, "generate_rods_per_s": st.n_accepted / t_gen,
+        "contacts": int(c32.n), "contacts_call_ms": t_c * 1e3, "kernel_ms": km,
+        "pair_tests": cs["pair_tests"], "rechecks": cs["rechecks"],
+        "pair_kernel_roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                                  "tile_bytes_per_pass": bytes_pass},
+        "fp64_contacts_call_ms": t_c64 * 1e3, "fp64_kernel_ms": cs64["kernel_ms"],
+        "identical_fp32_fp64_lists": bool(same_contacts(c32, c64)),
+        "histograms_ms": t_h * 1e3,
+        "histogram_totals": {k: int(v.sum()) for k, v in h.items()},
+    }
+
+
+def leg_network(a, out: dict) -> None:
+    p = configs.cfg5(nf.NF_PREC_FP32_RECHECK)
+    log("network: 8M rods, generate")
+    with nf.Packing(p) as pk:
+        t0 = time.perf_counter()
+        st = pk.generate()[0]
+        t_gen = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        c = pk.contacts(p.contact_gap)
+        t_c = time.perf_counter() - t0
+        pk.network()  # warm-up
+        t0 = time.perf_counter()
+        info, _ = pk.network()
+        t_n = time.perf_counter() - t0
+    log(f"network: {c.n} contacts, {info['n_components']} components, largest {info['largest']}, "
+        f"nnz {info['nnz']}, {t_n * 1e3:.1f} ms")
+    out["extra"]["network"] = {
+        "workload": "8,000,000 rods, 40 um cube, l 1 um, d 10 nm, 16 agglomerates, vMF(+x, 3), "
+                    "gap <= 2 nm, electrodes x = 0 / 40 um, g0 exp(-gap / 0.1 nm)",
+        "generate_ms": t_gen * 1e3, "generate_rods_per_s": st.n_accepted / t_gen,
+        "rounds": st.rounds, "contacts": int(c.n), "contacts_call_ms": t_c * 1e3,
+        "network_call_ms": t_n * 1e3, "info": info,
+    }
 
 
 def cpu_baseline(threads: int):
@@ -68,7 +238,7 @@ def cpu_baseline(threads: int):
     of the volume and the same rod density (62,500 rods in a 10 um cube)."""
     import oracle
 
-    p = cfg2(nf.NF_PREC_FP64, box=(10, 10, 10), n_rods=62500)
+    p = configs.cfg2(nf.NF_PREC_FP64, box=(10, 10, 10), n_rods=62500)
     t0 = time.perf_counter()
     rods, st, _ = oracle.generate(p, threads=threads)
     dt = time.perf_counter() - t0
@@ -83,6 +253,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--legs", default="main,batch,contacts,network")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison leg")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -93,76 +264,31 @@ def main() -> int:
               "MI355X (multi-GPU sharding is a design note only); run with --gpus 1",
               file=sys.stderr)
         return 2
-
+    legs = set(a.legs.split(","))
     if a.calibrate_copy:
         x = np.arange(1 << 23, dtype=np.float64)
         assert np.array_equal(nf.selftest("copy", x), x)
         log("copy kernel: 67108864 bytes read, 67108864 bytes written")
-    p32 = cfg2(nf.NF_PREC_FP32_RECHECK)
-    log("FP32 + FP64 re-check leg")
-    dt, sts, rods32, c32, cst32 = timed_generation(p32, a.steps, a.warmup, profile=True)
-    st = sts[-1]
-    n_acc = st.n_accepted
-    rods_s = n_acc * a.steps / dt
-    pair_s = sum(s.pair_tests for s in sts) / dt
-    ms_step = dt / a.steps * 1e3
-    # dominant kernel: reject vs accepted rods
-    k_old = sum(s.kern_ms[nf.KERNEL_CLASSES.index("reject_old")] for s in sts) / a.steps
-    k_new = sum(s.kern_ms[nf.KERNEL_CLASSES.index("reject_new")] for s in sts) / a.steps
-    launches = st.rounds_launched  # the rocprof dispatch count of this kernel per step
-    bytes_old = st.tile_bytes
-    achieved = (bytes_old / launches) / (k_old / launches * 1e-3) / 1e9 if k_old > 0 else 0.0
-    log(f"FP32: {rods_s:.4e} rods/s, {ms_step:.2f} ms/step, {st.rounds} rounds, "
-        f"reject_old {k_old:.3f} ms/step, reject_new {k_new:.3f} ms/step")
-
-    extra = {
-        "fp32_recheck": {
-            "accepted_rods_per_s": rods_s, "pair_tests_per_s": pair_s, "ms_per_step": ms_step,
-            "rounds": st.rounds, "candidates": st.candidates, "proposals": st.proposals,
-            "pair_tests": st.pair_tests, "rechecks": st.rechecks,
-            "recheck_fraction": st.rechecks / max(st.pair_tests, 1),
-            "status": nf.STATUS[st.status],
-            "kernel_ms_per_step": {k: sum(s.kern_ms[i] for s in sts) / a.steps
-                                   for i, k in enumerate(nf.KERNEL_CLASSES)},
-        }
-    }
-    if not a.no_fp64:
-        p64 = cfg2(nf.NF_PREC_FP64)
-        log("plain FP64 leg")
-        steps64 = max(1, min(a.steps, 2))
-        dt64, sts64, rods64, c64, _ = timed_generation(p64, steps64, 1, profile=False)
-        rods64_s = sts64[-1].n_accepted * steps64 / dt64
-        same_rods = rods64.shape == rods32.shape and np.array_equal(rods64.view(np.uint32),
-                                                                      rods32.view(np.uint32))
-        same_contacts = (np.array_equal(c64.row_ptr, c32.row_ptr) and np.array_equal(c64.col, c32.col)
-                         and np.array_equal(c64.image, c32.image)
-                         and np.array_equal(c64.gap.view(np.uint32), c32.gap.view(np.uint32)))
-        extra["fp64_plain"] = {
-            "accepted_rods_per_s": rods64_s,
-            "pair_tests_per_s": sum(s.pair_tests for s in sts64) / dt64,
-            "ms_per_step": dt64 / steps64 * 1e3,
-            "speedup_fp32_recheck_over_fp64": rods_s / rods64_s,
-            "identical_packing": bool(same_rods),
-            "identical_contact_lists": bool(same_contacts),
-            "contacts": int(c32.n),
-        }
-        log(f"FP64: {rods64_s:.4e} rods/s; identical packing {same_rods}, contacts {same_contacts}")
-
+    res = {"extra": {}}
+    if "main" in legs:
+        leg_main(a, res)
+    for name, fn in (("batch", leg_batch), ("contacts", leg_contacts), ("network", leg_network)):
+        if name in legs:
+            fn(a, res)
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and "main" in legs:
         threads = min(16, os.cpu_count() or 1)
         log(f"CPU oracle baseline on {threads} threads")
         cpu = cpu_baseline(threads)
         log(f"CPU: {cpu['value']:.4e} rods/s")
-
     out = {
         "metric": METRIC,
-        "value": rods_s,
+        "value": res.get("value"),
         "unit": "rods/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": res.get("ms_per_step"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -172,19 +298,13 @@ def main() -> int:
             "workload": "500,000 rods, 20 um periodic cube, log-normal L (median 1 um, sigma 0.3, "
                         "[0.3, 3] um), a = 5 nm, 8 Gaussian agglomerates (sigma 1.5 um, 3x peak), "
                         "vMF(+z, kappa 5), hard core 10 nm, contacts <= 2 nm",
-            "n_rods": p32.n_rods, "box_um": list(p32.box), "seed": p32.seed,
-            "pair_tests_per_s": pair_s,
+            "n_rods": res.get("n_rods"), "box_um": [20.0, 20.0, 20.0], "seed": 2,
+            "pair_tests_per_s": res.get("pair_s"),
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_pair<reject vs accepted, FP32+recheck>",
-            "bytes_per_launch": bytes_old / max(launches, 1),
-            "avg_launch_ms": k_old / max(launches, 1),
-        },
+        "roofline": res.get("roofline"),
         "cpu_baseline": cpu,
-        "extra": extra,
+        "extra": res["extra"],
     }
     print(json.dumps(out), flush=True)
     return 0

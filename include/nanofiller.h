@@ -74,7 +74,9 @@ typedef struct nf_params {
   uint32_t stall_rounds;/* stop after this many rounds with no acceptance  */
   uint32_t min_batch;   /* candidates per round, lower clamp               */
   uint32_t max_batch;   /* candidates per round, upper clamp (0: derived)  */
-  double cell_size;     /* requested generation cell edge (0: auto)        */
+  double cell_size;     /* generation cell edge; 0: auto (about one typical
+                           pair reach, smaller for long rods in small boxes);
+                           < 0: |cell_size| times the auto edge              */
   int32_t precision;    /* NF_PREC_*                                       */
   int32_t elec_axis;    /* network electrode axis 0/1/2 (x/y/z)            */
   double tunnel_lambda; /* tunnelling decay length (um), g = g0 exp(-gap/l)*/
@@ -241,8 +243,9 @@ int nf_set_option(nf_handle* h, int key, int64_t value);
 
 /* Statistics of the last nf_contacts call: capsule tests on pairs in reach,
  * FP64 rechecks, max |d32-d64| (NF_OPT_DIAG), HIP-event times in ms of the
- * four contact stages (bin, count, record, row sort; kernel_ms[4]) and the
- * probe + tile bytes the record pass streamed. */
+ * four contact stages (binning, the pair pass that finds and appends every
+ * contact, CSR build, row sort; kernel_ms[4]) and the probe + tile bytes the
+ * pair pass streamed. */
 int nf_contacts_stats(nf_handle* h, uint64_t* pair_tests, uint64_t* rechecks, double* max_dd,
                       float* kernel_ms, uint64_t* tile_bytes);
 

@@ -321,7 +321,7 @@ class Packing:
                self._h)
         return {"pair_tests": t.value, "rechecks": r.value, "max_abs_d32_d64": m.value,
                 "tile_bytes": tb.value,
-                "kernel_ms": {"bin": ms[0], "count": ms[1], "record": ms[2], "sort": ms[3]}}
+                "kernel_ms": {"bin": ms[0], "pairs": ms[1], "csr": ms[2], "sort": ms[3]}}
 
     def histograms(self, bins=64, centre_axis=0, pol_axis=(1.0, 0.0, 0.0), mutual=True,
                    phi_bins=0, theta_bins=0) -> dict:

@@ -133,6 +133,16 @@ extern "C" void nf_params_default(nf_params* p) {
   p->stream = 0;
 }
 
+double nf::default_cell_edge(double reach_typ, double density) {
+  double e = reach_typ;
+  if (density > 0.0 && density * e * e * e > 256.0) {
+    e = std::cbrt(32.0 / density);
+    if (e < 0.25 * reach_typ) e = 0.25 * reach_typ;
+    if (e > reach_typ) e = reach_typ;
+  }
+  return e;
+}
+
 static double excluded_volume(double L, double a) {
   const double D = 2.0 * a;
   const double PI = 3.14159265358979323846;
@@ -286,11 +296,15 @@ static int plan_instance(const nf_params& p, InstDesc& d, std::vector<double>& a
   const double Rmax = p.soft_frac * ((double)af + (double)af) + p.sep;
   d.R_max = (float)Rmax * 1.000001f;
   const double reach = ((double)d.hl_max + (double)d.hl_max) * REACH_PAD + Rmax;
-  // cell edge: requested, or about one typical reach
+  // cell edge: requested, or about one typical reach; when that would put
+  // more than 256 rods in an average cell (long rods, small boxes) the edge
+  // drops towards ~32 rods per cell, but not below a quarter of the reach
+  // (a negative cell_size scales that default edge)
   double target = p.cell_size;
+  const double V = p.box[0] * p.box[1] * p.box[2];
   if (!(target > 0.0)) {
-    const double ltyp = p.len;
-    target = ltyp * REACH_PAD + Rmax;
+    const double e = default_cell_edge(p.len * REACH_PAD + Rmax, (double)p.n_rods / V);
+    target = target < 0.0 ? -target * e : e;
   }
   uint64_t ncells = 1;
   for (int a = 0; a < 3; ++a) {
@@ -308,7 +322,6 @@ static int plan_instance(const nf_params& p, InstDesc& d, std::vector<double>& a
   stencil = build_stencil(d.ncell, p.box, reach, false);
   d.st_n = (uint32_t)stencil.size();
   // per-cell capacity for accepted rods
-  const double V = p.box[0] * p.box[1] * p.box[2];
   const double cellV = V / (double)ncells;
   const double peak = p.den_law == NF_DEN_AGGLOM ? p.agg_peak : 1.0;
   const double lam = (double)p.n_rods * cellV / V * peak;

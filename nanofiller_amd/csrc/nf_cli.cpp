@@ -1,0 +1,205 @@
+// nf_cli.cpp - command-line front end with the verbs of the reference's
+// main (src/NanoFiller.cpp:48-149):
+//
+//   nanofiller params.txt generate [write_vtk] [stat_phitheta]
+//              [stat_mutangle] [stat_contacts] [stat_conduct d [internal]]
+//
+// generate          insert/reject generation (cudaMain); prints the count and
+//                   the volume fraction reached
+// write_vtk         output.vtk (write_CNTvtk, src/IO.cpp:38-76)
+// stat_phitheta     rho.dat, 32 x 32 phi-theta counts (collect_PhiTheta_distr,
+//                   src/statistics_collection.cu:769-861)
+// stat_mutangle     gamma.dat, 32 bins of the mutual angle of contacting
+//                   pairs (collect_1D_statDistr_weightedFloat, :640-765)
+// stat_contacts     contacts per rod: mean and maximum (collect_CntctPerInc)
+// stat_conduct d    contacts, network along axis d, COO files internal.srt.coo
+//                   / source.elec.coo / sink.elec.coo (collect_cunduct_data,
+//                   src/conductivity.cu:140-423; "internal" is accepted)
+//
+// The reference reads output.bin when generate is absent; state files are
+// not part of this library, so generate must come first.  Verbs outside
+// the hot path (plot, expand, reduce, stat_vdw, stat_nematic, write_ort,
+// write_lmp, stat_exclsurf) are reported as unsupported.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <string>
+#include <vector>
+
+#include "nanofiller.h"
+
+static int die(nf_handle* h, const char* what, int rc) {
+  std::fprintf(stderr, "nanofiller: %s failed (%d): %s\n", what, rc, nf_last_error(h));
+  return 1;
+}
+
+static bool write_rho(const char* path, int nphi, int ntheta, const std::vector<int64_t>& v) {
+  std::ofstream f(path, std::ios::out | std::ios::trunc);
+  if (!f.is_open()) return false;
+  f << "rho = {";
+  for (int j = 0; j < ntheta; ++j) {
+    f << "\t{";
+    for (int i = 0; i < nphi; ++i) {
+      f << std::setw(12) << v[(size_t)i + (size_t)j * nphi];
+      if (i != nphi - 1) f << ",";
+    }
+    f << "}";
+    if (j != ntheta - 1) f << "," << std::endl << "\t";
+  }
+  f << "}";
+  return f.good();
+}
+
+static bool write_gamma(const char* path, const std::vector<int64_t>& v) {
+  std::ofstream f(path, std::ios::out | std::ios::trunc);
+  if (!f.is_open() || v.empty()) return false;
+  f << "gamma={\t";
+  for (size_t i = 0; i + 1 < v.size(); ++i) f << std::setw(10) << v[i] << "," << std::endl;
+  f << std::setw(10) << v.back() << "\t}" << std::endl;
+  return f.good();
+}
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    std::fprintf(stderr,
+                 "usage: %s params.txt generate [write_vtk] [stat_phitheta] [stat_mutangle]\n"
+                 "       [stat_contacts] [stat_conduct d [internal]]\n",
+                 argv[0]);
+    return 2;
+  }
+  nf_params p;
+  nf_params_default(&p);
+  int rc = nf_params_load(argv[1], &p);
+  if (rc != NF_OK) return die(nullptr, "reading parameters", rc);
+  if (*nf_last_error(nullptr)) std::printf("note: %s\n", nf_last_error(nullptr));
+  rc = nf_params_normalize(&p);
+  if (rc != NF_OK) return die(nullptr, "parameters", rc);
+  if (std::strcmp(argv[2], "generate") != 0) {
+    std::fprintf(stderr, "nanofiller: state files (output.bin) are not supported; "
+                         "the first verb must be generate\n");
+    return 2;
+  }
+  nf_handle* h = nullptr;
+  rc = nf_create(&p, &h);
+  if (rc != NF_OK) return die(nullptr, "device set-up", rc);
+  nf_gen_stats st;
+  rc = nf_generate(h, &st);
+  if (rc != NF_OK) {
+    const int r = die(h, "generate", rc);
+    nf_destroy(h);
+    return r;
+  }
+  const double PI = 3.14159265358979323846;
+  const double V = p.box[0] * p.box[1] * p.box[2];
+  std::vector<float> rods((size_t)st.n_accepted * 8);
+  uint32_t n = 0;
+  nf_get_rods(h, 0, rods.data(), st.n_accepted, &n);
+  double vol = 0.0;
+  for (uint32_t i = 0; i < n; ++i) vol += PI * (double)rods[8 * i + 7] * rods[8 * i + 7] * rods[8 * i + 6];
+  std::printf("%u inclusions generated! \n", n);
+  std::printf("Volume fraction of %5.2f percent is achieved! \n", 100.0 * vol / V);
+  std::printf("rounds %u, pair tests %llu, rechecks %llu, %.2f ms\n", st.rounds,
+              (unsigned long long)st.pair_tests, (unsigned long long)st.rechecks, st.wall_ms);
+  bool have_contacts = false;
+  uint64_t nc = 0;
+  auto ensure_contacts = [&]() -> int {
+    if (have_contacts) return NF_OK;
+    const int r = nf_contacts(h, p.contact_gap, &nc);
+    if (r == NF_OK) have_contacts = true;
+    return r;
+  };
+  int status = 0;
+  for (int ai = 3; ai < argc && status == 0; ++ai) {
+    const char* v = argv[ai];
+    if (!std::strcmp(v, "write_vtk")) {
+      std::printf("CNTs created: %u \n", n);
+      rc = nf_write_vtk(h, "output.vtk");
+      if (rc != NF_OK) status = die(h, "write_vtk", rc);
+    } else if (!std::strcmp(v, "stat_phitheta") || !std::strcmp(v, "stat_mutangle")) {
+      rc = ensure_contacts();
+      if (rc != NF_OK) {
+        status = die(h, "contacts", rc);
+        break;
+      }
+      const bool pt = !std::strcmp(v, "stat_phitheta");
+      nf_hist_spec spec;
+      std::memset(&spec, 0, sizeof(spec));
+      spec.bins = 32;
+      spec.centre_axis = 0;
+      spec.pol_axis[2] = 1.0;
+      spec.phi_bins = pt ? 32 : 0;
+      spec.theta_bins = pt ? 32 : 0;
+      std::vector<int64_t> c(32), po(32), mu(32), ph(32 * 32);
+      rc = nf_histograms(h, &spec, c.data(), po.data(), mu.data(), pt ? ph.data() : nullptr);
+      if (rc != NF_OK) {
+        status = die(h, v, rc);
+        break;
+      }
+      const bool ok = pt ? write_rho("rho.dat", 32, 32, ph) : write_gamma("gamma.dat", mu);
+      std::printf("file %s created!\n", pt ? "rho.dat" : "gamma.dat");
+      if (!ok) status = die(h, "writing statistics", NF_E_IO);
+    } else if (!std::strcmp(v, "stat_contacts")) {
+      rc = ensure_contacts();
+      if (rc != NF_OK) {
+        status = die(h, "contacts", rc);
+        break;
+      }
+      std::vector<uint32_t> row((size_t)n + 1), col(nc ? nc : 1);
+      nf_get_contacts(h, row.data(), col.data(), nullptr, nullptr, nc);
+      std::vector<uint32_t> per(n, 0);
+      for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = row[i]; k < row[i + 1]; ++k) {
+          ++per[i];
+          ++per[col[k]];
+        }
+      uint32_t mx = 0;
+      for (uint32_t x : per) mx = x > mx ? x : mx;
+      std::printf("contacts (gap <= %g um): %llu; per inclusion: mean %.4f, max %u\n", p.contact_gap,
+                  (unsigned long long)nc, n ? 2.0 * (double)nc / n : 0.0, mx);
+    } else if (!std::strcmp(v, "stat_conduct")) {
+      if (ai + 1 >= argc) {
+        std::fprintf(stderr, "nanofiller: stat_conduct needs an axis 0|1|2\n");
+        status = 2;
+        break;
+      }
+      const int dir = std::atoi(argv[++ai]);
+      if (ai + 1 < argc && !std::strcmp(argv[ai + 1], "internal")) ++ai;
+      if (dir < 0 || dir > 2) {
+        std::fprintf(stderr, "nanofiller: stat_conduct axis must be 0, 1 or 2\n");
+        status = 2;
+        break;
+      }
+      // the electrode axis is a parameter of the handle: rebuild it with the
+      // same rods for this axis
+      nf_params q = p;
+      q.elec_axis = dir;
+      nf_handle* h2 = nullptr;
+      rc = nf_create(&q, &h2);
+      if (rc == NF_OK) rc = nf_set_rods(h2, 0, rods.data(), n);
+      uint64_t nc2 = 0;
+      if (rc == NF_OK) rc = nf_contacts(h2, q.contact_gap, &nc2);
+      nf_network_info ni;
+      if (rc == NF_OK) rc = nf_network(h2, &ni);
+      if (rc == NF_OK) rc = nf_write_coo(h2, "");
+      if (rc != NF_OK) {
+        status = die(h2, "stat_conduct", rc);
+      } else {
+        std::printf("Building conductivity sparse matrix \n");
+        std::printf("contacts %llu, network edges %llu, components %u (largest %u), "
+                    "percolating %u, electrode couplings %u / %u, matrix nnz %llu\n",
+                    (unsigned long long)nc2, (unsigned long long)ni.n_edges, ni.n_components,
+                    ni.largest, ni.percolating, ni.n_source, ni.n_sink,
+                    (unsigned long long)ni.nnz);
+      }
+      if (h2) nf_destroy(h2);
+    } else {
+      std::fprintf(stderr, "nanofiller: verb %s is not supported by this build\n", v);
+      status = 2;
+    }
+  }
+  nf_destroy(h);
+  return status;
+}

@@ -375,6 +375,55 @@ NF_HD F32Pair pair_f32(const float4 P1, const float4 D1, const float4 P2, const 
 }
 
 // --------------------------------------------------------------------------
+// Cheap screen in front of pair_f32: the rigorous reach band, then a lower
+// bound on the segment distance by the distance of the two infinite lines,
+// |w . (u1 x u2)| / |u1 x u2| (Lumelsky 1985: the segments' closest points
+// lie on the lines, so the line distance never exceeds the segment
+// distance).  Returns
+//   SCREEN_UNDECIDED  reach inside its error band: the caller decides reach
+//                     and cut in FP64;
+//   SCREEN_OUT        not within reach (FP64 agrees);
+//   SCREEN_FAR        within reach and certainly not cut: the exact segment
+//                     distance exceeds sqrt(R^2 (1 + 16 u) + e64), so the
+//                     FP64 squared distance is above R64^2 as well;
+//   SCREEN_FULL       within reach, run pair_f32 for the cut.
+// Error terms: |w - w_exact| <= sqrt(3) ew; each component of u1 x u2 is
+// within 2u of the exact product (|u| <= 1), so |v - v_exact| <= 3.5u; the
+// dot product adds 3u |w||v|; e64 = EPS64_REL S^2 bounds the FP64
+// algorithm's distance from the exact minimum, as in pair_f32.
+// --------------------------------------------------------------------------
+enum { SCREEN_UNDECIDED = -1, SCREEN_OUT = 0, SCREEN_FAR = 1, SCREEN_FULL = 2 };
+
+NF_HD int pair_f32_screen(const float4 P1, const float4 D1, const float4 P2, const float4 D2,
+                          const float sx, const float sy, const float sz, const float R,
+                          const float ew) {
+  const float wx = (P1.x - P2.x) - sx;
+  const float wy = (P1.y - P2.y) - sy;
+  const float wz = (P1.z - P2.z) - sz;
+  const float w2 = fmaf(wz, wz, fmaf(wy, wy, wx * wx));
+  const float rr = (P1.w + P2.w) * 1.000001f + R;
+  const float rr2 = rr * rr;
+  const float nw = sqrtf(w2) + 2.0f * ew;
+  const float ew2 = 1.7321f * ew;
+  const float e_w2 = 4.0f * U32 * w2 + (2.0f * nw + ew2) * ew2;
+  const float rlo = rr2 * (1.0f - 1.5e-6f), rhi = rr2 * (1.0f + 1.5e-6f);
+  if (w2 - e_w2 > rhi) return SCREEN_OUT;
+  if (!(w2 + e_w2 < rlo)) return SCREEN_UNDECIDED;
+  const float vx = fmaf(D1.y, D2.z, -(D1.z * D2.y));
+  const float vy = fmaf(D1.z, D2.x, -(D1.x * D2.z));
+  const float vz = fmaf(D1.x, D2.y, -(D1.y * D2.x));
+  const float v2 = fmaf(vz, vz, fmaf(vy, vy, vx * vx));
+  const float t = fmaf(wz, vz, fmaf(wy, vy, wx * vx));
+  const float nv = sqrtf(v2) * (1.0f + 4.0f * U32) + 3.5f * U32;
+  const float Et = (3.5f * U32 * nw + ew2 * nv + 3.0f * U32 * nw * nv) * 1.01f;
+  const float A = fabsf(t) - Et;
+  const float S = nw + P1.w * 1.000001f + P2.w * 1.000001f;
+  const float Rb2 = (R * R * (1.0f + 16.0f * U32) + EPS64_REL * S * S) * 1.0001f;
+  if (A > 0.0f && A * A > Rb2 * (nv * nv)) return SCREEN_FAR;
+  return SCREEN_FULL;
+}
+
+// --------------------------------------------------------------------------
 // Candidate sampler.  The job of cuda_CNTgenerate_orient
 // (inc/cuda_CNTgenerate.h:136-260): uniform position, von Neumann accept
 // against the density target (:227-232), orientation from the target law

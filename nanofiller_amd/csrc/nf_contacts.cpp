@@ -53,6 +53,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   // longest rods and per-cell length bounds prune the rest
   double edge = (p.len_law == NF_LEN_LOGNORMAL ? p.len : (double)hlmax * 2.0) * REACH_PAD + Rmax;
   if (edge > reach) edge = reach;
+  edge = default_cell_edge(edge, (double)n / (p.box[0] * p.box[1] * p.box[2]));
   uint64_t ncells = 1;
   double scale[3];
   for (int a = 0; a < 3; ++a) {
@@ -84,7 +85,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   ALLOC(c.cnt, (uint64_t)n + 1);
   ALLOC(c.rowptr, (uint64_t)n + 1);
   ALLOC(c.fill, n);
-  ALLOC(ctr, 4);
+  ALLOC(ctr, 5);  // pair tests, rechecks, max dd, tile bytes, append count
   ALLOC(tmp, std::max(scan_tmp_words(ncells), scan_tmp_words((uint64_t)n + 1)));
   ALLOC(queues, 2 * QUEUES_PER_KERNEL);
 #undef ALLOC
@@ -101,13 +102,13 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   fill_u32(c.ccount, 0u, ncells, s);
   fill_u32((uint32_t*)c.chl, 0u, ncells, s);
   fill_u32(c.ccur, 0u, ncells, s);
-  fill_u32((uint32_t*)ctr, 0u, 8, s);
+  fill_u32((uint32_t*)ctr, 0u, 10, s);
   fill_u32(queues, 0u, 2 * QUEUES_PER_KERNEL, s);
   launch_cell_count(P, n, c.ncell, scale, c.ccount, c.chl, s);
   scan_u32(c.ccount, c.cstart, ncells, tmp, s);
   launch_cell_scatter(P, D, n, c.ncell, scale, c.cstart, c.ccur, c.cP, c.cD, c.cid, s);
   hipEventRecord(ev[1], s);
-  // count
+  // one pair pass appends every contact to an unsorted list
   PairArgs a{};
   a.desc = nullptr;
   a.st = nullptr;
@@ -128,7 +129,6 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   a.nbCount = c.ccount;
   a.nbHl = c.chl;
   a.qctr = queues;
-  a.cnt = c.cnt;
   for (int k = 0; k < 3; ++k) {
     a.box[k] = p.box[k];
     a.boxf[k] = (float)p.box[k];
@@ -142,16 +142,73 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   a.R_max = (float)Rmax * 1.000001f;
   a.ctr = ctr;
   a.diag = h->opt_diag;
-  fill_u32(c.cnt, 0u, (uint64_t)n + 1, s);
-  launch_pair(PM_CNT_COUNT, prec, a, s);
-  hipEventRecord(ev[2], s);
-  scan_u32(c.cnt, c.rowptr, n, tmp, s);
+  // append capacity: the previous call's count, else one contact per two
+  // rods; a pass that overflows is repeated with the exact size
+  uint64_t cap = std::max<uint64_t>(h->contact_hint + h->contact_hint / 8 + 1024, (uint64_t)n / 2 + 4096);
+  if (cap > 0xFFFFFFF0ull) cap = 0xFFFFFFF0ull;
+  uint32_t* appI = nullptr;
+  uint32_t* appJ = nullptr;
+  float* appG = nullptr;
+  double* appG64 = nullptr;
+  int32_t* appImg = nullptr;
+  auto free_app = [&]() {
+    dfree(h, appI);
+    dfree(h, appJ);
+    dfree(h, appG);
+    dfree(h, appG64);
+    dfree(h, appImg);
+    appI = appJ = nullptr;
+    appG = nullptr;
+    appG64 = nullptr;
+    appImg = nullptr;
+  };
+  auto free_all = [&]() {
+    free_app();
+    dfree(h, ctr);
+    dfree(h, tmp);
+    dfree(h, queues);
+  };
   uint32_t total = 0;
-  trace("contacts: %u rods, %u cells, %u stencil offsets, count pass launched", n, c.ncells, c.st_n);
-  HIPCHK(h, hipMemcpyAsync(&total, c.rowptr + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(h, sync_stream(s, "contact count"));
-  trace("contacts: %u pairs counted", total);
-  HIPCHK(h, hipGetLastError());
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    rc = NF_OK;
+    if (rc == NF_OK) rc = dalloc(h, &appI, cap);
+    if (rc == NF_OK) rc = dalloc(h, &appJ, cap);
+    if (rc == NF_OK) rc = dalloc(h, &appG, cap);
+    if (rc == NF_OK) rc = dalloc(h, &appG64, cap);
+    if (rc == NF_OK) rc = dalloc(h, &appImg, cap);
+    if (rc != NF_OK) {
+      free_all();
+      return rc;
+    }
+    if (attempt > 0) {
+      fill_u32((uint32_t*)ctr, 0u, 10, s);
+      fill_u32(queues, 0u, 2 * QUEUES_PER_KERNEL, s);
+    }
+    a.app_n = (uint32_t*)(ctr + 4);
+    a.app_cap = (uint32_t)cap;
+    a.outI = appI;
+    a.outJ = appJ;
+    a.outGap = appG;
+    a.outGap64 = appG64;
+    a.outImg = appImg;
+    hipEventRecord(ev[1], s);
+    launch_pair(PM_CNT, prec, a, s);
+    hipEventRecord(ev[2], s);
+    trace("contacts: %u rods, %u cells, %u stencil offsets, pair pass launched", n, c.ncells, c.st_n);
+    HIPCHK(h, hipMemcpyAsync(&total, a.app_n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, sync_stream(s, "contact pair pass"));
+    HIPCHK(h, hipGetLastError());
+    if ((uint64_t)total <= cap) break;
+    trace("contacts: %u contacts exceed the append capacity %llu, repeating the pass", total,
+          (unsigned long long)cap);
+    free_app();
+    cap = (uint64_t)total;
+    if (attempt == 1) {
+      free_all();
+      return fail(h, NF_E_STATE, "contact count changed between two identical passes");
+    }
+  }
+  h->contact_hint = total;
   c.n_contacts = total;
   c.cap_contacts = total;
   rc = NF_OK;
@@ -160,29 +217,23 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   if (rc == NF_OK) rc = dalloc(h, &c.gap64, total);
   if (rc == NF_OK) rc = dalloc(h, &c.img, total);
   if (rc != NF_OK) {
-    dfree(h, ctr);
-    dfree(h, tmp);
-    dfree(h, queues);
+    free_all();
     return rc;
   }
+  // CSR over I: counts, scan, scatter, canonical row order
   hipEventRecord(ev[3], s);
+  fill_u32(c.cnt, 0u, (uint64_t)n + 1, s);
   fill_u32(c.fill, 0u, n, s);
-  fill_u32(queues + QUEUES_PER_KERNEL, 0u, QUEUES_PER_KERNEL, s);
-  fill_u32((uint32_t*)ctr, 0u, 8, s);  // the record pass repeats the count's tests
-  a.qctr = queues + QUEUES_PER_KERNEL;
-  a.rowptr = c.rowptr;
-  a.fill = c.fill;
-  a.outJ = c.J;
-  a.outGap = c.gap;
-  a.outGap64 = c.gap64;
-  a.outImg = c.img;
-  launch_pair(PM_CNT_RECORD, prec, a, s);
+  launch_csr_count(appI, total, c.cnt, s);
+  scan_u32(c.cnt, c.rowptr, n, tmp, s);
+  launch_csr_scatter(appI, appJ, appG, appG64, appImg, total, c.rowptr, c.fill, c.J, c.gap, c.gap64,
+                     c.img, s);
   hipEventRecord(ev[4], s);
   launch_row_sort(c.rowptr, n, c.J, c.gap, c.gap64, c.img, s);
   hipEventRecord(ev[5], s);
   HIPCHK(h, hipMemcpyAsync(h->cb_stats, ctr, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s));
-  HIPCHK(h, sync_stream(s, "contact record"));
-  trace("contacts: recorded and sorted");
+  HIPCHK(h, sync_stream(s, "contact list"));
+  trace("contacts: %u recorded and sorted", total);
   HIPCHK(h, hipGetLastError());
   float ms = 0.0f;
   const int from[4] = {0, 1, 3, 4}, to[4] = {1, 2, 4, 5};
@@ -190,18 +241,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
     (void)hipEventElapsedTime(&ms, ev[from[k]], ev[to[k]]);
     h->cb_ms[k] = ms;
   }
-  // verify the record pass filled every counted slot (reference warns the
-  // same way, src/get_contact_arrays.cu:132-134)
-  std::vector<uint32_t> hc(n), hf(n);
-  if (n) {
-    HIPCHK(h, hipMemcpy(hc.data(), c.cnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-    HIPCHK(h, hipMemcpy(hf.data(), c.fill, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-  }
-  dfree(h, ctr);
-  dfree(h, tmp);
-  dfree(h, queues);
-  for (uint32_t i = 0; i < n; ++i)
-    if (hc[i] != hf[i]) return fail(h, NF_E_STATE, "contact record pass disagrees with count pass");
+  free_all();
   c.valid = true;
   if (n_out) *n_out = total;
   return NF_OK;

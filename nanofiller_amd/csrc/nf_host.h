@@ -26,7 +26,8 @@ struct nf_handle {
   bool generated = false;
   std::vector<void*> allocs;
   unsigned long long cb_stats[4] = {0, 0, 0, 0};  // contact pair tests, rechecks, max dd bits
-  float cb_ms[4] = {0, 0, 0, 0};                  // contact kernel times: bin, count, record, sort
+  float cb_ms[4] = {0, 0, 0, 0};                  // contact kernel times: bin, pair pass, CSR build, row sort
+  uint64_t contact_hint = 0;                      // contacts found by the last call (append capacity)
   nf_network_info net_info = {};
   float net_ms = 0.0f;  // device time of the last nf_network call
 };
@@ -43,6 +44,8 @@ void trace(const char* fmt, ...);
 int dalloc_bytes(nf_handle* h, void** p, size_t bytes);
 void dfree(nf_handle* h, void* p);
 void release_contacts(nf_handle* h);
+// Default cell edge for a typical pair reach and a mean rod density.
+double default_cell_edge(double reach_typ, double density);
 std::vector<int4> build_stencil(const int32_t nc[3], const double box[3], double reach, bool half);
 
 template <class T>

@@ -161,9 +161,10 @@ struct PairArgs {
   // outputs
   uint32_t* alive_out;
   uint32_t* surv;
-  uint32_t* cnt;
-  const uint32_t* rowptr;
-  uint32_t* fill;
+  // contacts: unsorted append buffer (capacity app_cap, count *app_n)
+  uint32_t* app_n;
+  uint32_t app_cap;
+  uint32_t* outI;
   uint32_t* outJ;
   float* outGap;
   double* outGap64;
@@ -180,7 +181,7 @@ struct PairArgs {
   int diag;                 // record max |d32-d64| over full tests
 };
 
-enum PairMode { PM_GEN_OLD = 0, PM_GEN_NEW = 1, PM_CNT_COUNT = 2, PM_CNT_RECORD = 3 };
+enum PairMode { PM_GEN_OLD = 0, PM_GEN_NEW = 1, PM_CNT = 2 };
 
 // ---- launchers (nf_scan.hip) ----
 void scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
@@ -203,6 +204,12 @@ void launch_cell_scatter(const float4* P, const float4* D, uint32_t n, const int
                          float4* oD, uint32_t* oid, hipStream_t s);
 void launch_row_sort(const uint32_t* rowptr, uint32_t n, uint32_t* J, float* gap, double* gap64,
                      int32_t* img, hipStream_t s);
+// CSR over I from the unsorted contact list: per-row counts, then scatter
+// (rowptr from scan_u32 of the counts in between).
+void launch_csr_count(const uint32_t* I, uint64_t c, uint32_t* cnt, hipStream_t s);
+void launch_csr_scatter(const uint32_t* I, const uint32_t* J, const float* gap, const double* gap64,
+                        const int32_t* img, uint64_t c, const uint32_t* rowptr, uint32_t* fill,
+                        uint32_t* oJ, float* ogap, double* ogap64, int32_t* oimg, hipStream_t s);
 void launch_hist(const float4* P, const float4* D, uint32_t n, const uint32_t* rowptr,
                  const uint32_t* J, const double* edges_p, const double* edges_m,
                  const double* edges_t, int nb, int npb, int ntb, int axis, double L,

@@ -13,8 +13,8 @@
 //   PM_GEN_OLD    new vs accepted rods   cuda_OldExtIsectLB (inc/intersection_kern.h:245-364)
 //   PM_GEN_NEW    new vs new, priority   cuda_InternalIsect_noLck / cuda_NewExtIsectLB_parted
 //                                        (inc/intersection_kern.h:12-241)
-//   PM_CNT_COUNT  contacts per rod       core_cell_interact_kernel CONTACT_COUNT
-//   PM_CNT_RECORD contact records        core_cell_interact_kernel CONTACT_RECORD
+//   PM_CNT        contact records        core_cell_interact_kernel CONTACT_COUNT and
+//                                        CONTACT_RECORD in one pass (appended, unsorted)
 //                                        (src/core_cell_interaction_kernel.cu:70-315)
 #include "nf_internal.h"
 
@@ -110,9 +110,312 @@ __device__ __forceinline__ void flush(const PairArgs& a, const CellGeom& g, Acc&
   acc.bytes = 0;
 }
 
+// LDS of one wave (about 10 KB, so 16 waves fit a CU): the probe chunk,
+// the staged neighbour rods, the two pair queues and the list of relevant
+// neighbour cells of the current stencil block.
+constexpr int SLOTS = 2 * WAVE;
+constexpr int QCAP = 3 * WAVE;
+struct PairLds {
+  float4 sP[WAVE], sD[WAVE];  // probes of the chunk
+  uint32_t sX[WAVE];
+  uint32_t sA[WAVE];           // probe takes part (stage-1 survivor in the same-round pass)
+  uint32_t hit[WAVE];          // per probe: overlaps something (generation)
+  uint32_t dummy[WAVE];        // sink of branch-free stores
+  float4 bP[SLOTS], bD[SLOTS];  // staged neighbour rods
+  uint32_t bX[SLOTS];           // rod id (0xFFFFFFFF: not a candidate)
+  uint32_t bM[SLOTS];           // image code | same cell << 31
+  uint32_t qE[QCAP];            // queued pairs: probe | slot << 8
+  uint32_t q2[QCAP];            // pairs for the full test: probe | slot << 8 | fp64 << 16
+  uint32_t lS[WAVE], lN[WAVE], lM[WAVE];  // relevant neighbour cells: start, count, meta
+};
+
+__device__ __forceinline__ uint32_t lane_prefix(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ int img_x(uint32_t meta) { return (int)(int8_t)(meta & 0xFF); }
+__device__ __forceinline__ int img_y(uint32_t meta) { return (int)(int8_t)((meta >> 8) & 0xFF); }
+__device__ __forceinline__ int img_z(uint32_t meta) { return (int)(int8_t)((meta >> 16) & 0xFF); }
+
+// FP32 image shift of a neighbour: the FP64 shift img * box rounded once.
+__device__ __forceinline__ float3 shift_f32(const CellGeom& g, uint32_t meta) {
+  return make_float3((float)((double)img_x(meta) * g.box[0]), (float)((double)img_y(meta) * g.box[1]),
+                     (float)((double)img_z(meta) * g.box[2]));
+}
+
+// Screen of one queued pair (valid lanes only): the reach band and the
+// line-distance bound of pair_f32_screen.  Counts the pair test when reach
+// is decided.  Returns 0 (settled), 1 (full FP32 test needed) or 2 (reach
+// undecided, or the plain FP64 path: the full test decides reach and cut in
+// FP64 and counts).
 template <int MODE, int PREC>
-__device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, float4* tP,
-                             float4* tD, uint32_t* tX, uint32_t* tDummy, Acc& acc) {
+__device__ __forceinline__ uint32_t screen_test(const PairArgs& a, const CellGeom& g, PairLds& L,
+                                                uint32_t e, bool valid, Acc& acc) {
+  constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
+  const uint32_t ent = L.qE[e];
+  const uint32_t p = ent & 0xFFu, j = ent >> 8;
+  if (PREC == NF_PREC_FP64) return valid ? 2u : 0u;
+  const float4 P2 = L.bP[j], D2 = L.bD[j];
+  const uint32_t meta = L.bM[j];
+  const float4 P1 = L.sP[p], D1 = L.sD[p];
+  if (!valid) return 0u;
+  const float3 sh = shift_f32(g, meta);
+  const float R32 = GEN ? (g.sf32 * (D1.w + D2.w) + g.sep32) : ((D1.w + D2.w) + a.gap32);
+  const int r = pair_f32_screen(P1, D1, P2, D2, sh.x, sh.y, sh.z, R32, g.ew);
+  if (r == SCREEN_UNDECIDED) return 2u;
+  if (r == SCREEN_OUT) return 0u;
+  ++acc.tests;
+  return r == SCREEN_FULL ? 1u : 0u;
+}
+
+// Full capsule test of one pair of the second queue (valid lanes only): the
+// FP32 band with FP64 recheck, or plain FP64 (flag bit 16: reach not yet
+// decided, count here).  No cross-lane operation inside.
+template <int MODE, int PREC>
+__device__ __forceinline__ void full_test(const PairArgs& a, const CellGeom& g, PairLds& L,
+                                          uint32_t e, bool valid, Acc& acc) {
+  constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
+  const uint32_t ent = L.q2[e];
+  const uint32_t p = ent & 0xFFu, j = (ent >> 8) & 0xFFu;
+  const bool f64 = PREC == NF_PREC_FP64 || ((ent >> 16) & 1u) != 0u;
+  const float4 P2 = L.bP[j], D2 = L.bD[j];
+  const uint32_t meta = L.bM[j];
+  const uint32_t xj = L.bX[j];
+  const float4 P1 = L.sP[p], D1 = L.sD[p];
+  const uint32_t X1 = L.sX[p];
+  if (!valid) return;
+  const int ix = img_x(meta), iy = img_y(meta), iz = img_z(meta);
+  const double sx = (double)ix * g.box[0], sy = (double)iy * g.box[1], sz = (double)iz * g.box[2];
+  // FP64 evaluation in the canonical orientation the decisions are defined
+  // by: generation puts the probe (proposal, or the later slot) first;
+  // contacts put the lower rod id first, with the periodic image applied to
+  // the higher id.  The closest-point solve is not bitwise symmetric under
+  // swapping the segments.
+  const bool swap = !GEN && (X1 > xj);
+  const float4 Pa = swap ? P2 : P1, Da = swap ? D2 : D1;
+  const float4 Pb = swap ? P1 : P2, Db = swap ? D1 : D2;
+  const double ux = swap ? -sx : sx, uy = swap ? -sy : sy, uz = swap ? -sz : sz;
+  bool within = true;
+  bool cut;
+  double d2_64 = -1.0;
+  double R64;
+  if (GEN)
+    R64 = g.sf64 * ((double)D1.w + (double)D2.w) + g.sep64;
+  else
+    R64 = ((double)Da.w + (double)Db.w) + a.gap64;
+  if (f64) {
+    if (PREC != NF_PREC_FP64) ++acc.rech;
+    d2_64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &within);
+    cut = GEN ? (d2_64 < R64 * R64) : (d2_64 <= R64 * R64);
+    if (!within) return;
+    ++acc.tests;
+  } else {
+    // reach was decided 'within' by the screen with the same band
+    const float R32 = GEN ? (g.sf32 * (D1.w + D2.w) + g.sep32) : ((D1.w + D2.w) + a.gap32);
+    const F32Pair r = pair_f32(P1, D1, P2, D2, (float)sx, (float)sy, (float)sz, R32, g.ew, GEN);
+    if (r.decided < 0) {
+      ++acc.rech;
+      bool w64;
+      d2_64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &w64);
+      cut = GEN ? (d2_64 < R64 * R64) : (d2_64 <= R64 * R64);
+    } else {
+      cut = r.decided != 0;
+      if (a.diag) {
+        bool w2;
+        const double e64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &w2);
+        const double dd = fabs(sqrt((double)r.d2) - sqrt(e64));
+        acc.maxdd = dd > acc.maxdd ? dd : acc.maxdd;
+      }
+    }
+  }
+  if (!cut) return;
+  if (GEN) {
+    L.hit[p] = 1u;
+  } else {
+    if (d2_64 < 0.0) {
+      bool w2;
+      d2_64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &w2);
+    }
+    // append to the unsorted list; the host sizes the buffer again and
+    // repeats the pass when the count exceeds its capacity
+    const uint32_t I = X1 < xj ? X1 : xj;
+    const uint32_t J = X1 < xj ? xj : X1;
+    const int sgn = X1 < xj ? 1 : -1;
+    const uint32_t pos = atomicAdd(a.app_n, 1u);
+    if (pos >= a.app_cap) return;
+    const double gv = sqrt_cr(d2_64) - ((double)Da.w + (double)Db.w);
+    a.outI[pos] = I;
+    a.outJ[pos] = J;
+    a.outGap[pos] = (float)gv;
+    a.outGap64[pos] = gv;
+    a.outImg[pos] = pack_img(sgn * ix, sgn * iy, sgn * iz);
+  }
+}
+
+// Moves entries [base, base + cnt) of the first queue through the screen
+// and appends those that need the full test to the second queue; runs the
+// full test when 64 are waiting.  Wave-uniform control; the caller holds a
+// barrier before the call.
+template <int MODE, int PREC>
+__device__ __forceinline__ void screen_batch(const PairArgs& a, const CellGeom& g, PairLds& L,
+                                             uint32_t base, uint32_t cnt, uint32_t& q2n,
+                                             Acc& acc) {
+  const int lane = threadIdx.x;
+  const bool valid = (uint32_t)lane < cnt;
+  const uint32_t e = base + min((uint32_t)lane, cnt - 1u);
+  const uint32_t code = screen_test<MODE, PREC>(a, g, L, e, valid, acc);
+  const bool f = code != 0u;
+  const unsigned long long mask = __ballot(f);
+  if (f) L.q2[q2n + lane_prefix(mask)] = L.qE[e] | (code == 2u ? 0x10000u : 0u);
+  q2n += (uint32_t)__popcll(mask);
+  if (q2n >= (uint32_t)WAVE) {
+    __syncthreads();
+    full_test<MODE, PREC>(a, g, L, q2n - WAVE + (uint32_t)lane, true, acc);
+    q2n -= WAVE;
+  }
+  __syncthreads();
+}
+
+// Walk state over the sorted stencil of one probe chunk.  The stencil is
+// taken 64 offsets at a time: every lane looks up one neighbour cell (count,
+// start and longest rod in parallel, one round trip), and the relevant cells
+// are compacted in stencil order into the LDS list.  All fields are
+// wave-uniform.
+struct StencilCursor {
+  uint32_t sb;      // next stencil block
+  uint32_t li, ln;  // position in / length of the cell list
+  uint32_t start;   // current neighbour cell: first index, rod count, staged so far
+  uint32_t n;
+  uint32_t off;
+  uint32_t meta;    // image code | same cell << 31
+  bool last;        // no stencil block left after the current list
+  bool done;
+};
+
+template <int MODE>
+__device__ void refill_cells(const PairArgs& a, const CellGeom& g, PairLds& L, StencilCursor& c,
+                             int c0, int c1, int c2, float chunk_hl, float rr_chunk2) {
+  const int lane = threadIdx.x;
+  while (!c.last) {
+    const uint32_t si = c.sb + (uint32_t)lane;
+    const bool valid = si < g.st_n;
+    const int4 so = g.st[min(si, g.st_n - 1u)];
+    const float md2 = __int_as_float(so.w);
+    const int raw[3] = {c0 + so.x, c1 + so.y, c2 + so.z};
+    int img[3], m[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int mm = raw[d] % g.nc[d];
+      if (mm < 0) mm += g.nc[d];
+      img[d] = (raw[d] - mm) / g.nc[d];
+      m[d] = mm;
+    }
+    const uint32_t nc = g.cell_base + (uint32_t)(m[0] + g.nc[0] * (m[1] + g.nc[1] * m[2]));
+    uint32_t n = a.nbCount[nc];
+    if (MODE == PM_GEN_OLD) n = min(n, g.cap);
+    const float hl = a.nbHl[nc];
+    const uint32_t st0 = a.nbStart[nc];
+    const float rr_c = (chunk_hl + hl) * 1.000001f + g.R_max + CELL_MARGIN;
+    const bool beyond = valid && md2 > rr_chunk2;
+    const bool rel = valid && !beyond && n > 0u && md2 <= rr_c * rr_c;
+    const unsigned long long mk = __ballot(rel);
+    const bool same = so.x == 0 && so.y == 0 && so.z == 0;
+    if (rel) {
+      const uint32_t pos = lane_prefix(mk);
+      L.lS[pos] = st0;
+      L.lN[pos] = n;
+      L.lM[pos] = (uint32_t)pack_img(img[0], img[1], img[2]) | (same ? 0x80000000u : 0u);
+    }
+    const bool stop = (__ballot(beyond) != 0ull) || (c.sb + (uint32_t)WAVE >= g.st_n);
+    c.sb += (uint32_t)WAVE;
+    c.ln = (uint32_t)__popcll(mk);
+    c.li = 0u;
+    if (stop) c.last = true;
+    __syncthreads();
+    if (c.ln > 0u) return;
+  }
+}
+
+// Advances to the next relevant neighbour cell (stencil sorted by minimum
+// cell distance: the first block with an offset past the chunk's reach is
+// the last one).
+template <int MODE>
+__device__ __forceinline__ void next_cell(const PairArgs& a, const CellGeom& g, PairLds& L,
+                                          StencilCursor& c, int c0, int c1, int c2,
+                                          float chunk_hl, float rr_chunk2) {
+  if (c.li >= c.ln) {
+    if (!c.last) refill_cells<MODE>(a, g, L, c, c0, c1, c2, chunk_hl, rr_chunk2);
+    if (c.li >= c.ln) {
+      c.done = true;
+      return;
+    }
+  }
+  c.start = uni(L.lS[c.li]);
+  c.n = uni(L.lN[c.li]);
+  c.meta = uni(L.lM[c.li]);
+  c.off = 0u;
+  c.li += 1u;
+}
+
+// Prefilter of one (probe, slot) pair, the slot's data in registers:
+// centre distance against the batch-wide reach bound rr2 of the probe, and
+// the pass-specific pair filters.  Unfilled slots sit at 1e30, so the
+// distance test drops them.  Keeps every pair the full test could count or
+// cut (see process_cell).
+template <int MODE>
+__device__ __forceinline__ bool prefilter(const float4 P1, float rr2, uint32_t X1, uint32_t qprobe,
+                                          const float4 Q2, uint32_t xj, uint32_t qa, uint32_t qb) {
+  constexpr bool CNT = (MODE == PM_CNT);
+  const float wx = P1.x - Q2.x, wy = P1.y - Q2.y, wz = P1.z - Q2.z;
+  const float w2 = fmaf(wz, wz, fmaf(wy, wy, wx * wx));
+  bool f = w2 <= rr2;
+  if (MODE == PM_GEN_NEW) f = f && xj < X1;
+  // contacts: a slot of the home cell only after the probe's index; another
+  // cell (an image may map back to the home cell) never the probe itself
+  if (CNT) f = f && (qprobe < qa || qprobe > qb);
+  return f;
+}
+
+// Appends the prefilter results of every lane for one probe (both slot
+// halves) to the first queue, screening 64 at a time when enough wait.
+template <int MODE, int PREC>
+__device__ __forceinline__ void push_pairs(const PairArgs& a, const CellGeom& g, PairLds& L,
+                                           bool f0, bool f1, uint32_t pp, uint32_t& qn,
+                                           uint32_t& q2n, Acc& acc) {
+  const int lane = threadIdx.x;
+  const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
+  const uint32_t n0 = (uint32_t)__popcll(m0);
+  if (f0) L.qE[qn + lane_prefix(m0)] = pp | ((uint32_t)lane << 8);
+  if (f1) L.qE[qn + n0 + lane_prefix(m1)] = pp | ((uint32_t)(lane + WAVE) << 8);
+  qn += n0 + (uint32_t)__popcll(m1);
+  while (qn >= (uint32_t)WAVE) {
+    __syncthreads();
+    screen_batch<MODE, PREC>(a, g, L, qn - WAVE, WAVE, q2n, acc);
+    qn -= WAVE;
+  }
+}
+
+// One home cell.  For each chunk of up to 64 probes: walk the sorted
+// stencil and pack the rods of consecutive relevant neighbour cells into a
+// 128-slot staging buffer (two slots per lane, both loads in flight
+// together, no lanes wasted on sparse cells); then loop over the chunk's
+// probes (read by broadcast from LDS) with the lanes spanning the slots,
+// keep the pairs that pass a conservative centre-distance prefilter in an
+// LDS queue (ballot + lane prefix), screen them 64 at a time (reach band,
+// line distance) and run the full test on the survivors 64 at a time, so
+// every expensive step runs on dense work.
+//
+// Prefilter: the full test first rejects pairs whose FP32 centre distance
+// w2 exceeds rr^2 by more than its error band, rr = (hl1 + hl2) * 1.000001
+// + R.  The prefilter keeps every pair with w2' <= rr'^2 * 1.00001, where
+// rr' = hl1 * 1.000001 + hl2 * 1.000001 + R_max + 16 ew >= rr + 16 ew and
+// w' = P1 - (P2 + shift) differs from the full test's ((P1 - P2) - shift)
+// by a few roundings of coordinates bounded by box + reach, i.e. by less
+// than 6 ew; so every pair the full test would count or cut is queued.
+template <int MODE, int PREC>
+__device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, PairLds& L,
+                             Acc& acc) {
   constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
   const int lane = threadIdx.x;
   const uint32_t pc = uni(a.prCount[gc]);
@@ -122,170 +425,136 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
   const int c0 = (int)(lc % (uint32_t)g.nc[0]);
   const int c1 = (int)((lc / (uint32_t)g.nc[0]) % (uint32_t)g.nc[1]);
   const int c2 = (int)(lc / ((uint32_t)g.nc[0] * (uint32_t)g.nc[1]));
+  const float pad_r = g.R_max + 16.0f * g.ew;
 
   for (uint32_t pb = 0; pb < pc; pb += WAVE) {
     const uint32_t G = min((uint32_t)WAVE, pc - pb);
-    uint32_t Gp = 1;
-    while (Gp < G) Gp <<= 1;
-    const uint32_t K = WAVE / Gp;
-    const uint32_t lk = __builtin_ctz(K);
-    const uint32_t p = (uint32_t)lane >> lk;
-    const uint32_t sub = (uint32_t)lane & (K - 1);
-    const bool inrange = p < G;
-    const uint32_t pidx = ps + pb + (inrange ? p : 0u);  // clamped: every lane loads
+    // lane l holds probe l of the chunk (clamped loads, every lane loads)
+    const bool inrange = (uint32_t)lane < G;
+    const uint32_t pidx = ps + pb + min((uint32_t)lane, G - 1u);
     const float4 P1 = a.prP[pidx];
     const float4 D1 = a.prD[pidx];
     const uint32_t X1 = a.prX[pidx];
     bool have = inrange;
     if (MODE == PM_GEN_NEW) have = have && (a.prAlive[pidx] != 0u);
+    L.sP[lane] = P1;
+    L.sD[lane] = D1;
+    L.sX[lane] = X1;
+    L.sA[lane] = have ? 1u : 0u;
+    L.hit[lane] = 0u;
     const float chunk_hl = unif(wave_max(have ? P1.w : 0.0f));
     acc.bytes += (unsigned long long)G * 36ull;
-    bool hit = false;
     const float rr_chunk = (chunk_hl + g.hl_max) * 1.000001f + g.R_max + CELL_MARGIN;
     const float rr_chunk2 = rr_chunk * rr_chunk;
+    uint32_t qn = 0;   // wave-uniform queue lengths: prefiltered pairs,
+    uint32_t q2n = 0;  // and pairs waiting for the full test
+    StencilCursor cur{0u, 0u, 0u, 0u, 0u, 0u, 0u, false, false};
+    __syncthreads();
 
-    for (uint32_t si = 0; si < g.st_n; ++si) {
-      const int4 so = g.st[si];
-      const int ox = unii(so.x), oy = unii(so.y), oz = unii(so.z);
-      const float md2 = unif(__int_as_float(so.w));
-      if (md2 > rr_chunk2) break;
-      const int raw[3] = {c0 + ox, c1 + oy, c2 + oz};
-      int img[3], m[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        int mm = raw[d] % g.nc[d];
-        if (mm < 0) mm += g.nc[d];
-        img[d] = (raw[d] - mm) / g.nc[d];
-        m[d] = mm;
-      }
-      const uint32_t nc = g.cell_base + (uint32_t)(m[0] + g.nc[0] * (m[1] + g.nc[1] * m[2]));
-      uint32_t n = uni(a.nbCount[nc]);
-      if (MODE == PM_GEN_OLD) n = min(n, g.cap);
-      if (n == 0) continue;
-      const float rr_c = (chunk_hl + unif(a.nbHl[nc])) * 1.000001f + g.R_max + CELL_MARGIN;
-      if (md2 > rr_c * rr_c) continue;
-      const double sx = (double)img[0] * g.box[0], sy = (double)img[1] * g.box[1],
-                   sz = (double)img[2] * g.box[2];
-      const float fx = (float)sx, fy = (float)sy, fz = (float)sz;
-      const bool same = (ox == 0 && oy == 0 && oz == 0);
-      const uint32_t nstart = uni(a.nbStart[nc]);
-      for (uint32_t tb = 0; tb < n; tb += WAVE) {
-        const uint32_t tn = min((uint32_t)WAVE, n - tb);
-        acc.bytes += (unsigned long long)tn * 36ull;
-        // every lane loads (index clamped to the tile) and writes its slot;
-        // slots >= tn are never read
-        const uint32_t q = nstart + tb + min((uint32_t)lane, tn - 1u);
-        const float4 lp = a.nbP[q];
-        const float4 ld = a.nbD[q];
-        uint32_t x = a.nbX[q];
-        if (MODE == PM_GEN_NEW) x = (a.nbAlive[q] != 0u) ? x : 0xFFFFFFFFu;
-        tP[lane] = lp;
-        tD[lane] = ld;
-        tX[lane] = x;
-        __syncthreads();
-        if (have) {
-          for (uint32_t j = sub; j < tn; j += K) {
-            const uint32_t xj = tX[j];
-            if (MODE == PM_GEN_NEW && xj >= X1) continue;
-            if (MODE == PM_CNT_COUNT || MODE == PM_CNT_RECORD) {
-              if (same && tb + j <= pb + p) continue;
-              if (xj == X1) continue;
-            }
-            const float4 P2 = tP[j], D2 = tD[j];
-            // FP64 evaluation in the canonical orientation the decisions are
-            // defined by: generation puts the probe (proposal, or the later
-            // slot) first; contacts put the lower rod id first, with the
-            // periodic image applied to the higher id.  The closest-point
-            // solve is not bitwise symmetric under swapping the segments.
-            const bool swap = !GEN && (X1 > xj);
-            const float4 Pa = swap ? P2 : P1, Da = swap ? D2 : D1;
-            const float4 Pb = swap ? P1 : P2, Db = swap ? D1 : D2;
-            const double ux = swap ? -sx : sx, uy = swap ? -sy : sy, uz = swap ? -sz : sz;
-            bool within;
-            bool cut;
-            double d2_64 = -1.0;
-            double R64;
-            if (GEN)
-              R64 = g.sf64 * ((double)D1.w + (double)D2.w) + g.sep64;
-            else
-              R64 = ((double)Da.w + (double)Db.w) + a.gap64;
-            if (PREC == NF_PREC_FP64) {
-              d2_64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &within);
-              cut = GEN ? (d2_64 < R64 * R64) : (d2_64 <= R64 * R64);
-            } else {
-              const float R32 = GEN ? (g.sf32 * (D1.w + D2.w) + g.sep32) : ((D1.w + D2.w) + a.gap32);
-              const F32Pair r = pair_f32(P1, D1, P2, D2, fx, fy, fz, R32, g.ew, GEN);
-              if (r.decided < 0) {
-                ++acc.rech;
-                d2_64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &within);
-                cut = GEN ? (d2_64 < R64 * R64) : (d2_64 <= R64 * R64);
-              } else {
-                within = r.within != 0;
-                cut = r.decided != 0;
-                if (a.diag && within) {
-                  bool w2;
-                  const double e64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &w2);
-                  const double dd = fabs(sqrt((double)r.d2) - sqrt(e64));
-                  acc.maxdd = dd > acc.maxdd ? dd : acc.maxdd;
-                }
-              }
-            }
-            if (!within) continue;
-            ++acc.tests;
-            if (!cut) continue;
-            if (GEN) {
-              hit = true;
-            } else {
-              if (d2_64 < 0.0) {
-                bool w2;
-                d2_64 = pair_d2_f64(Pa, Da, Pb, Db, ux, uy, uz, R64, &w2);
-              }
-              const uint32_t I = X1 < xj ? X1 : xj;
-              const uint32_t J = X1 < xj ? xj : X1;
-              if (MODE == PM_CNT_COUNT) {
-                atomicAdd(&a.cnt[I], 1u);
-              } else {
-                const int sgn = X1 < xj ? 1 : -1;
-                const uint32_t k = atomicAdd(&a.fill[I], 1u);
-                if (k >= a.rowptr[I + 1] - a.rowptr[I]) continue;  // host reports the mismatch
-                const uint32_t pos = a.rowptr[I] + k;
-                const double gv = sqrt_cr(d2_64) - ((double)Da.w + (double)Db.w);
-                a.outJ[pos] = J;
-                a.outGap[pos] = (float)gv;
-                a.outGap64[pos] = gv;
-                a.outImg[pos] = pack_img(sgn * img[0], sgn * img[1], sgn * img[2]);
-              }
-            }
-          }
+    while (!cur.done) {
+      // ---- stage: the lane's two slots take their source index from the
+      // cell that covers them (registers only)
+      uint32_t fill = 0;
+      uint32_t src0 = 0u, src1 = 0u, m0 = 0u, m1 = 0u;
+      while (fill < (uint32_t)SLOTS) {
+        if (cur.off >= cur.n) {
+          next_cell<MODE>(a, g, L, cur, c0, c1, c2, chunk_hl, rr_chunk2);
+          if (cur.done) break;
         }
-        __syncthreads();
+        const uint32_t take = min(cur.n - cur.off, (uint32_t)SLOTS - fill);
+        const uint32_t r0 = (uint32_t)lane - fill;  // wraps past 2^32 when lane < fill
+        const uint32_t r1 = (uint32_t)lane + (uint32_t)WAVE - fill;
+        if (r0 < take) {
+          src0 = cur.start + cur.off + r0;
+          m0 = cur.meta;
+        }
+        if (r1 < take) {
+          src1 = cur.start + cur.off + r1;
+          m1 = cur.meta;
+        }
+        fill += take;
+        cur.off += take;
       }
+      if (fill == 0) break;
+      acc.bytes += (unsigned long long)fill * 36ull;
+      // ---- load: both slots' loads in flight together; every lane loads
+      // (unfilled slots read index 0 of a non-empty array and are never used)
+      const bool v0 = (uint32_t)lane < fill, v1 = (uint32_t)lane + WAVE < fill;
+      const float4 p0 = a.nbP[src0], p1 = a.nbP[src1];
+      const float4 d0 = a.nbD[src0], d1 = a.nbD[src1];
+      uint32_t x0 = a.nbX[src0], x1 = a.nbX[src1];
+      if (MODE == PM_GEN_NEW) {
+        x0 = (a.nbAlive[src0] != 0u) ? x0 : 0xFFFFFFFFu;
+        x1 = (a.nbAlive[src1] != 0u) ? x1 : 0xFFFFFFFFu;
+      }
+      L.bP[lane] = p0;
+      L.bD[lane] = d0;
+      L.bX[lane] = x0;
+      L.bM[lane] = m0;
+      L.bP[lane + WAVE] = p1;
+      L.bD[lane + WAVE] = d1;
+      L.bX[lane + WAVE] = x1;
+      L.bM[lane + WAVE] = m1;
+      const float3 s0 = shift_f32(g, m0), s1 = shift_f32(g, m1);
+      const float far = 1e30f;  // unfilled slots: out of every reach
+      const float4 Q0 = v0 ? make_float4(p0.x + s0.x, p0.y + s0.y, p0.z + s0.z, 0.0f)
+                           : make_float4(far, far, far, 0.0f);
+      const float4 Q1 = v1 ? make_float4(p1.x + s1.x, p1.y + s1.y, p1.z + s1.z, 0.0f)
+                           : make_float4(far, far, far, 0.0f);
+      // batch-wide half-length bound for the prefilter reach
+      const float bmax = unif(wave_max(fmaxf(v0 ? p0.w : 0.0f, v1 ? p1.w : 0.0f))) * 1.000001f;
+      // contacts: keep slot iff qprobe < qa || qprobe > qb (home cell: only
+      // later indices; elsewhere: anything but the probe itself)
+      const uint32_t qa0 = src0, qa1 = src1;
+      const uint32_t qb0 = (m0 & 0x80000000u) ? 0xFFFFFFFFu : src0;
+      const uint32_t qb1 = (m1 & 0x80000000u) ? 0xFFFFFFFFu : src1;
+      __syncthreads();
+      // ---- prefilter every (probe, slot) pair, queue the survivors
+      const bool two = fill > (uint32_t)WAVE;
+      for (uint32_t pp = 0; pp < G; ++pp) {
+        if (uni(L.sA[pp]) == 0u) continue;  // scalar branch: the flag is the same for every lane
+        const float4 Pp = L.sP[pp];
+        const uint32_t Xp = L.sX[pp];
+        const float rr = Pp.w * 1.000001f + pad_r + bmax;
+        const float rr2 = rr * rr * 1.00001f;
+        const uint32_t qprobe = ps + pb + pp;
+        const bool f0 = prefilter<MODE>(Pp, rr2, Xp, qprobe, Q0, x0, qa0, qb0);
+        const bool f1 = prefilter<MODE>(Pp, rr2, Xp, qprobe, Q1, x1, qa1, qb1) && two;
+        push_pairs<MODE, PREC>(a, g, L, f0, f1, pp, qn, q2n, acc);
+      }
+      // ---- drain both queues before the slots are restaged
+      if (qn > 0) {
+        __syncthreads();
+        screen_batch<MODE, PREC>(a, g, L, 0u, qn, q2n, acc);
+        qn = 0;
+      }
+      if (q2n > 0) {
+        __syncthreads();
+        full_test<MODE, PREC>(a, g, L, min((uint32_t)lane, q2n - 1u), (uint32_t)lane < q2n, acc);
+        q2n = 0;
+      }
+      __syncthreads();
     }
     if (GEN) {
-      const unsigned long long mk = __ballot(hit);
-      const unsigned long long gm = (K == 64) ? ~0ull : ((1ull << K) - 1ull);
-      const bool hitp = ((mk >> (p * K)) & gm) != 0ull;
-      const bool lead = (sub == 0) && have;
-      acc.kills += (lead && hitp) ? 1u : 0u;
+      const bool hitp = have && L.hit[lane] != 0u;
+      acc.kills += hitp ? 1u : 0u;
       // branch-free result store: lanes without a result write a private
       // LDS slot instead
       if (MODE == PM_GEN_OLD) {
-        uint32_t* dst = (lead && hitp) ? &a.alive_out[pidx] : &tDummy[lane];
+        uint32_t* dst = hitp ? &a.alive_out[pidx] : &L.dummy[lane];
         *dst = 0u;
       } else {
-        uint32_t* dst = (have && !hitp) ? &a.surv[X1] : &tDummy[lane];
+        uint32_t* dst = (have && !hitp) ? &a.surv[X1] : &L.dummy[lane];
         *dst = 1u;
       }
     }
+    __syncthreads();
   }
 }
 
 template <int MODE, int PREC>
-__global__ void __launch_bounds__(PAIR_THREADS) k_pair(PairArgs a) {
-  __shared__ float4 tP[WAVE];
-  __shared__ float4 tD[WAVE];
-  __shared__ uint32_t tX[WAVE];
-  __shared__ uint32_t tDummy[WAVE];
+__global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
+  __shared__ PairLds L;
   constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
   const uint32_t W = a.ncells_total;
   const int xcd = blockIdx.x & 7;
@@ -353,7 +622,7 @@ __global__ void __launch_bounds__(PAIR_THREADS) k_pair(PairArgs a) {
           cur_inst = inst;
         }
       }
-      process_cell<MODE, PREC>(a, g, gc, tP, tD, tX, tDummy, acc);
+      process_cell<MODE, PREC>(a, g, gc, L, acc);
     }
   }
   if (!GEN || cur_inst != 0xFFFFFFFFu) flush<GEN>(a, g, acc, MODE);
@@ -380,15 +649,13 @@ void launch_pair(int mode, int prec, const PairArgs& a, hipStream_t s) {
     switch (mode) {
       case PM_GEN_OLD: NF_LAUNCH(PM_GEN_OLD, NF_PREC_FP64); break;
       case PM_GEN_NEW: NF_LAUNCH(PM_GEN_NEW, NF_PREC_FP64); break;
-      case PM_CNT_COUNT: NF_LAUNCH(PM_CNT_COUNT, NF_PREC_FP64); break;
-      default: NF_LAUNCH(PM_CNT_RECORD, NF_PREC_FP64); break;
+      default: NF_LAUNCH(PM_CNT, NF_PREC_FP64); break;
     }
   } else {
     switch (mode) {
       case PM_GEN_OLD: NF_LAUNCH(PM_GEN_OLD, NF_PREC_FP32_RECHECK); break;
       case PM_GEN_NEW: NF_LAUNCH(PM_GEN_NEW, NF_PREC_FP32_RECHECK); break;
-      case PM_CNT_COUNT: NF_LAUNCH(PM_CNT_COUNT, NF_PREC_FP32_RECHECK); break;
-      default: NF_LAUNCH(PM_CNT_RECORD, NF_PREC_FP32_RECHECK); break;
+      default: NF_LAUNCH(PM_CNT, NF_PREC_FP32_RECHECK); break;
     }
   }
 #undef NF_LAUNCH

@@ -123,6 +123,44 @@ void launch_row_sort(const uint32_t* rowptr, uint32_t n, uint32_t* J, float* gap
                      gap64, img);
 }
 
+// CSR over I from the unsorted contact list (the pair pass appends in any
+// order; the row sort makes every row canonical afterwards).
+__global__ void k_csr_count(const uint32_t* __restrict__ I, uint64_t c, uint32_t* __restrict__ cnt) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < c;
+       k += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[I[k]], 1u);
+}
+
+__global__ void k_csr_scatter(const uint32_t* __restrict__ I, const uint32_t* __restrict__ J,
+                              const float* __restrict__ gap, const double* __restrict__ gap64,
+                              const int32_t* __restrict__ img, uint64_t c,
+                              const uint32_t* __restrict__ rowptr, uint32_t* __restrict__ fill,
+                              uint32_t* __restrict__ oJ, float* __restrict__ ogap,
+                              double* __restrict__ ogap64, int32_t* __restrict__ oimg) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < c;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t i = I[k];
+    const uint32_t pos = rowptr[i] + atomicAdd(&fill[i], 1u);
+    oJ[pos] = J[k];
+    ogap[pos] = gap[k];
+    ogap64[pos] = gap64[k];
+    oimg[pos] = img[k];
+  }
+}
+
+void launch_csr_count(const uint32_t* I, uint64_t c, uint32_t* cnt, hipStream_t s) {
+  if (c == 0) return;
+  hipLaunchKernelGGL(k_csr_count, dim3(grid_for(c, 256)), dim3(256), 0, s, I, c, cnt);
+}
+
+void launch_csr_scatter(const uint32_t* I, const uint32_t* J, const float* gap, const double* gap64,
+                        const int32_t* img, uint64_t c, const uint32_t* rowptr, uint32_t* fill,
+                        uint32_t* oJ, float* ogap, double* ogap64, int32_t* oimg, hipStream_t s) {
+  if (c == 0) return;
+  hipLaunchKernelGGL(k_csr_scatter, dim3(grid_for(c, 256)), dim3(256), 0, s, I, J, gap, gap64, img, c,
+                     rowptr, fill, oJ, ogap, ogap64, oimg);
+}
+
 // ---------------------------------------------------------------------------
 // Histograms.  Angle bins by comparison with cos(bin edge): bin = number of
 // edges k in 1..nb-1 with cos(angle) <= cos(edge_k), found by binary search

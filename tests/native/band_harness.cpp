@@ -27,6 +27,22 @@ void nfh_pair_f32_batch(int n, const float* ra, const float* rb, const double* s
   }
 }
 
+void nfh_pair_screen_batch(int n, const float* ra, const float* rb, const double* sh, int contact,
+                           double sep_or_gap, double soft_frac, float ew, int* out) {
+  for (int i = 0; i < n; ++i) {
+    const float* a = ra + 8 * i;
+    const float* b = rb + 8 * i;
+    const float4 P1 = make_float4(a[0], a[1], a[2], 0.5f * a[6]);
+    const float4 D1 = make_float4(a[3], a[4], a[5], a[7]);
+    const float4 P2 = make_float4(b[0], b[1], b[2], 0.5f * b[6]);
+    const float4 D2 = make_float4(b[3], b[4], b[5], b[7]);
+    const float R32 = contact ? ((D1.w + D2.w) + (float)sep_or_gap)
+                              : ((float)soft_frac * (D1.w + D2.w) + (float)sep_or_gap);
+    out[i] = nf::pair_f32_screen(P1, D1, P2, D2, (float)sh[3 * i], (float)sh[3 * i + 1],
+                                 (float)sh[3 * i + 2], R32, ew);
+  }
+}
+
 double nfh_sqrt_cr(double x) { return nf::sqrt_cr(x); }
 
 }  // extern "C"

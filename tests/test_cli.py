@@ -1,0 +1,60 @@
+"""The command-line front end (reference verbs of src/NanoFiller.cpp:48-149)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import nanofiller_amd as nf
+
+CLI = os.path.join(os.path.dirname(nf.__file__), "nanofiller")
+PARAMS = ("SPAT_DISTR_FILE NONE 3 3 3 10 10 10\nINC_LEN_MED 1.0\nINC_RAD_MED 0.005\n"
+          "NUM_RODS 2000\nSTAT_DIST_LIM 0.01\nRNG_SEED 4\n")
+
+
+def _run(tmp_path, *verbs):
+    (tmp_path / "params.txt").write_text(PARAMS)
+    return subprocess.run([CLI, "params.txt", *verbs], cwd=tmp_path, capture_output=True,
+                          text=True, timeout=300)
+
+
+def test_cli_usage_and_state_file_refusal(tmp_path):
+    r = subprocess.run([CLI], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = _run(tmp_path, "write_vtk")
+    assert r.returncode == 2 and "generate" in r.stderr
+
+
+def test_cli_reports_missing_device(tmp_path):
+    r = _run(tmp_path, "generate")
+    if r.returncode == 0:
+        pytest.skip("a GPU is visible")
+    assert r.returncode == 1 and "no HIP device" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_end_to_end(tmp_path):
+    r = _run(tmp_path, "generate", "write_vtk", "stat_phitheta", "stat_mutangle", "stat_contacts",
+             "stat_conduct", "0", "internal")
+    assert r.returncode == 0, r.stderr
+    assert "2000 inclusions generated!" in r.stdout
+    for f in ("output.vtk", "rho.dat", "gamma.dat", "internal.srt.coo", "source.elec.coo",
+              "sink.elec.coo"):
+        assert (tmp_path / f).exists(), f
+    # same packing as the library call with the same parameters
+    p = nf.normalize(nf.load_params(str(tmp_path / "params.txt")))
+    with nf.Packing(p) as pk:
+        pk.generate()
+        rods = pk.rods()
+        c = pk.contacts(p.contact_gap)
+        h = pk.histograms(bins=32, centre_axis=0, pol_axis=(0, 0, 1), phi_bins=32, theta_bins=32)
+    lines = (tmp_path / "output.vtk").read_text().splitlines()
+    pts = np.array([list(map(float, l.split())) for l in lines[5:5 + len(rods)]])
+    np.testing.assert_allclose(pts, rods[:, :3], rtol=1e-5, atol=1e-6)
+    gamma = (tmp_path / "gamma.dat").read_text().replace("gamma={", "").replace("}", "")
+    g = np.array([int(x) for x in gamma.replace(",", " ").split()])
+    np.testing.assert_array_equal(g, h["mutual"])
+    rho = (tmp_path / "rho.dat").read_text().replace("rho = {", "").replace("{", "").replace("}", "")
+    rv = np.array([int(x) for x in rho.replace(",", " ").split()])
+    np.testing.assert_array_equal(rv, h["phitheta"].ravel())
+    assert f"contacts (gap <= 0.01 um): {c.n};" in r.stdout

@@ -94,3 +94,33 @@ def test_batch_matches_single():
         assert sts[i].n_accepted == so["n_accepted"]
         assert sts[i].pair_tests == so["pair_tests"]
         np.testing.assert_array_equal(rods[i].view(np.uint32), ro.view(np.uint32))
+
+
+@pytest.mark.parametrize("cell_size", [-0.5, -1.4, 0.6])
+def test_cell_size_does_not_change_packing(cell_size):
+    """Accept decisions depend on the candidate stream only: any cell edge
+    (what the autotuner picks) gives the same packing and pair-test count."""
+    p = cases.small_cfg2(nf.NF_PREC_FP32_RECHECK, n_rods=3000)
+    q = p.copy()
+    q.cell_size = cell_size
+    with nf.Packing(p) as pk:
+        s0 = pk.generate()[0]
+        r0 = pk.rods()
+    with nf.Packing(q) as pk:
+        s1 = pk.generate()[0]
+        r1 = pk.rods()
+    np.testing.assert_array_equal(r0.view(np.uint32), r1.view(np.uint32))
+    assert s0.pair_tests == s1.pair_tests and s0.rounds == s1.rounds
+
+
+def test_long_rods_in_small_box_default_cells():
+    """Rods almost as long as the box: the default cell edge keeps several
+    cells per axis; the packing still equals the oracle's."""
+    p = nf.normalize(nf.default_params(box=(9.2, 9.2, 9.2), n_rods=2500, len=4.9, radius=0.005,
+                                       seed=3, stream=247, precision=nf.NF_PREC_FP32_RECHECK))
+    ro, so, _ = oracle.generate(p, threads=16)
+    with nf.Packing(p) as pk:
+        st = pk.generate()[0]
+        rg = pk.rods()
+    np.testing.assert_array_equal(rg.view(np.uint32), ro.view(np.uint32))
+    assert st.pair_tests == so["pair_tests"]
