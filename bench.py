@@ -46,6 +46,23 @@ K_OLD = nf.KERNEL_CLASSES.index("reject_old")
 K_NEW = nf.KERNEL_CLASSES.index("reject_new")
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_pair_generate_cfg2_500k_hbm_counters.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the reject-vs-accepted kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated
+    against a plain copy kernel), or None when the summary is absent."""
+    import re
+
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            m = re.search(r"traffic_bytes_per_launch k_pair<0, 1>: (\d+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -90,7 +107,8 @@ def leg_main(a, out: dict) -> None:
     out.update(value=rods_s, ms_per_step=ms_step, pair_s=pair_s, n_rods=p32.n_rods)
     out["roofline"] = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
+        "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT),
         "kernel": "k_pair<reject vs accepted, FP32+recheck>",
         "bytes_per_launch": sp.tile_bytes / launches, "avg_launch_ms": k_old / launches,
         "launches": launches,
