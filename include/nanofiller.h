@@ -191,6 +191,26 @@ typedef struct nf_hist_spec {
 int nf_histograms(nf_handle* h, const nf_hist_spec* spec, int64_t* out_centre,
                   int64_t* out_polar, int64_t* out_mutual, int64_t* out_phitheta);
 
+/* Centre-centre distance statistics, statKernel_CENT (inc/statKernel_CENT.h:
+ * 8-122) driven by collectStats (src/statistics_collection.cu:866-1033), on
+ * instance 0: every unordered pair of distinct rods over all periodic images
+ * whose centre distance d is below bins * width (at most 8 box edges).
+ * counts[k] (int64, exact) holds d in [k width, (k+1) width); *n_pairs the
+ * pair count; *sum_d_q24 and *sum_d2_q16 the exact sums of round(d * 2^24)
+ * and round(d^2 * 2^16) (mean = sum_d_q24 / 2^24 / n_pairs; the reference's
+ * float accumulators of d and d^2).  bins <= 1024.  Any pointer may be NULL. */
+int nf_centre_distance(nf_handle* h, int32_t bins, double width, int64_t* counts,
+                       uint64_t* n_pairs, uint64_t* sum_d_q24, uint64_t* sum_d2_q16);
+
+/* Van der Waals energy, statKernel_VdW_naive (inc/statKernel_MUT.h:531-628)
+ * driven by collect_vdw (src/statistics_collection.cu:53-154), over the
+ * contacts of the last nf_contacts call with dist_limit = that call's gap:
+ * pairs with interior closest points and gap < dist_limit contribute
+ * excA / theta / a1 * curve2surface_limit(gap, dist_limit, a1, a2, A, B,
+ * theta) (lengths in Angstrom; the reference's constants are A = 15.2,
+ * B = 24100).  FP64, summed in a fixed order. */
+int nf_vdw(nf_handle* h, double A, double B, double* energy, uint64_t* n_pairs);
+
 /* ---- network: collect_cunduct_data (src/conductivity.cu:140-423) and
  *      adjacency_graph (src/adjacency_graph.cu:57-219) --------------------- */
 

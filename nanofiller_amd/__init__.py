@@ -164,6 +164,11 @@ _SIGS = [
     ("nf_get_contact_gap64", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_contacts_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("nf_centre_distance", C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p,
+                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64)]),
+    ("nf_vdw", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double),
+                         C.POINTER(C.c_uint64)]),
     ("nf_histograms", C.c_int, [C.c_void_p, C.POINTER(HistSpec), C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p]),
     ("nf_network", C.c_int, [C.c_void_p, C.POINTER(NetworkInfo)]),
@@ -345,6 +350,25 @@ class Packing:
         if pt is not None:
             out["phitheta"] = pt.reshape(theta_bins, phi_bins)
         return out
+
+    def centre_distance(self, bins: int, width: float) -> dict:
+        """Centre-centre distance histogram (statKernel_CENT) with exact
+        fixed-point sums; mean and variance derived from them."""
+        counts = np.zeros(bins, np.int64)
+        n, s1, s2 = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().nf_centre_distance(self._h, bins, width, counts.ctypes.data, C.byref(n),
+                                        C.byref(s1), C.byref(s2)), self._h)
+        npairs = n.value
+        mean = s1.value / 16777216.0 / npairs if npairs else 0.0
+        var = s2.value / 65536.0 / npairs - mean * mean if npairs else 0.0
+        return {"counts": counts, "pairs": npairs, "sum_d_q24": s1.value, "sum_d2_q16": s2.value,
+                "mean": mean, "var": var}
+
+    def vdw(self, A: float = 15.2, B: float = 24100.0) -> tuple[float, int]:
+        """Van der Waals energy over the last contact list (statKernel_VdW_naive)."""
+        e, n = C.c_double(), C.c_uint64()
+        _check(lib().nf_vdw(self._h, A, B, C.byref(e), C.byref(n)), self._h)
+        return e.value, n.value
 
     def network(self) -> tuple[dict, dict]:
         info = NetworkInfo()

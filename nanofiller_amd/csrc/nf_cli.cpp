@@ -12,13 +12,15 @@
 // stat_mutangle     gamma.dat, 32 bins of the mutual angle of contacting
 //                   pairs (collect_1D_statDistr_weightedFloat, :640-765)
 // stat_contacts     contacts per rod: mean and maximum (collect_CntctPerInc)
+// stat_vdw          van der Waals energy of the contacting pairs (collect_vdw,
+//                   src/statistics_collection.cu:53-154)
 // stat_conduct d    contacts, network along axis d, COO files internal.srt.coo
 //                   / source.elec.coo / sink.elec.coo (collect_cunduct_data,
 //                   src/conductivity.cu:140-423; "internal" is accepted)
 //
 // The reference reads output.bin when generate is absent; state files are
 // not part of this library, so generate must come first.  Verbs outside
-// the hot path (plot, expand, reduce, stat_vdw, stat_nematic, write_ort,
+// the hot path (plot, expand, reduce, stat_nematic, write_ort,
 // write_lmp, stat_exclsurf) are reported as unsupported.
 #include <cmath>
 #include <cstdio>
@@ -66,7 +68,7 @@ int main(int argc, char** argv) {
   if (argc < 3) {
     std::fprintf(stderr,
                  "usage: %s params.txt generate [write_vtk] [stat_phitheta] [stat_mutangle]\n"
-                 "       [stat_contacts] [stat_conduct d [internal]]\n",
+                 "       [stat_contacts] [stat_vdw] [stat_conduct d [internal]]\n",
                  argv[0]);
     return 2;
   }
@@ -159,6 +161,17 @@ int main(int argc, char** argv) {
       for (uint32_t x : per) mx = x > mx ? x : mx;
       std::printf("contacts (gap <= %g um): %llu; per inclusion: mean %.4f, max %u\n", p.contact_gap,
                   (unsigned long long)nc, n ? 2.0 * (double)nc / n : 0.0, mx);
+    } else if (!std::strcmp(v, "stat_vdw")) {
+      rc = ensure_contacts();
+      double ev = 0.0;
+      uint64_t np_ = 0;
+      if (rc == NF_OK) rc = nf_vdw(h, 15.2, 24100.0, &ev, &np_);
+      if (rc != NF_OK) {
+        status = die(h, "stat_vdw", rc);
+        break;
+      }
+      std::printf("van der Waals energy over %llu contacting pairs (gap < %g um): %.10e\n",
+                  (unsigned long long)np_, p.contact_gap, ev);
     } else if (!std::strcmp(v, "stat_conduct")) {
       if (ai + 1 >= argc) {
         std::fprintf(stderr, "nanofiller: stat_conduct needs an axis 0|1|2\n");

@@ -262,6 +262,114 @@ NF_HD double pair_d2_f64(const float4 P1, const float4 D1, const float4 P2, cons
   return (vx * vx + vy * vy) + vz * vz;
 }
 
+// Deterministic atan from +, -, *, / and the correctly rounded sqrt: two
+// half-angle reductions atan(a) = 2 atan(a / (1 + sqrt(1 + a^2))) bring
+// |a| <= tan(pi/16), then the alternating series to a^35; |x| > 1 uses
+// pi/2 - atan(1/x).  CPU and GPU produce identical bits.
+NF_HD double datan(double x) {
+  if (x != x) return x;
+  const bool neg = x < 0.0;
+  double a = neg ? -x : x;
+  const bool inv = a > 1.0;
+  if (inv) a = 1.0 / a;
+  a = a / (1.0 + sqrt_cr(1.0 + a * a));
+  a = a / (1.0 + sqrt_cr(1.0 + a * a));
+  const double t = a * a;
+  double s = 1.0 / 35.0;
+  s = 1.0 / 33.0 - t * s;
+  s = 1.0 / 31.0 - t * s;
+  s = 1.0 / 29.0 - t * s;
+  s = 1.0 / 27.0 - t * s;
+  s = 1.0 / 25.0 - t * s;
+  s = 1.0 / 23.0 - t * s;
+  s = 1.0 / 21.0 - t * s;
+  s = 1.0 / 19.0 - t * s;
+  s = 1.0 / 17.0 - t * s;
+  s = 1.0 / 15.0 - t * s;
+  s = 1.0 / 13.0 - t * s;
+  s = 1.0 / 11.0 - t * s;
+  s = 1.0 / 9.0 - t * s;
+  s = 1.0 / 7.0 - t * s;
+  s = 1.0 / 5.0 - t * s;
+  s = 1.0 / 3.0 - t * s;
+  s = 1.0 - t * s;
+  double r = 4.0 * (a * s);
+  if (inv) r = 1.57079632679489661923 - r;
+  return neg ? -r : r;
+}
+
+// Van der Waals energy of one contacting pair, the model of
+// statKernel_VdW_naive (inc/statKernel_MUT.h:531-628): pairs whose closest
+// points are interior to both rods and whose surface gap is below dlim
+// contribute  excA / theta / a1 * curve2surface_limit(gap, dlim, a1, a2,
+// A, B, theta)  with lengths in Angstrom (x 1e4), theta = atan(a1 / (d - a2
+// - dlim)), the excluded area of exclArea_naive (:800-842) and the
+// curve-to-surface integral of curve2surface_limit (:891-894).  Rod 2 is
+// shifted by s.  Evaluated in FP64 in a fixed order (CPU and GPU agree bit
+// for bit).  Deviation: the skew excluded area uses the second rod's radius
+// where exclArea_naive reads its length (exclArea_skew, :869-871, uses the
+// radius).  Returns false when the pair does not contribute.
+NF_HD bool vdw_pair(const float4 P1, const float4 D1, const float4 P2, const float4 D2,
+                    double sx, double sy, double sz, double dlim, double A, double B,
+                    double* energy) {
+  const double PI = 3.14159265358979323846;
+  const double c1x = (double)P1.x, c1y = (double)P1.y, c1z = (double)P1.z;
+  const double c2x = (double)P2.x + sx, c2y = (double)P2.y + sy, c2z = (double)P2.z + sz;
+  const double u1x = (double)D1.x, u1y = (double)D1.y, u1z = (double)D1.z;
+  const double u2x = (double)D2.x, u2y = (double)D2.y, u2z = (double)D2.z;
+  const double h1 = (double)P1.w, h2 = (double)P2.w, a1 = (double)D1.w, a2 = (double)D2.w;
+  const double wx = c1x - c2x, wy = c1y - c2y, wz = c1z - c2z;
+  const double aa = u1x * u1x + u1y * u1y + u1z * u1z;
+  const double ee = u2x * u2x + u2y * u2y + u2z * u2z;
+  const double bb = u1x * u2x + u1y * u2y + u1z * u2z;
+  const double cg = bb / sqrt_cr(aa * ee);
+  const double sg2 = 1.0 - cg * cg;
+  const double sg = sg2 > 0.0 ? sqrt_cr(sg2) : 0.0;
+  const double eps = 1e-6;
+  double excA = 0.0, dline;
+  if (sg > eps) {
+    const double d1 = u1x * wx + u1y * wy + u1z * wz;
+    const double d2 = u2x * wx + u2y * wy + u2z * wz;
+    const double den = aa * ee - bb * bb;
+    const double t1 = (bb * d2 - ee * d1) / den;
+    const double t2 = (aa * d2 - bb * d1) / den;
+    if (!(fabs(t1) <= h1 && fabs(t2) <= h2)) return false;  // closest points not interior
+    const double px = (wx + t1 * u1x) - t2 * u2x;
+    const double py = (wy + t1 * u1y) - t2 * u2y;
+    const double pz = (wz + t1 * u1z) - t2 * u2z;
+    dline = sqrt_cr(px * px + py * py + pz * pz);
+    const double ea = 2.0 * a1 / sg, eb = 2.0 * a2 / sg;
+    excA = ea * eb * sg;
+    const double cx = h1 - fabs(t1), cy = h2 - fabs(t2);
+    if (fabs(cx) < ea || fabs(cy) < eb) excA = excA / 2.0;
+    if (cx <= -ea || cy <= -eb) excA = 0.0;
+  } else {
+    // parallel: distance of the axes, overlap of the projections on axis 1
+    const double n1 = sqrt_cr(aa);
+    const double pr = (-(u1x * wx + u1y * wy + u1z * wz)) / n1;  // (c2 - c1) . u1
+    const double qx = -wx - pr * u1x / n1, qy = -wy - pr * u1y / n1, qz = -wz - pr * u1z / n1;
+    dline = sqrt_cr(qx * qx + qy * qy + qz * qz);
+    const double tip0 = fabs(pr) - h2, tip1 = tip0 + 2.0 * h2;
+    if (tip1 < -h1 || tip0 > h1) return false;  // axes do not overlap: not interior
+    if (tip0 <= -h1 + eps && tip1 >= h1 - eps) excA = 2.0 * a1 * (2.0 * h1);
+    if (tip0 > h1 - eps) excA = 0.0;
+    if (tip0 > -h1 + eps && tip1 < h1 - eps) excA = 2.0 * 2.0 * a1 * ((tip1 < h1 ? tip1 : h1) - tip0);
+  }
+  const double gap = (dline - a1) - a2;
+  if (!(gap < dlim)) return false;
+  const double theta = datan(a1 / ((dline - a2) - dlim));
+  // curve2surface_limit in Angstrom
+  const double d = gap * 1e4, dl = dlim * 1e4, R1 = a1 * 1e4, R2 = a2 * 1e4;
+  const double q = 1.0 - theta * d / dl / dl / (R1 / R2 + 1.0);
+  const double d4 = (d * d) * (d * d), dl4 = (dl * dl) * (dl * dl);
+  const double d10 = d4 * d4 * (d * d), dl10 = dl4 * dl4 * (dl * dl);
+  const double q4 = (q * q) * (q * q);
+  const double c2s = -A * PI * R1 / 2.0 * theta * (1.0 / d4 - 1.0 / dl4 * (1.0 / q)) +
+                     B * 2.0 * PI * R1 / 5.0 * theta * (1.0 / d10 - 1.0 / dl10 * (1.0 / q4));
+  *energy = excA / theta / a1 * c2s;
+  return true;
+}
+
 // --------------------------------------------------------------------------
 // Capsule predicate, FP32 with a rigorous error band.
 //

@@ -368,3 +368,142 @@ extern "C" int nf_histograms(nf_handle* h, const nf_hist_spec* spec, int64_t* ou
   if (two_d) std::memcpy(out_phitheta, hout.data() + 3 * nb, sizeof(int64_t) * npb * ntb);
   return NF_OK;
 }
+
+// Centre-centre distance statistics of instance 0 (statKernel_CENT via
+// collectStats, src/statistics_collection.cu:866-1033).
+extern "C" int nf_centre_distance(nf_handle* h, int32_t bins, double width, int64_t* counts,
+                                  uint64_t* n_pairs, uint64_t* sum_d_q24, uint64_t* sum_d2_q16) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  if (bins < 1 || bins > CENT_BINS_MAX || !(width > 0.0))
+    return fail(h, NF_E_INVALID, "need 1 <= bins <= 1024 and width > 0");
+  if (h->params.size() != 1) return fail(h, NF_E_INVALID, "statistics run on single-instance handles");
+  const nf_params& p = h->params[0];
+  const uint32_t n = h->n_rods[0];
+  const double rmax = bins * width;
+  const double bmin = std::min(p.box[0], std::min(p.box[1], p.box[2]));
+  if (rmax > 8.0 * bmin) return fail(h, NF_E_INVALID, "bins * width exceeds 8 box edges");
+  hipStream_t s = h->stream;
+  const float4* P = h->gb.bP + h->desc[0].rod_off;
+  const float4* D = h->gb.bD + h->desc[0].rod_off;
+  // grid: edge about rmax (half stencil of +-1 cells), finer when crowded
+  const double edge = default_cell_edge(rmax, (double)n / (p.box[0] * p.box[1] * p.box[2]));
+  int32_t nc[3];
+  double scale[3];
+  uint64_t ncells = 1;
+  for (int a = 0; a < 3; ++a) {
+    double k = std::floor(p.box[a] / edge);
+    if (k < 1.0) k = 1.0;
+    if (k > 1024.0) k = 1024.0;
+    nc[a] = (int32_t)k;
+    scale[a] = (double)nc[a] / p.box[a];
+    ncells *= (uint64_t)nc[a];
+  }
+  if (ncells > (1ull << 26)) return fail(h, NF_E_INVALID, "statistics grid too large");
+  std::vector<int4> st = build_stencil(nc, p.box, rmax, true);
+  std::vector<double> e2(bins + 1);
+  for (int k = 0; k <= bins; ++k) e2[k] = ((double)k * width) * ((double)k * width);
+  float4* cP = nullptr;
+  float4* cD = nullptr;
+  uint32_t *cid = nullptr, *cstart = nullptr, *ccount = nullptr, *ccur = nullptr, *tmp = nullptr;
+  float* chl = nullptr;
+  int4* dst = nullptr;
+  double* de2 = nullptr;
+  unsigned long long* out = nullptr;
+  int rc = NF_OK;
+  const uint64_t n1 = std::max<uint64_t>(n, 1);
+  if (rc == NF_OK) rc = dalloc(h, &cP, n1);
+  if (rc == NF_OK) rc = dalloc(h, &cD, n1);
+  if (rc == NF_OK) rc = dalloc(h, &cid, n1);
+  if (rc == NF_OK) rc = dalloc(h, &cstart, ncells + 1);
+  if (rc == NF_OK) rc = dalloc(h, &ccount, ncells);
+  if (rc == NF_OK) rc = dalloc(h, &ccur, ncells);
+  if (rc == NF_OK) rc = dalloc(h, &chl, ncells);
+  if (rc == NF_OK) rc = dalloc(h, &tmp, scan_tmp_words(ncells));
+  if (rc == NF_OK) rc = dalloc(h, &dst, st.size());
+  if (rc == NF_OK) rc = dalloc(h, &de2, (size_t)bins + 1);
+  if (rc == NF_OK) rc = dalloc(h, &out, (size_t)bins + 3);
+  auto release = [&]() {
+    void* w[] = {cP, cD, cid, cstart, ccount, ccur, chl, tmp, dst, de2, out};
+    for (void* q : w) dfree(h, q);
+  };
+  if (rc != NF_OK) {
+    release();
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(dst, st.data(), sizeof(int4) * st.size(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(de2, e2.data(), sizeof(double) * e2.size(), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) {
+    release();
+    return fail(h, NF_E_HIP, std::string("statistics upload: ") + hipGetErrorString(e));
+  }
+  fill_u32(ccount, 0u, ncells, s);
+  fill_u32((uint32_t*)chl, 0u, ncells, s);
+  fill_u32(ccur, 0u, ncells, s);
+  fill_u32((uint32_t*)out, 0u, 2 * ((uint64_t)bins + 3), s);
+  launch_cell_count(P, n, nc, scale, ccount, chl, s);
+  scan_u32(ccount, cstart, ncells, tmp, s);
+  launch_cell_scatter(P, D, n, nc, scale, cstart, ccur, cP, cD, cid, s);
+  const double Lmax = std::max(p.box[0], std::max(p.box[1], p.box[2]));
+  const float ew = (float)(5.9604644775390625e-8 * (Lmax + rmax) * 1.01);
+  const float rf = (float)rmax + 8.0f * ew;
+  launch_cent(cP, cid, cstart, ccount, dst, (uint32_t)st.size(), nc, p.box, de2, bins,
+              rf * rf * 1.00001f, out, s);
+  std::vector<unsigned long long> hout((size_t)bins + 3);
+  e = hipMemcpyAsync(hout.data(), out, sizeof(unsigned long long) * hout.size(), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = sync_stream(s, "centre distance");
+  if (e == hipSuccess) e = hipGetLastError();
+  release();
+  if (e != hipSuccess) return fail(h, NF_E_HIP, std::string("centre distance: ") + hipGetErrorString(e));
+  if (counts)
+    for (int k = 0; k < bins; ++k) counts[k] = (int64_t)hout[k];
+  if (n_pairs) *n_pairs = hout[bins];
+  if (sum_d_q24) *sum_d_q24 = hout[bins + 1];
+  if (sum_d2_q16) *sum_d2_q16 = hout[bins + 2];
+  return NF_OK;
+}
+
+// Van der Waals energy over the contacts of the last nf_contacts call
+// (collect_vdw, src/statistics_collection.cu:53-154).
+extern "C" int nf_vdw(nf_handle* h, double A, double B, double* energy, uint64_t* n_pairs) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  ContactBuffers& c = h->cb;
+  if (!c.valid) return fail(h, NF_E_STATE, "no contacts computed");
+  if (!(c.gap_cut > 0.0)) return fail(h, NF_E_INVALID, "the contact gap must be > 0");
+  hipStream_t s = h->stream;
+  const uint32_t n = c.n_rods;
+  const float4* P = h->gb.bP + h->desc[0].rod_off;
+  const float4* D = h->gb.bD + h->desc[0].rod_off;
+  double *rowsum = nullptr, *bsum = nullptr, *tot = nullptr;
+  unsigned long long* pairs = nullptr;
+  int rc = NF_OK;
+  const uint64_t nb = ((uint64_t)n + 1023) / 1024;
+  if (rc == NF_OK) rc = dalloc(h, &rowsum, std::max<uint64_t>(n, 1));
+  if (rc == NF_OK) rc = dalloc(h, &bsum, std::max<uint64_t>(nb, 1));
+  if (rc == NF_OK) rc = dalloc(h, &tot, 1);
+  if (rc == NF_OK) rc = dalloc(h, &pairs, 1);
+  auto release = [&]() {
+    dfree(h, rowsum);
+    dfree(h, bsum);
+    dfree(h, tot);
+    dfree(h, pairs);
+  };
+  if (rc != NF_OK) {
+    release();
+    return rc;
+  }
+  fill_u32((uint32_t*)pairs, 0u, 2, s);
+  launch_vdw(P, D, n, c.rowptr, c.J, c.img, h->params[0].box, c.gap_cut, A, B, rowsum, bsum, tot,
+             pairs, s);
+  double e_h = 0.0;
+  unsigned long long p_h = 0;
+  hipError_t e = hipMemcpyAsync(&e_h, tot, sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&p_h, pairs, sizeof(p_h), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = sync_stream(s, "van der Waals");
+  if (e == hipSuccess) e = hipGetLastError();
+  release();
+  if (e != hipSuccess) return fail(h, NF_E_HIP, std::string("van der Waals: ") + hipGetErrorString(e));
+  if (energy) *energy = e_h;
+  if (n_pairs) *n_pairs = p_h;
+  return NF_OK;
+}

@@ -215,6 +215,14 @@ void launch_hist(const float4* P, const float4* D, uint32_t n, const uint32_t* r
                  const double* edges_t, int nb, int npb, int ntb, int axis, double L,
                  const double* mu, unsigned long long* out, hipStream_t s);
 void launch_selftest(int which, const double* in, double* out, uint64_t n, hipStream_t s);
+constexpr int CENT_BINS_MAX = 1024;
+void launch_cent(const float4* P, const uint32_t* X, const uint32_t* start, const uint32_t* count,
+                 const int4* st, uint32_t st_n, const int32_t nc[3], const double box[3],
+                 const double* e2, int bins, float r2f, unsigned long long* out, hipStream_t s);
+void launch_vdw(const float4* P, const float4* D, uint32_t n, const uint32_t* rowptr,
+                const uint32_t* J, const int32_t* img, const double box[3], double dlim, double A,
+                double B, double* rowsum, double* bsum, double* out, unsigned long long* pairs,
+                hipStream_t s);
 
 // ---- network extraction (nf_net.hip) ----
 enum NetSummary {

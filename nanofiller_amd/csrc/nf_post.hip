@@ -162,6 +162,212 @@ void launch_csr_scatter(const uint32_t* I, const uint32_t* J, const float* gap, 
 }
 
 // ---------------------------------------------------------------------------
+// Centre-centre distance statistics (statKernel_CENT, inc/statKernel_CENT.h:
+// 8-122, driven by collectStats, src/statistics_collection.cu:866-1033):
+// every unordered pair of distinct rods over all periodic images with centre
+// distance below bins * width, binned by comparing d^2 with the squared bin
+// edges, plus exact fixed-point sums of round(d 2^24) and round(d^2 2^16).
+// One workgroup (one wave) per home cell over a half stencil; each neighbour
+// cell streams through LDS in 64-rod tiles; the FP64 distance is evaluated
+// with the lower id first and the image applied to the higher id, so every
+// bin decision matches a sequential evaluation.
+// ---------------------------------------------------------------------------
+constexpr int CENT_MAX_BINS = 1024;
+
+// wave-uniform copies (SGPR) of values every lane already agrees on, so the
+// loops that hold barriers have scalar trip counts
+__device__ __forceinline__ uint32_t uni_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+struct CentArg {
+  const float4* P;       // compact cell list (positions, half length)
+  const uint32_t* X;     // rod ids
+  const uint32_t* start;
+  const uint32_t* count;
+  const int4* st;
+  uint32_t st_n;
+  int32_t nc[3];
+  double box[3];
+  const double* e2;      // bins + 1 squared edges
+  int bins;
+  float r2f;             // FP32 prefilter bound
+  unsigned long long* out;  // bins counts, then pairs, sum_q, sum2_q
+};
+
+__global__ void __launch_bounds__(WAVE) k_cent(CentArg a) {
+  __shared__ float4 tP[WAVE];
+  __shared__ uint32_t tX[WAVE];
+  __shared__ uint32_t hist[CENT_MAX_BINS];
+  const int lane = threadIdx.x;
+  for (int k = lane; k < a.bins; k += WAVE) hist[k] = 0u;
+  __syncthreads();
+  unsigned long long cnt = 0, sq = 0, s2q = 0;
+  const uint32_t ncells = (uint32_t)a.nc[0] * (uint32_t)a.nc[1] * (uint32_t)a.nc[2];
+  const double e2max = a.e2[a.bins];
+  for (uint32_t cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+    const uint32_t pc = uni_u32(a.count[cell]), ps = uni_u32(a.start[cell]);
+    const int c0 = (int)(cell % (uint32_t)a.nc[0]);
+    const int c1 = (int)((cell / (uint32_t)a.nc[0]) % (uint32_t)a.nc[1]);
+    const int c2 = (int)(cell / ((uint32_t)a.nc[0] * (uint32_t)a.nc[1]));
+    for (uint32_t pb = 0; pb < pc; pb += WAVE) {
+      const bool valid = pb + (uint32_t)lane < pc;
+      const uint32_t pi = ps + min(pb + (uint32_t)lane, pc - 1u);
+      const float4 P1 = a.P[pi];
+      const uint32_t X1 = a.X[pi];
+      for (uint32_t si = 0; si < a.st_n; ++si) {
+        const int4 so = a.st[si];
+        const int raw[3] = {c0 + so.x, c1 + so.y, c2 + so.z};
+        int img[3], m[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          int mm = raw[d] % a.nc[d];
+          if (mm < 0) mm += a.nc[d];
+          img[d] = (raw[d] - mm) / a.nc[d];
+          m[d] = mm;
+        }
+        const uint32_t nc = (uint32_t)(m[0] + a.nc[0] * (m[1] + a.nc[1] * m[2]));
+        const uint32_t n = uni_u32(a.count[nc]), ns = uni_u32(a.start[nc]);
+        const bool same = so.x == 0 && so.y == 0 && so.z == 0;
+        const double sx = (double)img[0] * a.box[0], sy = (double)img[1] * a.box[1],
+                     sz = (double)img[2] * a.box[2];
+        const float fx = (float)sx, fy = (float)sy, fz = (float)sz;
+        for (uint32_t tb = 0; tb < n; tb += WAVE) {
+          const uint32_t tn = min((uint32_t)WAVE, n - tb);
+          const uint32_t q = ns + tb + min((uint32_t)lane, tn - 1u);
+          tP[lane] = a.P[q];
+          tX[lane] = a.X[q];
+          __syncthreads();
+          if (valid) {
+            for (uint32_t j = 0; j < tn; ++j) {
+              if (same && tb + j <= pb + (uint32_t)lane) continue;
+              const uint32_t xj = tX[j];
+              if (xj == X1) continue;  // a rod and its own image
+              const float4 P2 = tP[j];
+              const float wx = (P1.x - P2.x) - fx, wy = (P1.y - P2.y) - fy, wz = (P1.z - P2.z) - fz;
+              if (fmaf(wz, wz, fmaf(wy, wy, wx * wx)) > a.r2f) continue;
+              const bool swap = X1 > xj;
+              const float4 Pa = swap ? P2 : P1, Pb = swap ? P1 : P2;
+              const double ux = swap ? -sx : sx, uy = swap ? -sy : sy, uz = swap ? -sz : sz;
+              const double dx = ((double)Pa.x - (double)Pb.x) - ux;
+              const double dy = ((double)Pa.y - (double)Pb.y) - uy;
+              const double dz = ((double)Pa.z - (double)Pb.z) - uz;
+              const double d2 = dx * dx + dy * dy + dz * dz;
+              if (!(d2 < e2max)) continue;
+              int lo = 0, hi = a.bins;
+              while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (a.e2[mid] <= d2)
+                  lo = mid;
+                else
+                  hi = mid;
+              }
+              atomicAdd(&hist[lo], 1u);
+              ++cnt;
+              sq += (unsigned long long)floor(sqrt_cr(d2) * 16777216.0 + 0.5);
+              s2q += (unsigned long long)floor(d2 * 65536.0 + 0.5);
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = lane; k < a.bins; k += WAVE)
+    if (hist[k]) atomicAdd(&a.out[k], (unsigned long long)hist[k]);
+  atomicAdd(&a.out[a.bins + 0], cnt);
+  atomicAdd(&a.out[a.bins + 1], sq);
+  atomicAdd(&a.out[a.bins + 2], s2q);
+}
+
+void launch_cent(const float4* P, const uint32_t* X, const uint32_t* start, const uint32_t* count,
+                 const int4* st, uint32_t st_n, const int32_t nc[3], const double box[3],
+                 const double* e2, int bins, float r2f, unsigned long long* out, hipStream_t s) {
+  CentArg a;
+  a.P = P;
+  a.X = X;
+  a.start = start;
+  a.count = count;
+  a.st = st;
+  a.st_n = st_n;
+  for (int d = 0; d < 3; ++d) {
+    a.nc[d] = nc[d];
+    a.box[d] = box[d];
+  }
+  a.e2 = e2;
+  a.bins = bins;
+  a.r2f = r2f;
+  a.out = out;
+  const uint64_t ncells = (uint64_t)nc[0] * nc[1] * nc[2];
+  hipLaunchKernelGGL(k_cent, dim3(grid_for(ncells, 1)), dim3(WAVE), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Van der Waals energy over the sorted contact list (statKernel_VdW_naive,
+// inc/statKernel_MUT.h:531-628, driven by collect_vdw,
+// src/statistics_collection.cu:53-154): one thread per row sums its
+// contributing pairs in row order; rows are summed in blocks of 1024 in
+// order, block sums in order (deterministic).
+// ---------------------------------------------------------------------------
+__global__ void k_vdw_rows(const float4* __restrict__ P, const float4* __restrict__ D, uint32_t n,
+                           const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ J,
+                           const int32_t* __restrict__ img, double bx, double by, double bz,
+                           double dlim, double A, double B, double* __restrict__ rowsum,
+                           unsigned long long* __restrict__ pairs) {
+  unsigned long long local = 0;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    const float4 P1 = P[r], D1 = D[r];
+    for (uint32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+      const int32_t c = img[k];
+      const double sx = (double)(int)(int8_t)(c & 0xFF) * bx;
+      const double sy = (double)(int)(int8_t)((c >> 8) & 0xFF) * by;
+      const double sz = (double)(int)(int8_t)((c >> 16) & 0xFF) * bz;
+      double e;
+      if (vdw_pair(P1, D1, P[J[k]], D[J[k]], sx, sy, sz, dlim, A, B, &e)) {
+        s = s + e;
+        ++local;
+      }
+    }
+    rowsum[r] = s;
+  }
+  if (local) atomicAdd(pairs, local);
+}
+
+__global__ void k_block_sums(const double* __restrict__ v, uint64_t n, double* __restrict__ bsum) {
+  const uint64_t nb = (n + 1023) / 1024;
+  for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb;
+       b += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t e = (b + 1) * 1024 < n ? (b + 1) * 1024 : n;
+    double s = 0.0;
+    for (uint64_t k = b * 1024; k < e; ++k) s = s + v[k];
+    bsum[b] = s;
+  }
+}
+
+__global__ void k_block_total(const double* __restrict__ bsum, uint64_t nb, double* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double t = 0.0;
+  for (uint64_t b = 0; b < nb; ++b) t = t + bsum[b];
+  *out = t;
+}
+
+void launch_vdw(const float4* P, const float4* D, uint32_t n, const uint32_t* rowptr,
+                const uint32_t* J, const int32_t* img, const double box[3], double dlim, double A,
+                double B, double* rowsum, double* bsum, double* out, unsigned long long* pairs,
+                hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_vdw_rows, dim3(grid_for(n, 256)), dim3(256), 0, s, P, D, n, rowptr, J, img,
+                       box[0], box[1], box[2], dlim, A, B, rowsum, pairs);
+  const uint64_t nb = ((uint64_t)n + 1023) / 1024;
+  if (nb > 0)
+    hipLaunchKernelGGL(k_block_sums, dim3(grid_for(nb, 256)), dim3(256), 0, s, rowsum, (uint64_t)n,
+                       bsum);
+  hipLaunchKernelGGL(k_block_total, dim3(1), dim3(64), 0, s, bsum, nb, out);
+}
+
+// ---------------------------------------------------------------------------
 // Histograms.  Angle bins by comparison with cos(bin edge): bin = number of
 // edges k in 1..nb-1 with cos(angle) <= cos(edge_k), found by binary search
 // on the decreasing edge table; the counts are exact (LDS u32 per block,

@@ -66,6 +66,15 @@ def lib() -> C.CDLL:
             ("orc_histograms", C.c_int, [P(Params), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                          C.c_int64, P(HistSpec), C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p]),
+            ("orc_atan", C.c_double, [C.c_double]),
+            ("orc_vdw_pair", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                       C.c_double, P(C.c_double)]),
+            ("orc_vdw", C.c_uint64, [P(Params), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                     P(C.c_double)]),
+            ("orc_centre_distance", C.c_uint64, [P(Params), C.c_void_p, C.c_uint32, C.c_int32,
+                                                 C.c_double, C.c_void_p, P(C.c_uint64),
+                                                 P(C.c_uint64)]),
             ("orc_network", C.c_int64, [P(Params), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_int64, P(NetworkInfo)]),
@@ -199,3 +208,36 @@ def network(p, rods, I, J, gap64, image):
                       im.ctypes.data, len(I), labels.ctypes.data, row.ctypes.data, col.ctypes.data,
                       val.ctypes.data, nnz, C.byref(info))
     return info.to_dict(), {"labels": labels, "row_ptr": row, "col": col, "val": val}
+
+
+def atan(x: float) -> float:
+    return lib().orc_atan(float(x))
+
+
+def vdw_pair(r1, r2, shift, dlim, A=15.2, B=24100.0):
+    a = np.ascontiguousarray(r1, np.float32)
+    b = np.ascontiguousarray(r2, np.float32)
+    sh = np.ascontiguousarray(shift, np.float64)
+    e = C.c_double()
+    ok = lib().orc_vdw_pair(a.ctypes.data, b.ctypes.data, sh.ctypes.data, dlim, A, B, C.byref(e))
+    return (e.value if ok else None)
+
+
+def vdw(p, rods, I, J, image, dlim, A=15.2, B=24100.0):
+    r = np.ascontiguousarray(rods, np.float32)
+    I = np.ascontiguousarray(I, np.uint32)
+    J = np.ascontiguousarray(J, np.uint32)
+    im = np.ascontiguousarray(image, np.int32)
+    e = C.c_double()
+    n = lib().orc_vdw(C.byref(p), r.ctypes.data, r.shape[0], I.ctypes.data, J.ctypes.data,
+                      im.ctypes.data, len(I), dlim, A, B, C.byref(e))
+    return e.value, n
+
+
+def centre_distance(p, rods, bins, width):
+    r = np.ascontiguousarray(rods, np.float32)
+    counts = np.zeros(bins, np.int64)
+    s1, s2 = C.c_uint64(), C.c_uint64()
+    n = lib().orc_centre_distance(C.byref(p), r.ctypes.data, r.shape[0], bins, width,
+                                  counts.ctypes.data, C.byref(s1), C.byref(s2))
+    return {"counts": counts, "pairs": n, "sum_d_q24": s1.value, "sum_d2_q16": s2.value}

@@ -1107,3 +1107,163 @@ int64_t orc_network(const nf_params* p, const float* rods, uint32_t n, const uin
   free(tmp);
   return nnz;
 }
+
+/* ===================================================================== */
+/* atan, van der Waals energy, centre-centre distance statistics          */
+/* ===================================================================== */
+/* atan from +, -, *, / and the correctly rounded C sqrt: two half-angle
+ * reductions to |a| <= tan(pi/16), the alternating series to a^35, and
+ * pi/2 - atan(1/x) above 1 (the library evaluates the same operations). */
+double orc_atan(double x) {
+  if (x != x) return x;
+  int neg = x < 0.0;
+  double a = neg ? -x : x;
+  int inv = a > 1.0;
+  if (inv) a = 1.0 / a;
+  a = a / (1.0 + sqrt(1.0 + a * a));
+  a = a / (1.0 + sqrt(1.0 + a * a));
+  double t = a * a;
+  double s = 1.0 / 35.0;
+  static const double den[] = {33, 31, 29, 27, 25, 23, 21, 19, 17, 15, 13, 11, 9, 7, 5, 3, 1};
+  for (int k = 0; k < 17; ++k) s = 1.0 / den[k] - t * s;
+  double r = 4.0 * (a * s);
+  if (inv) r = 1.57079632679489661923 - r;
+  return neg ? -r : r;
+}
+
+/* One pair of the van der Waals model of statKernel_VdW_naive
+ * (inc/statKernel_MUT.h:531-628; exclArea_naive :800-842 with the second
+ * radius in the skew branch as exclArea_skew :869-871; curve2surface_limit
+ * :891-894), rod 2 shifted by sh.  Returns 1 and the energy when the pair's
+ * closest points are interior to both rods and its gap is below dlim. */
+int orc_vdw_pair(const float* r1, const float* r2, const double sh[3], double dlim, double A,
+                 double B, double* energy) {
+  const double PI = 3.14159265358979323846;
+  double c1[3] = {r1[0], r1[1], r1[2]}, c2[3], u1[3] = {r1[3], r1[4], r1[5]},
+         u2[3] = {r2[3], r2[4], r2[5]};
+  for (int d = 0; d < 3; ++d) c2[d] = (double)r2[d] + sh[d];
+  double h1 = (double)(0.5f * r1[6]), h2 = (double)(0.5f * r2[6]), a1 = r1[7], a2 = r2[7];
+  double w[3] = {c1[0] - c2[0], c1[1] - c2[1], c1[2] - c2[2]};
+  double aa = u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2];
+  double ee = u2[0] * u2[0] + u2[1] * u2[1] + u2[2] * u2[2];
+  double bb = u1[0] * u2[0] + u1[1] * u2[1] + u1[2] * u2[2];
+  double cg = bb / sqrt(aa * ee);
+  double sg2 = 1.0 - cg * cg;
+  double sg = sg2 > 0.0 ? sqrt(sg2) : 0.0;
+  const double eps = 1e-6;
+  double excA = 0.0, dline;
+  if (sg > eps) {
+    double d1 = u1[0] * w[0] + u1[1] * w[1] + u1[2] * w[2];
+    double d2 = u2[0] * w[0] + u2[1] * w[1] + u2[2] * w[2];
+    double den = aa * ee - bb * bb;
+    double t1 = (bb * d2 - ee * d1) / den;
+    double t2 = (aa * d2 - bb * d1) / den;
+    if (!(fabs(t1) <= h1 && fabs(t2) <= h2)) return 0;
+    double p[3];
+    for (int d = 0; d < 3; ++d) p[d] = (w[d] + t1 * u1[d]) - t2 * u2[d];
+    dline = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    double ea = 2.0 * a1 / sg, eb = 2.0 * a2 / sg;
+    excA = ea * eb * sg;
+    double cx = h1 - fabs(t1), cy = h2 - fabs(t2);
+    if (fabs(cx) < ea || fabs(cy) < eb) excA = excA / 2.0;
+    if (cx <= -ea || cy <= -eb) excA = 0.0;
+  } else {
+    double n1 = sqrt(aa);
+    double pr = (-(u1[0] * w[0] + u1[1] * w[1] + u1[2] * w[2])) / n1;
+    double q[3];
+    for (int d = 0; d < 3; ++d) q[d] = -w[d] - pr * u1[d] / n1;
+    dline = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    double tip0 = fabs(pr) - h2, tip1 = tip0 + 2.0 * h2;
+    if (tip1 < -h1 || tip0 > h1) return 0;
+    if (tip0 <= -h1 + eps && tip1 >= h1 - eps) excA = 2.0 * a1 * (2.0 * h1);
+    if (tip0 > h1 - eps) excA = 0.0;
+    if (tip0 > -h1 + eps && tip1 < h1 - eps) excA = 2.0 * 2.0 * a1 * ((tip1 < h1 ? tip1 : h1) - tip0);
+  }
+  double gap = (dline - a1) - a2;
+  if (!(gap < dlim)) return 0;
+  double theta = orc_atan(a1 / ((dline - a2) - dlim));
+  double d = gap * 1e4, dl = dlim * 1e4, R1 = a1 * 1e4, R2 = a2 * 1e4;
+  double qq = 1.0 - theta * d / dl / dl / (R1 / R2 + 1.0);
+  double d4 = (d * d) * (d * d), dl4 = (dl * dl) * (dl * dl);
+  double d10 = d4 * d4 * (d * d), dl10 = dl4 * dl4 * (dl * dl);
+  double q4 = (qq * qq) * (qq * qq);
+  double c2s = -A * PI * R1 / 2.0 * theta * (1.0 / d4 - 1.0 / dl4 * (1.0 / qq)) +
+               B * 2.0 * PI * R1 / 5.0 * theta * (1.0 / d10 - 1.0 / dl10 * (1.0 / q4));
+  *energy = excA / theta / a1 * c2s;
+  return 1;
+}
+
+/* Van der Waals energy over a contact list sorted by (I, J, image): per-row
+ * sums in row order, rows summed in blocks of 1024 in order, block sums in
+ * order.  Returns the contributing pair count. */
+uint64_t orc_vdw(const nf_params* p, const float* rods, uint32_t n, const uint32_t* I,
+                 const uint32_t* J, const int32_t* image, int64_t nc, double dlim, double A,
+                 double B, double* energy) {
+  double* rs = (double*)calloc(n ? n : 1, sizeof(double));
+  uint64_t cnt = 0;
+  for (int64_t k = 0; k < nc; ++k) {
+    int img[3];
+    unpack_img(image[k], img);
+    double sh[3] = {img[0] * p->box[0], img[1] * p->box[1], img[2] * p->box[2]};
+    double e;
+    if (orc_vdw_pair(&rods[8 * (size_t)I[k]], &rods[8 * (size_t)J[k]], sh, dlim, A, B, &e)) {
+      rs[I[k]] = rs[I[k]] + e;
+      ++cnt;
+    }
+  }
+  double tot = 0.0;
+  for (uint32_t b = 0; b < n; b += 1024) {
+    double s = 0.0;
+    uint32_t e = b + 1024 < n ? b + 1024 : n;
+    for (uint32_t r = b; r < e; ++r) s = s + rs[r];
+    tot = tot + s;
+  }
+  free(rs);
+  *energy = tot;
+  return cnt;
+}
+
+/* Centre-centre distance statistics in the manner of statKernel_CENT
+ * (inc/statKernel_CENT.h:8-122): every unordered pair of distinct rods, over
+ * all periodic images, whose centre distance d is below bins * width;
+ * counts[k] for d in [k width, (k+1) width) (decided by comparing d^2 with
+ * the squared edges), and exact fixed-point sums of round(d * 2^24) and
+ * round(d^2 * 2^16).  d is evaluated with the lower id first and the image
+ * applied to the higher id.  Brute force over all pairs. */
+uint64_t orc_centre_distance(const nf_params* p, const float* rods, uint32_t n, int32_t bins,
+                             double width, int64_t* counts, uint64_t* sum_q, uint64_t* sum2_q) {
+  double rmax = bins * width;
+  double* e2 = (double*)malloc(sizeof(double) * (size_t)(bins + 1));
+  for (int k = 0; k <= bins; ++k) e2[k] = ((double)k * width) * ((double)k * width);
+  int nim[3];
+  for (int d = 0; d < 3; ++d) nim[d] = (int)ceil(rmax / p->box[d]) + 1;
+  memset(counts, 0, sizeof(int64_t) * (size_t)bins);
+  uint64_t cnt = 0, sq = 0, s2q = 0;
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t j = i + 1; j < n; ++j)
+      for (int iz = -nim[2]; iz <= nim[2]; ++iz)
+        for (int iy = -nim[1]; iy <= nim[1]; ++iy)
+          for (int ix = -nim[0]; ix <= nim[0]; ++ix) {
+            const float* a = &rods[8 * (size_t)i];
+            const float* b = &rods[8 * (size_t)j];
+            double dx = ((double)a[0] - (double)b[0]) - ix * p->box[0];
+            double dy = ((double)a[1] - (double)b[1]) - iy * p->box[1];
+            double dz = ((double)a[2] - (double)b[2]) - iz * p->box[2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (!(d2 < e2[bins])) continue;
+            int lo = 0, hi = bins; /* largest k with e2[k] <= d2 */
+            while (hi - lo > 1) {
+              int mid = (lo + hi) / 2;
+              if (e2[mid] <= d2) lo = mid; else hi = mid;
+            }
+            counts[lo]++;
+            ++cnt;
+            double d = sqrt(d2);
+            sq += (uint64_t)floor(d * 16777216.0 + 0.5);
+            s2q += (uint64_t)floor(d2 * 65536.0 + 0.5);
+          }
+  free(e2);
+  *sum_q = sq;
+  *sum2_q = s2q;
+  return cnt;
+}

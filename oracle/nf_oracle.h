@@ -91,6 +91,18 @@ int64_t orc_network(const nf_params* p, const float* rods, uint32_t n,
                     uint32_t* row_ptr, uint32_t* col, double* val, int64_t cap,
                     nf_network_info* info);
 
+/* Deterministic atan (+, -, *, / and the correctly rounded sqrt). */
+double orc_atan(double x);
+/* Van der Waals energy of one pair / summed over a sorted contact list. */
+int orc_vdw_pair(const float* r1, const float* r2, const double sh[3], double dlim, double A,
+                 double B, double* energy);
+uint64_t orc_vdw(const nf_params* p, const float* rods, uint32_t n, const uint32_t* I,
+                 const uint32_t* J, const int32_t* image, int64_t nc, double dlim, double A,
+                 double B, double* energy);
+/* Centre-centre distance histogram and exact fixed-point sums (brute force). */
+uint64_t orc_centre_distance(const nf_params* p, const float* rods, uint32_t n, int32_t bins,
+                             double width, int64_t* counts, uint64_t* sum_q, uint64_t* sum2_q);
+
 #ifdef __cplusplus
 }
 #endif

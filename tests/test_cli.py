@@ -34,7 +34,7 @@ def test_cli_reports_missing_device(tmp_path):
 
 @pytest.mark.gpu
 def test_cli_end_to_end(tmp_path):
-    r = _run(tmp_path, "generate", "write_vtk", "stat_phitheta", "stat_mutangle", "stat_contacts",
+    r = _run(tmp_path, "generate", "write_vtk", "stat_phitheta", "stat_mutangle", "stat_contacts", "stat_vdw",
              "stat_conduct", "0", "internal")
     assert r.returncode == 0, r.stderr
     assert "2000 inclusions generated!" in r.stdout
