@@ -448,7 +448,7 @@ extern "C" int nf_create_batch(const nf_params* ps, int n, nf_handle** out) {
   ALLOC(b.pcount, b.total_cells);
   ALLOC(b.pstart, b.total_cells + 1);
   ALLOC(b.phl, b.total_cells);
-  ALLOC(b.queues, Q_N * QUEUES_PER_KERNEL);
+  ALLOC(b.queues, Q_N * QWORDS);
   ALLOC(b.scan_tmp, std::max(scan_tmp_words(b.total_cells), scan_tmp_words(b.T_cap)));
 #undef ALLOC
   if (rc != NF_OK) {

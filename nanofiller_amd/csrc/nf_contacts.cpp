@@ -87,7 +87,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   ALLOC(c.fill, n);
   ALLOC(ctr, 5);  // pair tests, rechecks, max dd, tile bytes, append count
   ALLOC(tmp, std::max(scan_tmp_words(ncells), scan_tmp_words((uint64_t)n + 1)));
-  ALLOC(queues, 2 * QUEUES_PER_KERNEL);
+  ALLOC(queues, 2 * QWORDS);
 #undef ALLOC
   if (rc != NF_OK) {
     dfree(h, ctr);
@@ -103,7 +103,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   fill_u32((uint32_t*)c.chl, 0u, ncells, s);
   fill_u32(c.ccur, 0u, ncells, s);
   fill_u32((uint32_t*)ctr, 0u, 10, s);
-  fill_u32(queues, 0u, 2 * QUEUES_PER_KERNEL, s);
+  fill_u32(queues, 0u, 2 * QWORDS, s);
   launch_cell_count(P, n, c.ncell, scale, c.ccount, c.chl, s);
   scan_u32(c.ccount, c.cstart, ncells, tmp, s);
   launch_cell_scatter(P, D, n, c.ncell, scale, c.cstart, c.ccur, c.cP, c.cD, c.cid, s);
@@ -182,7 +182,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
     }
     if (attempt > 0) {
       fill_u32((uint32_t*)ctr, 0u, 10, s);
-      fill_u32(queues, 0u, 2 * QUEUES_PER_KERNEL, s);
+      fill_u32(queues, 0u, 2 * QWORDS, s);
     }
     a.app_n = (uint32_t*)(ctr + 4);
     a.app_cap = (uint32_t)cap;

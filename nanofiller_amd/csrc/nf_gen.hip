@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(WAVE) k_ctl_begin(const InstDesc* __restrict__
     slot_base[n_inst] = total;
     ri->T = total;
   }
-  if (lane < Q_N * QUEUES_PER_KERNEL) queues[lane] = 0;
+  for (int q = lane; q < Q_N * QUEUES_PER_KERNEL; q += WAVE) queues[q * QSTRIDE] = 0;
 }
 
 __device__ __forceinline__ int find_inst(const uint32_t* __restrict__ base, int n, uint32_t g) {
@@ -300,7 +300,7 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   a.nbCount = b.ocount;
   a.nbHl = b.ohl;
   a.nbAlive = nullptr;
-  a.qctr = b.queues + Q_OLD * QUEUES_PER_KERNEL;
+  a.qctr = b.queues + Q_OLD * QWORDS;
   a.alive_out = b.palive;
   launch_pair(PM_GEN_OLD, prec, a, s);
   if (ev) (void)hipEventRecord(ev[4], s);
@@ -313,7 +313,7 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   a.nbCount = b.pcount;
   a.nbHl = b.phl;
   a.nbAlive = b.palive;
-  a.qctr = b.queues + Q_NEW * QUEUES_PER_KERNEL;
+  a.qctr = b.queues + Q_NEW * QWORDS;
   a.alive_out = nullptr;
   a.surv = b.surv;
   launch_pair(PM_GEN_NEW, prec, a, s);

@@ -54,8 +54,30 @@ struct RoundInfo {
   uint32_t pad;
 };
 
+// Wave reductions for the end of a kernel (every lane converged): sum and
+// max over the 64 lanes, result in every lane.
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o, WAVE);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
 // Queue counters: 8 per pair kernel launch (one per XCD chunk).
-constexpr int QUEUES_PER_KERNEL = 8;
+// Work queues of the pair kernels: 64 per launch, each counter on its own
+// 128-byte line (QSTRIDE words apart) so that grabs on different queues do
+// not serialise on one line.  Queue q covers the cell range q of 64 equal
+// ranges; the eight queues of XCD x are the ranges 8x .. 8x+7.
+constexpr int QUEUES_PER_KERNEL = 64;
+constexpr int QSTRIDE = 32;
+constexpr int QWORDS = QUEUES_PER_KERNEL * QSTRIDE;  // words per launch
 enum { Q_OLD = 0, Q_NEW = 1, Q_CNT = 2, Q_REC = 3, Q_N = 4 };
 
 // Device buffers of a generation batch.
@@ -92,7 +114,7 @@ struct GenBuffers {
   uint32_t* pcount = nullptr;  // total_cells
   uint32_t* pstart = nullptr;  // total_cells + 1
   float* phl = nullptr;
-  uint32_t* queues = nullptr;  // Q_N * 8
+  uint32_t* queues = nullptr;  // Q_N * QWORDS
   uint32_t* scan_tmp = nullptr;
   uint64_t total_cells = 0, total_old = 0, T_cap = 0, total_rods = 0;
   int n_inst = 0;

@@ -128,7 +128,9 @@ __global__ void k_net_edges(NetArgs a) {
       uf_union(a.par, r, j);
     }
   }
-  if (local) atomicAdd(&a.summary[SUM_EDGES], local);
+  // one atomic per wave, after the loop (all lanes converged)
+  local = wave_sum_u64(local);
+  if ((threadIdx.x & (WAVE - 1)) == 0 && local) atomicAdd(&a.summary[SUM_EDGES], local);
 }
 
 __global__ void k_net_flatten(NetArgs a) {
@@ -172,9 +174,14 @@ __global__ void k_net_info(NetArgs a) {
     big = a.csize[i] > big ? a.csize[i] : big;
     perc += a.cflag[i] == 3u ? 1u : 0u;
   }
-  if (comps) atomicAdd(&a.summary[SUM_COMPONENTS], comps);
-  if (big) atomicMax(&a.summary[SUM_LARGEST], big);
-  if (perc) atomicAdd(&a.summary[SUM_PERCOLATING], perc);
+  comps = wave_sum_u64(comps);
+  big = wave_max_u64(big);
+  perc = wave_sum_u64(perc);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    if (comps) atomicAdd(&a.summary[SUM_COMPONENTS], comps);
+    if (big) atomicMax(&a.summary[SUM_LARGEST], big);
+    if (perc) atomicAdd(&a.summary[SUM_PERCOLATING], perc);
+  }
 }
 
 // Lower-triangle entries of row j gathered from the contact rows r < j,

@@ -96,15 +96,17 @@ __device__ __forceinline__ void flush(const PairArgs& a, const CellGeom& g, Acc&
     c2 = &a.ctr[3];  // unused for contacts
     c3 = &a.ctr[2];
   }
-  // every lane issues the atomics (lane 0 carries the value) so no branch
-  // precedes the next cross-lane operation of the caller
-  atomicAdd(c0, (unsigned long long)(l0 ? wt : 0u));
-  atomicAdd(c1, (unsigned long long)(l0 ? wr : 0u));
-  if (GEN) atomicAdd(c2, (unsigned long long)(l0 ? wk : 0u));
-  atomicMax(c3, (unsigned long long)(l0 ? __double_as_longlong(v) : 0ll));
+  // lane 0 alone issues the atomics: nothing in this branch is cross-lane,
+  // and the caller's next cross-lane operation follows the reconvergence
   unsigned long long* cb =
       GEN ? (mode == PM_GEN_OLD ? &g.state->tile_bytes : &g.state->tile_bytes_new) : &a.ctr[3];
-  atomicAdd(cb, l0 ? acc.bytes : 0ull);
+  if (l0) {
+    atomicAdd(c0, (unsigned long long)wt);
+    atomicAdd(c1, (unsigned long long)wr);
+    if (GEN) atomicAdd(c2, (unsigned long long)wk);
+    atomicMax(c3, (unsigned long long)__double_as_longlong(v));
+    atomicAdd(cb, acc.bytes);
+  }
   acc.tests = acc.rech = acc.kills = 0;
   acc.maxdd = 0.0;
   acc.bytes = 0;
@@ -583,17 +585,27 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
     g.cap = 0xFFFFFFFFu;
     g.state = nullptr;
   }
-  for (int k = 0; k < 8; ++k) {
-    const int q = (xcd + k) & 7;
-    const uint32_t qb = (uint32_t)(((uint64_t)W * (uint32_t)q) / 8u);
-    const uint32_t qe = (uint32_t)(((uint64_t)W * (uint32_t)(q + 1)) / 8u);
-    // a plain read first: an exhausted queue costs no atomic
-    if (uni(__hip_atomic_load(&a.qctr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= qe - qb)
-      continue;
+  // Lane l watches queue order l: the eight queues of this wave's XCD first
+  // (starting at its own sub-queue), then those of the other XCDs.  One
+  // vector load of the 64 counters finds the first queue with work left.
+  const int sub0 = (blockIdx.x >> 3) & 7;
+  const int my_q = (((xcd + (threadIdx.x >> 3)) & 7) << 3) | ((sub0 + (int)threadIdx.x) & 7);
+  const uint32_t my_qb = (uint32_t)(((uint64_t)W * (uint32_t)my_q) / QUEUES_PER_KERNEL);
+  const uint32_t my_qe = (uint32_t)(((uint64_t)W * (uint32_t)(my_q + 1)) / QUEUES_PER_KERNEL);
+  for (;;) {
+    const uint32_t seen =
+        __hip_atomic_load(&a.qctr[my_q * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long open = __ballot(seen < my_qe - my_qb);
+    if (open == 0ull) break;
+    const int pick = (int)__builtin_ctzll(open);  // uniform
+    const int q = __builtin_amdgcn_readlane(my_q, pick);
+    const uint32_t qb = uni((uint32_t)__builtin_amdgcn_readlane((int)my_qb, pick));
+    const uint32_t qe = uni((uint32_t)__builtin_amdgcn_readlane((int)my_qe, pick));
     for (;;) {
-      // all lanes take part in the atomic (lane 0 adds 1), lane 0's old
-      // value is the work item
-      const uint32_t old = atomicAdd(&a.qctr[q], l0 ? 1u : 0u);
+      // lane 0 takes the next item; readfirstlane makes it a scalar, so the
+      // loop exit below is uniform
+      uint32_t old = 0u;
+      if (l0) old = atomicAdd(&a.qctr[q * QSTRIDE], 1u);
       const uint32_t item = uni(old);
       if (item >= qe - qb) break;
       const uint32_t gc = qb + item;

@@ -276,9 +276,14 @@ __global__ void __launch_bounds__(WAVE) k_cent(CentArg a) {
   __syncthreads();
   for (int k = lane; k < a.bins; k += WAVE)
     if (hist[k]) atomicAdd(&a.out[k], (unsigned long long)hist[k]);
-  atomicAdd(&a.out[a.bins + 0], cnt);
-  atomicAdd(&a.out[a.bins + 1], sq);
-  atomicAdd(&a.out[a.bins + 2], s2q);
+  cnt = wave_sum_u64(cnt);
+  sq = wave_sum_u64(sq);
+  s2q = wave_sum_u64(s2q);
+  if (lane == 0) {
+    atomicAdd(&a.out[a.bins + 0], cnt);
+    atomicAdd(&a.out[a.bins + 1], sq);
+    atomicAdd(&a.out[a.bins + 2], s2q);
+  }
 }
 
 void launch_cent(const float4* P, const uint32_t* X, const uint32_t* start, const uint32_t* count,
@@ -332,7 +337,8 @@ __global__ void k_vdw_rows(const float4* __restrict__ P, const float4* __restric
     }
     rowsum[r] = s;
   }
-  if (local) atomicAdd(pairs, local);
+  local = wave_sum_u64(local);
+  if ((threadIdx.x & (WAVE - 1)) == 0 && local) atomicAdd(pairs, local);
 }
 
 __global__ void k_block_sums(const double* __restrict__ v, uint64_t n, double* __restrict__ bsum) {
