@@ -81,6 +81,7 @@ typedef struct nf_params {
   int32_t elec_axis;    /* network electrode axis 0/1/2 (x/y/z)            */
   double tunnel_lambda; /* tunnelling decay length (um), g = g0 exp(-gap/l)*/
   double tunnel_g0;     /* contact conductance prefactor                   */
+  int32_t write_state;  /* WRT_BIN: the CLI saves output.bin after generate */
 } nf_params;
 
 /* Generation statistics (reference prints these; cudaMain.cu:98-157). */
@@ -245,6 +246,34 @@ int nf_write_vtk(nf_handle* h, const char* path);
  * <prefix>sink.elec.coo, <prefix>source.elec.coo. */
 int nf_write_coo(nf_handle* h, const char* prefix);
 
+/* ---- state files ------------------------------------------------------- */
+
+/* Header of the reference's binary state file (output.bin, state_<vf>.bin;
+ * save_state_BIN / read_state_BIN, src/stateSR.cpp:54-134). */
+typedef struct nf_state_header {
+  float box[3];     /* physDim                                      */
+  float edge[3];    /* cell edges                                   */
+  int32_t ext[3];   /* cells per axis                               */
+  float len;        /* default length l                             */
+  float radius;     /* default radius a                             */
+  uint32_t slots;   /* ttlCNT: slots in the file                    */
+  uint32_t n_rods;  /* numCNT                                       */
+  float tol;        /* TOL: slots with l <= tol are empty           */
+} nf_state_header;
+
+/* Writes instance 0 as save_state_BIN after simPrms::repack
+ * (src/stateSR.cpp:54-88, src/simPrms.cpp:691-743): the generation cell
+ * grid, per-cell occupancy and slot prefix, and the nine field-major
+ * arrays x y z cx cy cz l a k (k = cell index), one slot per rod. */
+int nf_save_state(nf_handle* h, const char* path);
+/* Reads the header of a state file and checks the file size against it. */
+int nf_read_state_header(const char* path, nf_state_header* out);
+/* Replaces instance 0's rods with the state file's (read_state_BIN,
+ * src/stateSR.cpp:90-134): every slot with l > TOL, in slot order, as
+ * simPrms::asmbl_cnt_data (src/simPrms.cpp:329-391).  The file's box must
+ * equal the handle's; the rod count must not exceed the handle's target. */
+int nf_load_state(nf_handle* h, const char* path, uint32_t* n_rods);
+
 /* ---- library ----------------------------------------------------------- */
 
 /* Version string of the library build. */
@@ -253,7 +282,7 @@ const char* nf_version(void);
 /* Handle options (no counterpart in the reference, which has no runtime
  * switches beyond compile-time flags). */
 enum {
-  NF_OPT_GRAPH = 1,        /* 1: replay generation rounds from a HIP graph (default) */
+  NF_OPT_GRAPH = 1,        /* 1: replay generation rounds from a HIP graph (default 0) */
   NF_OPT_PROFILE = 2,      /* 1: time each kernel class with HIP events (no graph)   */
   NF_OPT_DIAG = 3,         /* 1: also compute FP64 for FP32-decided pairs and report
                               max |d32 - d64|                                         */

@@ -74,6 +74,7 @@ class Params(C.Structure):
         ("elec_axis", C.c_int32),
         ("tunnel_lambda", C.c_double),
         ("tunnel_g0", C.c_double),
+        ("write_state", C.c_int32),
     ]
 
     def copy(self) -> "Params":
@@ -144,6 +145,26 @@ class NetworkInfo(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class StateHeader(C.Structure):
+    """nf_state_header (include/nanofiller.h)."""
+
+    _fields_ = [
+        ("box", C.c_float * 3),
+        ("edge", C.c_float * 3),
+        ("ext", C.c_int32 * 3),
+        ("len", C.c_float),
+        ("radius", C.c_float),
+        ("slots", C.c_uint32),
+        ("n_rods", C.c_uint32),
+        ("tol", C.c_float),
+    ]
+
+    def to_dict(self) -> dict:
+        return {"box": tuple(self.box), "edge": tuple(self.edge), "ext": tuple(self.ext),
+                "len": self.len, "radius": self.radius, "slots": self.slots,
+                "n_rods": self.n_rods, "tol": self.tol}
+
+
 _lib_handle = None
 
 # (name, restype, argtypes) of every exported entry point
@@ -175,6 +196,9 @@ _SIGS = [
     ("nf_get_network", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_write_vtk", C.c_int, [C.c_void_p, C.c_char_p]),
     ("nf_write_coo", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("nf_save_state", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("nf_read_state_header", C.c_int, [C.c_char_p, C.POINTER(StateHeader)]),
+    ("nf_load_state", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint32)]),
     ("nf_version", C.c_char_p, []),
     ("nf_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
     ("nf_selftest", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint64]),
@@ -387,6 +411,23 @@ class Packing:
 
     def write_coo(self, prefix: str = "") -> None:
         _check(lib().nf_write_coo(self._h, prefix.encode()), self._h)
+
+    def save_state(self, path: str) -> None:
+        """Instance 0 as the reference's binary state file (output.bin)."""
+        _check(lib().nf_save_state(self._h, path.encode()), self._h)
+
+    def load_state(self, path: str) -> int:
+        """Replaces instance 0's rods with a state file's; returns the count."""
+        n = C.c_uint32()
+        _check(lib().nf_load_state(self._h, path.encode(), C.byref(n)), self._h)
+        return n.value
+
+
+def read_state_header(path: str) -> dict:
+    """Header of a binary state file (checked against the file size)."""
+    hd = StateHeader()
+    _check(lib().nf_read_state_header(path.encode(), C.byref(hd)), None)
+    return hd.to_dict()
 
 
 SELFTEST = {"sqrt": 0, "exp": 1, "log": 2, "div": 3, "u01": 4, "hw_sqrt": 5, "copy": 6}

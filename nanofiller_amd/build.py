@@ -33,6 +33,7 @@ SOURCES = [
     "nf_contacts.cpp",
     "nf_network.cpp",
     "nf_io.cpp",
+    "nf_state.cpp",
 ]
 HEADERS = ["nf_common.h", "nf_internal.h", "nf_host.h"]
 ARCH = os.environ.get("NF_OFFLOAD_ARCH", "gfx950")

@@ -18,10 +18,11 @@
 //                   / source.elec.coo / sink.elec.coo (collect_cunduct_data,
 //                   src/conductivity.cu:140-423; "internal" is accepted)
 //
-// The reference reads output.bin when generate is absent; state files are
-// not part of this library, so generate must come first.  Verbs outside
-// the hot path (plot, expand, reduce, stat_nematic, write_ort,
-// write_lmp, stat_exclsurf) are reported as unsupported.
+// As in the reference, generate saves output.bin when WRT_BIN is set
+// (save_state_BIN, src/stateSR.cpp:54-88), and without generate the rods
+// are read from output.bin (read_state_BIN, :90-134).  Verbs outside the
+// hot path (plot, expand, reduce, stat_nematic, write_ort, write_lmp,
+// stat_exclsurf) are reported as unsupported.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -79,32 +80,58 @@ int main(int argc, char** argv) {
   if (*nf_last_error(nullptr)) std::printf("note: %s\n", nf_last_error(nullptr));
   rc = nf_params_normalize(&p);
   if (rc != NF_OK) return die(nullptr, "parameters", rc);
-  if (std::strcmp(argv[2], "generate") != 0) {
-    std::fprintf(stderr, "nanofiller: state files (output.bin) are not supported; "
-                         "the first verb must be generate\n");
-    return 2;
+  const bool gen = std::strcmp(argv[2], "generate") == 0;
+  if (!gen) {
+    // the handle must hold the file's rods: size its target from the header
+    nf_state_header hd;
+    rc = nf_read_state_header("output.bin", &hd);
+    if (rc != NF_OK) return die(nullptr, "reading output.bin", rc);
+    if (hd.n_rods > p.n_rods) p.n_rods = hd.n_rods;
   }
   nf_handle* h = nullptr;
   rc = nf_create(&p, &h);
   if (rc != NF_OK) return die(nullptr, "device set-up", rc);
-  nf_gen_stats st;
-  rc = nf_generate(h, &st);
-  if (rc != NF_OK) {
-    const int r = die(h, "generate", rc);
-    nf_destroy(h);
-    return r;
-  }
   const double PI = 3.14159265358979323846;
   const double V = p.box[0] * p.box[1] * p.box[2];
-  std::vector<float> rods((size_t)st.n_accepted * 8);
   uint32_t n = 0;
-  nf_get_rods(h, 0, rods.data(), st.n_accepted, &n);
-  double vol = 0.0;
-  for (uint32_t i = 0; i < n; ++i) vol += PI * (double)rods[8 * i + 7] * rods[8 * i + 7] * rods[8 * i + 6];
-  std::printf("%u inclusions generated! \n", n);
-  std::printf("Volume fraction of %5.2f percent is achieved! \n", 100.0 * vol / V);
-  std::printf("rounds %u, pair tests %llu, rechecks %llu, %.2f ms\n", st.rounds,
-              (unsigned long long)st.pair_tests, (unsigned long long)st.rechecks, st.wall_ms);
+  nf_gen_stats st;
+  std::memset(&st, 0, sizeof(st));
+  if (gen) {
+    rc = nf_generate(h, &st);
+    if (rc != NF_OK) {
+      const int r = die(h, "generate", rc);
+      nf_destroy(h);
+      return r;
+    }
+    n = st.n_accepted;
+  } else {
+    rc = nf_load_state(h, "output.bin", &n);
+    if (rc != NF_OK) {
+      const int r = die(h, "reading output.bin", rc);
+      nf_destroy(h);
+      return r;
+    }
+    std::printf("Simulation state read from file output.bin \n%u inclusions found \n", n);
+  }
+  std::vector<float> rods((size_t)n * 8 + 8);
+  nf_get_rods(h, 0, rods.data(), n, &n);
+  if (gen) {
+    double vol = 0.0;
+    for (uint32_t i = 0; i < n; ++i) vol += PI * (double)rods[8 * i + 7] * rods[8 * i + 7] * rods[8 * i + 6];
+    std::printf("%u inclusions generated! \n", n);
+    std::printf("Volume fraction of %5.2f percent is achieved! \n", 100.0 * vol / V);
+    std::printf("rounds %u, pair tests %llu, rechecks %llu, %.2f ms\n", st.rounds,
+                (unsigned long long)st.pair_tests, (unsigned long long)st.rechecks, st.wall_ms);
+    if (p.write_state) {
+      rc = nf_save_state(h, "output.bin");
+      if (rc != NF_OK) {
+        const int r = die(h, "writing output.bin", rc);
+        nf_destroy(h);
+        return r;
+      }
+      std::printf("Simulation state dumped into file output.bin\n");
+    }
+  }
   bool have_contacts = false;
   uint64_t nc = 0;
   auto ensure_contacts = [&]() -> int {
@@ -114,7 +141,7 @@ int main(int argc, char** argv) {
     return r;
   };
   int status = 0;
-  for (int ai = 3; ai < argc && status == 0; ++ai) {
+  for (int ai = gen ? 3 : 2; ai < argc && status == 0; ++ai) {
     const char* v = argv[ai];
     if (!std::strcmp(v, "write_vtk")) {
       std::printf("CNTs created: %u \n", n);

@@ -293,6 +293,7 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   a.prStart = b.pstart;
   a.prCount = b.pcount;
   a.prAlive = nullptr;
+  a.prHl = b.phl;
   a.nbP = b.oP;
   a.nbD = b.oD;
   a.nbX = b.oid;

@@ -171,6 +171,7 @@ struct PairArgs {
   const uint32_t* prStart;
   const uint32_t* prCount;
   const uint32_t* prAlive;
+  const float* prHl;  // longest probe per cell (reject-vs-accepted pass run accepted-major)
   // neighbours
   const float4* nbP;
   const float4* nbD;

@@ -53,8 +53,9 @@ extern "C" int nf_params_load(const char* path, nf_params* p) {
                   std::string(path) + ":" + std::to_string(lineno) + ": " + key + ": " + what);
     };
     if (key == "WRT_BIN") {
-      int v;
-      is >> v;
+      int v = 0;
+      if (!(is >> v)) return bad("expected 0 or 1");
+      p->write_state = v != 0;
     } else if (key == "VOLUME_PERCENT") {
       double v;
       if (!(is >> v)) return bad("expected a number");

@@ -1,16 +1,22 @@
 // nf_pair.hip - the cell-pair engine.
 //
-// One 64-lane wavefront (one workgroup) per home cell, taken from an
-// XCD-aware work queue.  The home cell's probes sit in registers; each
-// stencil neighbour cell is streamed through LDS in 64-rod tiles.  When a
-// cell holds G < 64 probes, K = 64/next_pow2(G) lanes share a probe and
-// stride the tile, so a sparsely filled cell still keeps the wave busy.
-// Stencil offsets are sorted by the minimum cell-to-cell distance, so the
-// loop stops at the first offset beyond the chunk's reach; cells whose
-// longest rod cannot reach the probes are skipped.
+// One 64-lane wavefront (one workgroup) per home cell, taken from 64 work
+// queues (eight per XCD, each counter on its own cache line).  The home
+// cell's probes are taken 64 at a time, one per lane, and read by the
+// other lanes with readlane.  The stencil, sorted by minimum cell-to-cell
+// distance, is looked up 64 offsets at a time (one round trip), and the
+// neighbour cells that can reach the chunk are packed into a 128-slot
+// staging buffer (two slots per lane).  Every (probe, slot) pair passes a
+// conservative centre-distance prefilter; survivors queue in LDS, go
+// through the rigorous FP32 screen 64 at a time, and those the screen
+// cannot settle get the full FP32-band / FP64 test 64 at a time, so each
+// expensive step runs on dense work.
 //
 // Modes replace the reference's kernels:
 //   PM_GEN_OLD    new vs accepted rods   cuda_OldExtIsectLB (inc/intersection_kern.h:245-364)
+//                 (run accepted-major, with the roles of the two sets
+//                 swapped, once an instance's accepted rods outnumber the
+//                 round's proposals)
 //   PM_GEN_NEW    new vs new, priority   cuda_InternalIsect_noLck / cuda_NewExtIsectLB_parted
 //                                        (inc/intersection_kern.h:12-241)
 //   PM_CNT        contact records        core_cell_interact_kernel CONTACT_COUNT and
@@ -51,12 +57,29 @@ struct CellGeom {
   float sf32, sep32;
   uint32_t cap;
   InstState* state;
+  // home and neighbour rod sets of the pass; the reject-vs-accepted pass of
+  // an instance whose accepted rods outnumber its proposals runs
+  // accepted-major (inv): accepted rods are the probes, proposals the
+  // staged neighbours, and a cut clears the proposal's alive flag directly
+  const float4* hP;
+  const float4* hD;
+  const uint32_t* hX;
+  const uint32_t* hStart;
+  const uint32_t* hCount;
+  const float4* nP;
+  const float4* nD;
+  const uint32_t* nX;
+  const uint32_t* nStart;
+  const uint32_t* nCount;
+  const float* nHl;
+  bool inv;
 };
 
 // Per-lane statistics, flushed to the instance (or the contact counters)
 // by every lane together; lanes other than 0 add zero.
 struct Acc {
   uint32_t tests, rech, kills;
+  uint32_t kills_old;  // same-round pass: probes that arrive already cut by an accepted rod
   double maxdd;
   unsigned long long bytes;  // wave-uniform: probe and tile bytes streamed
 };
@@ -77,6 +100,7 @@ __device__ __forceinline__ float unif(float v) {
 template <bool GEN>
 __device__ __forceinline__ void flush(const PairArgs& a, const CellGeom& g, Acc& acc, int mode) {
   const uint32_t wt = wave_sum(acc.tests), wr = wave_sum(acc.rech), wk = wave_sum(acc.kills);
+  const uint32_t wo = wave_sum(acc.kills_old);
   double v = acc.maxdd;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
@@ -88,7 +112,7 @@ __device__ __forceinline__ void flush(const PairArgs& a, const CellGeom& g, Acc&
   if (GEN) {
     c0 = &g.state->pair_tests;
     c1 = &g.state->rechecks;
-    c2 = mode == PM_GEN_OLD ? &g.state->rej_old : &g.state->rej_new;
+    c2 = &g.state->rej_new;
     c3 = (unsigned long long*)&g.state->max_dd;
   } else {
     c0 = &a.ctr[0];
@@ -103,11 +127,16 @@ __device__ __forceinline__ void flush(const PairArgs& a, const CellGeom& g, Acc&
   if (l0) {
     atomicAdd(c0, (unsigned long long)wt);
     atomicAdd(c1, (unsigned long long)wr);
-    if (GEN) atomicAdd(c2, (unsigned long long)wk);
+    // rejections: both kinds are counted by the same-round pass, which
+    // reads every proposal's alive flag after the reject-vs-accepted pass
+    if (GEN && mode == PM_GEN_NEW) {
+      atomicAdd(c2, (unsigned long long)wk);
+      atomicAdd(&g.state->rej_old, (unsigned long long)wo);
+    }
     atomicMax(c3, (unsigned long long)__double_as_longlong(v));
     atomicAdd(cb, acc.bytes);
   }
-  acc.tests = acc.rech = acc.kills = 0;
+  acc.tests = acc.rech = acc.kills = acc.kills_old = 0;
   acc.maxdd = 0.0;
   acc.bytes = 0;
 }
@@ -120,7 +149,6 @@ constexpr int QCAP = 3 * WAVE;
 struct PairLds {
   float4 sP[WAVE], sD[WAVE];  // probes of the chunk
   uint32_t sX[WAVE];
-  uint32_t sA[WAVE];           // probe takes part (stage-1 survivor in the same-round pass)
   uint32_t hit[WAVE];          // per probe: overlaps something (generation)
   uint32_t dummy[WAVE];        // sink of branch-free stores
   float4 bP[SLOTS], bD[SLOTS];  // staged neighbour rods
@@ -194,7 +222,7 @@ __device__ __forceinline__ void full_test(const PairArgs& a, const CellGeom& g, 
   // contacts put the lower rod id first, with the periodic image applied to
   // the higher id.  The closest-point solve is not bitwise symmetric under
   // swapping the segments.
-  const bool swap = !GEN && (X1 > xj);
+  const bool swap = GEN ? g.inv : (X1 > xj);
   const float4 Pa = swap ? P2 : P1, Da = swap ? D2 : D1;
   const float4 Pb = swap ? P1 : P2, Db = swap ? D1 : D2;
   const double ux = swap ? -sx : sx, uy = swap ? -sy : sy, uz = swap ? -sz : sz;
@@ -233,7 +261,10 @@ __device__ __forceinline__ void full_test(const PairArgs& a, const CellGeom& g, 
   }
   if (!cut) return;
   if (GEN) {
-    L.hit[p] = 1u;
+    if (g.inv)
+      a.alive_out[xj] = 0u;  // accepted-major: the slot is the proposal (xj its index)
+    else
+      L.hit[p] = 1u;
   } else {
     if (d2_64 < 0.0) {
       bool w2;
@@ -314,10 +345,10 @@ __device__ void refill_cells(const PairArgs& a, const CellGeom& g, PairLds& L, S
       m[d] = mm;
     }
     const uint32_t nc = g.cell_base + (uint32_t)(m[0] + g.nc[0] * (m[1] + g.nc[1] * m[2]));
-    uint32_t n = a.nbCount[nc];
-    if (MODE == PM_GEN_OLD) n = min(n, g.cap);
-    const float hl = a.nbHl[nc];
-    const uint32_t st0 = a.nbStart[nc];
+    uint32_t n = g.nCount[nc];
+    if (MODE == PM_GEN_OLD && !g.inv) n = min(n, g.cap);  // accepted cells hold at most cap rods
+    const float hl = g.nHl[nc];
+    const uint32_t st0 = g.nStart[nc];
     const float rr_c = (chunk_hl + hl) * 1.000001f + g.R_max + CELL_MARGIN;
     const bool beyond = valid && md2 > rr_chunk2;
     const bool rel = valid && !beyond && n > 0u && md2 <= rr_c * rr_c;
@@ -420,9 +451,10 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
                              Acc& acc) {
   constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
   const int lane = threadIdx.x;
-  const uint32_t pc = uni(a.prCount[gc]);
+  uint32_t pc = uni(g.hCount[gc]);
+  if (MODE == PM_GEN_OLD && g.inv) pc = min(pc, g.cap);  // accepted cells hold at most cap rods
   if (pc == 0) return;
-  const uint32_t ps = uni(a.prStart[gc]);
+  const uint32_t ps = uni(g.hStart[gc]);
   const uint32_t lc = gc - g.cell_base;
   const int c0 = (int)(lc % (uint32_t)g.nc[0]);
   const int c1 = (int)((lc / (uint32_t)g.nc[0]) % (uint32_t)g.nc[1]);
@@ -434,16 +466,23 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
     // lane l holds probe l of the chunk (clamped loads, every lane loads)
     const bool inrange = (uint32_t)lane < G;
     const uint32_t pidx = ps + pb + min((uint32_t)lane, G - 1u);
-    const float4 P1 = a.prP[pidx];
-    const float4 D1 = a.prD[pidx];
-    const uint32_t X1 = a.prX[pidx];
+    const float4 P1 = g.hP[pidx];
+    const float4 D1 = g.hD[pidx];
+    const uint32_t X1 = g.hX[pidx];
     bool have = inrange;
-    if (MODE == PM_GEN_NEW) have = have && (a.prAlive[pidx] != 0u);
+    if (MODE == PM_GEN_NEW) {
+      const bool alive = a.prAlive[pidx] != 0u;
+      acc.kills_old += (inrange && !alive) ? 1u : 0u;
+      have = have && alive;
+    }
     L.sP[lane] = P1;
     L.sD[lane] = D1;
     L.sX[lane] = X1;
-    L.sA[lane] = have ? 1u : 0u;
     L.hit[lane] = 0u;
+    // probes taking part, as a wave-uniform mask; the prefilter reads each
+    // probe from its lane's registers
+    const unsigned long long pmask = __ballot(have);
+    if (pmask == 0ull) continue;  // every probe of the chunk already cut: nothing to stage
     const float chunk_hl = unif(wave_max(have ? P1.w : 0.0f));
     acc.bytes += (unsigned long long)G * 36ull;
     const float rr_chunk = (chunk_hl + g.hl_max) * 1.000001f + g.R_max + CELL_MARGIN;
@@ -482,9 +521,15 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
       // ---- load: both slots' loads in flight together; every lane loads
       // (unfilled slots read index 0 of a non-empty array and are never used)
       const bool v0 = (uint32_t)lane < fill, v1 = (uint32_t)lane + WAVE < fill;
-      const float4 p0 = a.nbP[src0], p1 = a.nbP[src1];
-      const float4 d0 = a.nbD[src0], d1 = a.nbD[src1];
-      uint32_t x0 = a.nbX[src0], x1 = a.nbX[src1];
+      const float4 p0 = g.nP[src0], p1 = g.nP[src1];
+      const float4 d0 = g.nD[src0], d1 = g.nD[src1];
+      // accepted-major pass: the slot keeps the proposal's index, which a
+      // cut clears in the alive flags
+      uint32_t x0 = src0, x1 = src1;
+      if (!(MODE == PM_GEN_OLD && g.inv)) {
+        x0 = g.nX[src0];
+        x1 = g.nX[src1];
+      }
       if (MODE == PM_GEN_NEW) {
         x0 = (a.nbAlive[src0] != 0u) ? x0 : 0xFFFFFFFFu;
         x1 = (a.nbAlive[src1] != 0u) ? x1 : 0xFFFFFFFFu;
@@ -513,10 +558,14 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
       __syncthreads();
       // ---- prefilter every (probe, slot) pair, queue the survivors
       const bool two = fill > (uint32_t)WAVE;
-      for (uint32_t pp = 0; pp < G; ++pp) {
-        if (uni(L.sA[pp]) == 0u) continue;  // scalar branch: the flag is the same for every lane
-        const float4 Pp = L.sP[pp];
-        const uint32_t Xp = L.sX[pp];
+      for (unsigned long long pm = pmask; pm != 0ull; pm &= pm - 1ull) {
+        const int pl = (int)__builtin_ctzll(pm);
+        const uint32_t pp = (uint32_t)pl;
+        const float4 Pp = make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.x), pl)),
+                                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.y), pl)),
+                                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.z), pl)),
+                                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.w), pl)));
+        const uint32_t Xp = (uint32_t)__builtin_amdgcn_readlane((int)X1, pl);
         const float rr = Pp.w * 1.000001f + pad_r + bmax;
         const float rr2 = rr * rr * 1.00001f;
         const uint32_t qprobe = ps + pb + pp;
@@ -537,15 +586,15 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
       }
       __syncthreads();
     }
-    if (GEN) {
+    if (GEN && !(MODE == PM_GEN_OLD && g.inv)) {
       const bool hitp = have && L.hit[lane] != 0u;
-      acc.kills += hitp ? 1u : 0u;
       // branch-free result store: lanes without a result write a private
       // LDS slot instead
       if (MODE == PM_GEN_OLD) {
         uint32_t* dst = hitp ? &a.alive_out[pidx] : &L.dummy[lane];
         *dst = 0u;
       } else {
+        acc.kills += hitp ? 1u : 0u;
         uint32_t* dst = (have && !hitp) ? &a.surv[X1] : &L.dummy[lane];
         *dst = 1u;
       }
@@ -564,7 +613,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
   if (GEN && uni(a.ri->T) == 0u) return;
   const bool l0 = threadIdx.x == 0;
   CellGeom g;
-  Acc acc = {0u, 0u, 0u, 0.0, 0ull};
+  Acc acc = {0u, 0u, 0u, 0u, 0.0, 0ull};
   uint32_t cur_inst = 0xFFFFFFFFu;
   if (!GEN) {
     for (int d = 0; d < 3; ++d) {
@@ -584,7 +633,21 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
     g.sep32 = 0.0f;
     g.cap = 0xFFFFFFFFu;
     g.state = nullptr;
+    g.inv = false;
   }
+  // role pointers of the usual orientation: probes from pr*, staged
+  // neighbours from nb*
+  g.hP = a.prP;
+  g.hD = a.prD;
+  g.hX = a.prX;
+  g.hStart = a.prStart;
+  g.hCount = a.prCount;
+  g.nP = a.nbP;
+  g.nD = a.nbD;
+  g.nX = a.nbX;
+  g.nStart = a.nbStart;
+  g.nCount = a.nbCount;
+  g.nHl = a.nbHl;
   // Lane l watches queue order l: the eight queues of this wave's XCD first
   // (starting at its own sub-queue), then those of the other XCDs.  One
   // vector load of the 64 counters finds the first queue with work left.
@@ -632,6 +695,27 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
           g.cap = d.cap;
           g.state = &a.st[inst];
           cur_inst = inst;
+          if (MODE == PM_GEN_OLD) {
+            // accepted-major when the instance's accepted rods outnumber
+            // this round's proposals: fewer, denser probe chunks stage the
+            // sparser set (the pair set and every decision are the same)
+            const uint32_t nacc = uni(a.st[inst].n_acc);
+            const uint32_t nprop = uni(a.prStart[d.cell_base + d.ncells]) - uni(a.prStart[d.cell_base]);
+            g.inv = nacc > nprop;
+            g.hP = g.inv ? a.nbP : a.prP;
+            g.hD = g.inv ? a.nbD : a.prD;
+            g.hX = g.inv ? a.nbX : a.prX;
+            g.hStart = g.inv ? a.nbStart : a.prStart;
+            g.hCount = g.inv ? a.nbCount : a.prCount;
+            g.nP = g.inv ? a.prP : a.nbP;
+            g.nD = g.inv ? a.prD : a.nbD;
+            g.nX = g.inv ? a.prX : a.nbX;
+            g.nStart = g.inv ? a.prStart : a.nbStart;
+            g.nCount = g.inv ? a.prCount : a.nbCount;
+            g.nHl = g.inv ? a.prHl : a.nbHl;
+          } else {
+            g.inv = false;
+          }
         }
       }
       process_cell<MODE, PREC>(a, g, gc, L, acc);

@@ -18,11 +18,12 @@ def _run(tmp_path, *verbs):
                           text=True, timeout=300)
 
 
-def test_cli_usage_and_state_file_refusal(tmp_path):
+def test_cli_usage_and_missing_state_file(tmp_path):
     r = subprocess.run([CLI], capture_output=True, text=True)
     assert r.returncode == 2 and "usage" in r.stderr
+    # without generate the rods come from output.bin, as in the reference
     r = _run(tmp_path, "write_vtk")
-    assert r.returncode == 2 and "generate" in r.stderr
+    assert r.returncode == 1 and "output.bin" in r.stderr
 
 
 def test_cli_reports_missing_device(tmp_path):
