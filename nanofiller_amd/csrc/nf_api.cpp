@@ -319,8 +319,12 @@ static int plan_instance(const nf_params& p, InstDesc& d, std::vector<double>& a
   }
   if (ncells > (1ull << 26)) return NF_E_INVALID;
   d.ncells = (uint32_t)ncells;
+  // full stencil, then the half stencil of the same-round pass
   stencil = build_stencil(d.ncell, p.box, reach, false);
   d.st_n = (uint32_t)stencil.size();
+  const std::vector<int4> half = build_stencil(d.ncell, p.box, reach, true);
+  d.hst_n = (uint32_t)half.size();
+  stencil.insert(stencil.end(), half.begin(), half.end());
   // per-cell capacity for accepted rods
   const double cellV = V / (double)ncells;
   const double peak = p.den_law == NF_DEN_AGGLOM ? p.agg_peak : 1.0;
@@ -401,6 +405,7 @@ extern "C" int nf_create_batch(const nf_params* ps, int n, nf_handle** out) {
     d.old_off = old_off;
     d.rod_off = (uint32_t)rod_off;
     d.st_off = (uint32_t)st_total;
+    d.hst_off = (uint32_t)st_total + d.st_n;
     cell_base += d.ncells;
     old_off += (uint64_t)d.ncells * d.cap;
     rod_off += d.n_target;

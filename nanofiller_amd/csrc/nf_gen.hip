@@ -142,7 +142,8 @@ __global__ void __launch_bounds__(GEN_THREADS) k_scatter(const RoundInfo* __rest
                                                          float4* __restrict__ pP,
                                                          float4* __restrict__ pD,
                                                          uint32_t* __restrict__ pslot,
-                                                         uint32_t* __restrict__ palive) {
+                                                         uint32_t* __restrict__ palive,
+                                                         uint32_t* __restrict__ surv) {
   const uint32_t T = ri->T;
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < T; g += gridDim.x * blockDim.x) {
     const int32_t c = scell[g];
@@ -152,6 +153,7 @@ __global__ void __launch_bounds__(GEN_THREADS) k_scatter(const RoundInfo* __rest
     pD[dst] = sD[g];
     pslot[dst] = g;
     palive[dst] = 1u;
+    surv[g] = 1u;  // cleared by the same-round pass for the proposals it rejects
   }
 }
 
@@ -275,7 +277,7 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   if (ev) (void)hipEventRecord(ev[2], s);
   scan_u32(b.pcount, b.pstart, b.total_cells, b.scan_tmp, s);
   hipLaunchKernelGGL(k_scatter, dim3(gen_grid(b.T_cap)), dim3(GEN_THREADS), 0, s, b.ri, b.sP,
-                     b.sD, b.scell, b.srk, b.pstart, b.pP, b.pD, b.pslot, b.palive);
+                     b.sD, b.scell, b.srk, b.pstart, b.pP, b.pD, b.pslot, b.palive, b.surv);
   if (ev) (void)hipEventRecord(ev[3], s);
 
   PairArgs a{};

@@ -28,7 +28,8 @@ struct InstDesc {
   uint32_t rod_off;      // offset of this instance in the by-id rod arrays
   uint64_t old_off;      // offset of this instance in the cell-slot arrays
   uint32_t n_target, min_batch, max_batch, max_rounds, stall_rounds;
-  uint32_t st_off, st_n;  // generation stencil (int4 entries)
+  uint32_t st_off, st_n;    // generation stencil (int4 entries), full
+  uint32_t hst_off, hst_n;  // half stencil of the same-round pass (each cell pair once)
   double sf64, sep64;     // hard core = sf*(a1+a2)+sep
   float sf32, sep32;
   float ew;               // FP32 bound on a centre-difference component error
@@ -217,7 +218,12 @@ void launch_ctl_init(GenBuffers& b, hipStream_t s);
 
 // ---- launchers (nf_pair.hip) ----
 void launch_pair(int mode, int prec, const PairArgs& a, hipStream_t s);
-int pair_grid();
+// Persistent grid of a pair pass (waves): 16 per CU for the
+// reject-vs-accepted and contact passes, 6 per CU for the same-round pass,
+// whose light, sparse cells run faster with fewer waves competing for the
+// memory system (measured on MI355X: 130 us per launch at 6 per CU against
+// 210 us at 16 on the 500k-rod workload).
+int pair_grid(int mode);
 
 // ---- launchers (nf_post.hip) ----
 void launch_cell_count(const float4* P, uint32_t n, const int32_t ncell[3], const double scale[3],

@@ -19,6 +19,8 @@
 //                 round's proposals)
 //   PM_GEN_NEW    new vs new, priority   cuda_InternalIsect_noLck / cuda_NewExtIsectLB_parted
 //                                        (inc/intersection_kern.h:12-241)
+//                 (half stencil: each pair of surviving proposals once; a
+//                 cut clears the later slot's survivor flag)
 //   PM_CNT        contact records        core_cell_interact_kernel CONTACT_COUNT and
 //                                        CONTACT_RECORD in one pass (appended, unsorted)
 //                                        (src/core_cell_interaction_kernel.cu:70-315)
@@ -49,6 +51,7 @@ struct CellGeom {
   double box[3];
   float boxf[3];
   int nc[3];
+  float rnc[3];  // 1 / nc (rounded): periodic wrap without integer division
   uint32_t cell_base;
   const int4* st;
   uint32_t st_n;
@@ -169,9 +172,12 @@ __device__ __forceinline__ int img_y(uint32_t meta) { return (int)(int8_t)((meta
 __device__ __forceinline__ int img_z(uint32_t meta) { return (int)(int8_t)((meta >> 16) & 0xFF); }
 
 // FP32 image shift of a neighbour: the FP64 shift img * box rounded once.
+// Box edges are float-exact (nf_params_normalize) and |img| < 128, so the
+// product is exact in FP64 and the FP32 product rounds the same exact value
+// once: (float)img * boxf is that rounding, without FP64 arithmetic.
 __device__ __forceinline__ float3 shift_f32(const CellGeom& g, uint32_t meta) {
-  return make_float3((float)((double)img_x(meta) * g.box[0]), (float)((double)img_y(meta) * g.box[1]),
-                     (float)((double)img_z(meta) * g.box[2]));
+  return make_float3((float)img_x(meta) * g.boxf[0], (float)img_y(meta) * g.boxf[1],
+                     (float)img_z(meta) * g.boxf[2]);
 }
 
 // Screen of one queued pair (valid lanes only): the reach band and the
@@ -222,7 +228,9 @@ __device__ __forceinline__ void full_test(const PairArgs& a, const CellGeom& g, 
   // contacts put the lower rod id first, with the periodic image applied to
   // the higher id.  The closest-point solve is not bitwise symmetric under
   // swapping the segments.
-  const bool swap = GEN ? g.inv : (X1 > xj);
+  // generation: the later slot first (same-round pass), the proposal first
+  // (reject-vs-accepted, swapped when run accepted-major)
+  const bool swap = MODE == PM_GEN_NEW ? (X1 < xj) : (GEN ? g.inv : (X1 > xj));
   const float4 Pa = swap ? P2 : P1, Da = swap ? D2 : D1;
   const float4 Pb = swap ? P1 : P2, Db = swap ? D1 : D2;
   const double ux = swap ? -sx : sx, uy = swap ? -sy : sy, uz = swap ? -sz : sz;
@@ -260,7 +268,11 @@ __device__ __forceinline__ void full_test(const PairArgs& a, const CellGeom& g, 
     }
   }
   if (!cut) return;
-  if (GEN) {
+  if (MODE == PM_GEN_NEW) {
+    // the later slot loses; several earlier ones may cut it, count it once
+    const uint32_t loser = X1 > xj ? X1 : xj;
+    if (atomicExch(&a.surv[loser], 0u) != 0u) ++acc.kills;
+  } else if (GEN) {
     if (g.inv)
       a.alive_out[xj] = 0u;  // accepted-major: the slot is the proposal (xj its index)
     else
@@ -337,12 +349,15 @@ __device__ void refill_cells(const PairArgs& a, const CellGeom& g, PairLds& L, S
     const float md2 = __int_as_float(so.w);
     const int raw[3] = {c0 + so.x, c1 + so.y, c2 + so.z};
     int img[3], m[3];
+    // image = floor(raw / nc): the float quotient of raw + 1/2 is at least
+    // 1/(2 nc) from an integer and its rounding error is far below that for
+    // |raw| < 2^21, so the floor is exact (checked against integer division
+    // for every nc <= 1024)
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      int mm = raw[d] % g.nc[d];
-      if (mm < 0) mm += g.nc[d];
-      img[d] = (raw[d] - mm) / g.nc[d];
-      m[d] = mm;
+      const int q = (int)floorf(((float)raw[d] + 0.5f) * g.rnc[d]);
+      img[d] = q;
+      m[d] = raw[d] - q * g.nc[d];
     }
     const uint32_t nc = g.cell_base + (uint32_t)(m[0] + g.nc[0] * (m[1] + g.nc[1] * m[2]));
     uint32_t n = g.nCount[nc];
@@ -403,29 +418,62 @@ __device__ __forceinline__ bool prefilter(const float4 P1, float rr2, uint32_t X
   const float wx = P1.x - Q2.x, wy = P1.y - Q2.y, wz = P1.z - Q2.z;
   const float w2 = fmaf(wz, wz, fmaf(wy, wy, wx * wx));
   bool f = w2 <= rr2;
-  if (MODE == PM_GEN_NEW) f = f && xj < X1;
-  // contacts: a slot of the home cell only after the probe's index; another
-  // cell (an image may map back to the home cell) never the probe itself
-  if (CNT) f = f && (qprobe < qa || qprobe > qb);
+  // same-round pass: the slot must have survived the reject-vs-accepted
+  // pass (dead slots carry id 0xFFFFFFFF)
+  if (MODE == PM_GEN_NEW) f = f && xj != 0xFFFFFFFFu;
+  // half-stencil passes (contacts, same-round): a slot of the home cell only
+  // after the probe's index; another cell (an image may map back to the home
+  // cell) never the probe itself
+  if (CNT || MODE == PM_GEN_NEW) f = f && (qprobe < qa || qprobe > qb);
   return f;
 }
 
 // Appends the prefilter results of every lane for one probe (both slot
 // halves) to the first queue, screening 64 at a time when enough wait.
-template <int MODE, int PREC>
+template <int MODE, int PREC, bool TWO>
 __device__ __forceinline__ void push_pairs(const PairArgs& a, const CellGeom& g, PairLds& L,
                                            bool f0, bool f1, uint32_t pp, uint32_t& qn,
                                            uint32_t& q2n, Acc& acc) {
   const int lane = threadIdx.x;
-  const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
+  const unsigned long long m0 = __ballot(f0);
   const uint32_t n0 = (uint32_t)__popcll(m0);
   if (f0) L.qE[qn + lane_prefix(m0)] = pp | ((uint32_t)lane << 8);
-  if (f1) L.qE[qn + n0 + lane_prefix(m1)] = pp | ((uint32_t)(lane + WAVE) << 8);
-  qn += n0 + (uint32_t)__popcll(m1);
+  qn += n0;
+  if (TWO) {
+    const unsigned long long m1 = __ballot(f1);
+    if (f1) L.qE[qn + lane_prefix(m1)] = pp | ((uint32_t)(lane + WAVE) << 8);
+    qn += (uint32_t)__popcll(m1);
+  }
   while (qn >= (uint32_t)WAVE) {
     __syncthreads();
     screen_batch<MODE, PREC>(a, g, L, qn - WAVE, WAVE, q2n, acc);
     qn -= WAVE;
+  }
+}
+
+// Prefilter of every (probe, slot) pair of one staged batch, looping over
+// the probes of the mask (read from their lanes' registers); TWO: the batch
+// fills the second slot of the lanes too.  rr2_l is the lane's own probe's
+// squared reach bound for this batch.
+template <int MODE, int PREC, bool TWO>
+__device__ __forceinline__ void probe_loop(const PairArgs& a, const CellGeom& g, PairLds& L,
+                                           unsigned long long pmask, const float4 P1, uint32_t X1,
+                                           float rr2_l, uint32_t qbase, const float4 Q0, uint32_t x0,
+                                           uint32_t qa0, uint32_t qb0, const float4 Q1, uint32_t x1,
+                                           uint32_t qa1, uint32_t qb1, uint32_t& qn, uint32_t& q2n,
+                                           Acc& acc) {
+  for (unsigned long long pm = pmask; pm != 0ull; pm &= pm - 1ull) {
+    const int pl = (int)__builtin_ctzll(pm);
+    const float4 Pp = make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.x), pl)),
+                                  __int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.y), pl)),
+                                  __int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.z), pl)),
+                                  0.0f);
+    const uint32_t Xp = (uint32_t)__builtin_amdgcn_readlane((int)X1, pl);
+    const float rr2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rr2_l), pl));
+    const uint32_t qprobe = qbase + (uint32_t)pl;
+    const bool f0 = prefilter<MODE>(Pp, rr2, Xp, qprobe, Q0, x0, qa0, qb0);
+    const bool f1 = TWO && prefilter<MODE>(Pp, rr2, Xp, qprobe, Q1, x1, qa1, qb1);
+    push_pairs<MODE, PREC, TWO>(a, g, L, f0, f1, (uint32_t)pl, qn, q2n, acc);
   }
 }
 
@@ -472,7 +520,10 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
     bool have = inrange;
     if (MODE == PM_GEN_NEW) {
       const bool alive = a.prAlive[pidx] != 0u;
-      acc.kills_old += (inrange && !alive) ? 1u : 0u;
+      const bool doa = inrange && !alive;  // already cut by an accepted rod
+      acc.kills_old += doa ? 1u : 0u;
+      uint32_t* dst = doa ? &a.surv[X1] : &L.dummy[lane];
+      *dst = 0u;
       have = have && alive;
     }
     L.sP[lane] = P1;
@@ -557,22 +608,15 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
       const uint32_t qb1 = (m1 & 0x80000000u) ? 0xFFFFFFFFu : src1;
       __syncthreads();
       // ---- prefilter every (probe, slot) pair, queue the survivors
-      const bool two = fill > (uint32_t)WAVE;
-      for (unsigned long long pm = pmask; pm != 0ull; pm &= pm - 1ull) {
-        const int pl = (int)__builtin_ctzll(pm);
-        const uint32_t pp = (uint32_t)pl;
-        const float4 Pp = make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.x), pl)),
-                                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.y), pl)),
-                                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.z), pl)),
-                                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.w), pl)));
-        const uint32_t Xp = (uint32_t)__builtin_amdgcn_readlane((int)X1, pl);
-        const float rr = Pp.w * 1.000001f + pad_r + bmax;
-        const float rr2 = rr * rr * 1.00001f;
-        const uint32_t qprobe = ps + pb + pp;
-        const bool f0 = prefilter<MODE>(Pp, rr2, Xp, qprobe, Q0, x0, qa0, qb0);
-        const bool f1 = prefilter<MODE>(Pp, rr2, Xp, qprobe, Q1, x1, qa1, qb1) && two;
-        push_pairs<MODE, PREC>(a, g, L, f0, f1, pp, qn, q2n, acc);
-      }
+      // each lane's own probe reach bound for this batch
+      const float rr_l = P1.w * 1.000001f + pad_r + bmax;
+      const float rr2_l = rr_l * rr_l * 1.00001f;
+      if (fill > (uint32_t)WAVE)
+        probe_loop<MODE, PREC, true>(a, g, L, pmask, P1, X1, rr2_l, ps + pb, Q0, x0, qa0, qb0, Q1,
+                                     x1, qa1, qb1, qn, q2n, acc);
+      else
+        probe_loop<MODE, PREC, false>(a, g, L, pmask, P1, X1, rr2_l, ps + pb, Q0, x0, qa0, qb0, Q1,
+                                      x1, qa1, qb1, qn, q2n, acc);
       // ---- drain both queues before the slots are restaged
       if (qn > 0) {
         __syncthreads();
@@ -586,18 +630,12 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
       }
       __syncthreads();
     }
-    if (GEN && !(MODE == PM_GEN_OLD && g.inv)) {
-      const bool hitp = have && L.hit[lane] != 0u;
+    if (MODE == PM_GEN_OLD && !g.inv) {
       // branch-free result store: lanes without a result write a private
       // LDS slot instead
-      if (MODE == PM_GEN_OLD) {
-        uint32_t* dst = hitp ? &a.alive_out[pidx] : &L.dummy[lane];
-        *dst = 0u;
-      } else {
-        acc.kills += hitp ? 1u : 0u;
-        uint32_t* dst = (have && !hitp) ? &a.surv[X1] : &L.dummy[lane];
-        *dst = 1u;
-      }
+      const bool hitp = have && L.hit[lane] != 0u;
+      uint32_t* dst = hitp ? &a.alive_out[pidx] : &L.dummy[lane];
+      *dst = 0u;
     }
     __syncthreads();
   }
@@ -620,6 +658,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
       g.box[d] = a.box[d];
       g.boxf[d] = a.boxf[d];
       g.nc[d] = a.ncell[d];
+      g.rnc[d] = 1.0f / (float)a.ncell[d];
     }
     g.cell_base = 0;
     g.st = a.stencil + a.st_off;
@@ -681,10 +720,11 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
             g.box[dd] = d.smp.box[dd];
             g.boxf[dd] = d.boxf[dd];
             g.nc[dd] = d.ncell[dd];
+            g.rnc[dd] = 1.0f / (float)d.ncell[dd];
           }
           g.cell_base = d.cell_base;
-          g.st = a.stencil + d.st_off;
-          g.st_n = d.st_n;
+          g.st = a.stencil + (MODE == PM_GEN_NEW ? d.hst_off : d.st_off);
+          g.st_n = MODE == PM_GEN_NEW ? d.hst_n : d.st_n;
           g.ew = d.ew;
           g.hl_max = d.hl_max;
           g.R_max = d.R_max;
@@ -724,22 +764,22 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
   if (!GEN || cur_inst != 0xFFFFFFFFu) flush<GEN>(a, g, acc, MODE);
 }
 
-static int g_pair_grid = 0;
+static int g_cus = 0;
 
-int pair_grid() {
-  if (g_pair_grid == 0) {
+int pair_grid(int mode) {
+  if (g_cus == 0) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus <= 0)
       cus = 256;
-    g_pair_grid = cus * 16;
+    g_cus = cus;
   }
-  return g_pair_grid;
+  return g_cus * (mode == PM_GEN_NEW ? 6 : 16);
 }
 
 void launch_pair(int mode, int prec, const PairArgs& a, hipStream_t s) {
-  const dim3 grid(pair_grid()), block(PAIR_THREADS);
+  const dim3 grid(pair_grid(mode)), block(PAIR_THREADS);
 #define NF_LAUNCH(M, P) hipLaunchKernelGGL((k_pair<M, P>), grid, block, 0, s, a)
   if (prec == NF_PREC_FP64) {
     switch (mode) {
