@@ -22,7 +22,7 @@ The same JSON line carries:
     FP64 Kirchhoff matrix;
   * cpu_baseline: the C oracle on a bounded sample of the headline workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs main,batch,contacts,network]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs main,batch,contacts,network,conduct]
 """
 from __future__ import annotations
 
@@ -251,6 +251,42 @@ def leg_network(a, out: dict) -> None:
     }
 
 
+def leg_conduct(a, out: dict) -> None:
+    """DC conductance of a percolating network: generation, contacts,
+    network, then the Jacobi-PCG solve between the electrode planes."""
+    p = nf.normalize(nf.default_params(box=(20, 20, 20), n_rods=1000000, len=1.5, radius=0.005,
+                                       seed=11, contact_gap=0.005, elec_axis=0, tunnel_lambda=5e-4,
+                                       precision=nf.NF_PREC_FP32_RECHECK))
+    log("conduct: 1M rods, generate, contacts, network")
+    with nf.Packing(p) as pk:
+        pk.generate()
+        c = pk.contacts(p.contact_gap)
+        info, net = pk.network()
+        pk.conductance(tol=1e-10, max_iter=20000)  # warm-up
+        t0 = time.perf_counter()
+        ci, _ = pk.conductance(tol=1e-10, max_iter=20000)
+        t_s = time.perf_counter() - t0
+    m = ci["unknowns"]
+    # reduced-system entries: a diagonal per unknown plus two per rod-rod edge
+    # inside the current-carrying components (bounded by all edges)
+    nnz = m + 2 * info["n_edges"]
+    # algorithmic bytes per iteration: CSR (12 B per entry) + p gathers, and
+    # the vector passes (spmv: p, q; update: dinv, p, q, x rw, r rw, z; dir: z, p rw)
+    per_it = 12 * nnz + 8 * nnz + 8 * m * (2 + 9 + 3)
+    ms_it = ci["solve_ms"] / max(ci["iterations"], 1)
+    log(f"conduct: {info['percolating']} percolating of {info['n_components']} components, {m} unknowns, "
+        f"{ci['iterations']} CG iterations, {ci['solve_ms']:.1f} ms ({ms_it * 1e3:.1f} us/iteration), "
+        f"G = {ci['conductance']:.6g}, residual {ci['residual']:.2e}")
+    out["extra"]["conduct"] = {
+        "workload": "1,000,000 rods, 20 um cube, l 1.5 um, d 10 nm, isotropic, gap <= 5 nm, "
+                    "electrodes x = 0 / 20 um, g = exp(-gap / 0.5 nm); Jacobi-PCG to 1e-10",
+        "contacts": int(c.n), "network": info, "solve": ci, "call_ms": t_s * 1e3,
+        "ms_per_iteration": ms_it, "bytes_per_iteration_upper": per_it,
+        "gb_per_s_upper": per_it / (ms_it * 1e-3) / 1e9 if ms_it > 0 else None,
+        "current_mismatch": abs(ci["conductance"] - ci["current_sink"]) / max(abs(ci["conductance"]), 1e-300),
+    }
+
+
 def cpu_baseline(threads: int):
     """Oracle (FP64 C restatement) on a bounded sample: the same laws at 1/8
     of the volume and the same rod density (62,500 rods in a 10 um cube)."""
@@ -271,7 +307,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--legs", default="main,batch,contacts,network")
+    ap.add_argument("--legs", default="main,batch,contacts,network,conduct")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison leg")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -290,7 +326,8 @@ def main() -> int:
     res = {"extra": {}}
     if "main" in legs:
         leg_main(a, res)
-    for name, fn in (("batch", leg_batch), ("contacts", leg_contacts), ("network", leg_network)):
+    for name, fn in (("batch", leg_batch), ("contacts", leg_contacts), ("network", leg_network),
+                     ("conduct", leg_conduct)):
         if name in legs:
             fn(a, res)
     cpu = None

@@ -238,6 +238,30 @@ int nf_network(nf_handle* h, nf_network_info* info);
 int nf_get_network(nf_handle* h, uint32_t* labels, uint32_t* row_ptr,
                    uint32_t* col, double* val, uint64_t cap);
 
+/* DC solve of the network between the electrodes (source at potential 1,
+ * sink at 0): replaces tools/kirchhoff/kirchhoff.py:256-272 (direct sparse
+ * solve of diag(S) - A - A^T with the source couplings as right-hand side,
+ * current I = sum (1 - V_i) B_i) for the real conductances of nf_network. */
+typedef struct nf_conductance_info {
+  double conductance;   /* I = current from the source (units of tunnel_g0)   */
+  double current_sink;  /* current into the sink; equals I at convergence     */
+  double residual;      /* final ||b - A x|| / ||b|| of the reduced system    */
+  double solve_ms;      /* device time of set-up-free CG iterations + currents */
+  uint32_t unknowns;    /* rods in components touching both electrodes         */
+  uint32_t iterations;  /* conjugate-gradient iterations                       */
+  int32_t converged;    /* 1 when residual <= tol                              */
+  uint32_t pad;
+} nf_conductance_info;
+
+/* Jacobi-preconditioned FP64 conjugate gradients on the rods of the
+ * components that touch both electrodes (the only ones carrying current),
+ * until ||r|| <= tol ||b|| or max_iter iterations.  potentials (N doubles,
+ * may be NULL) receives every rod's potential: solved, 1 or 0 for components
+ * touching only the source or only the sink, NaN for floating ones.
+ * Requires nf_network. */
+int nf_conductance(nf_handle* h, double tol, uint32_t max_iter, nf_conductance_info* info,
+                   double* potentials);
+
 /* ---- output writers ---------------------------------------------------- */
 
 /* Legacy-ASCII VTK of instance 0 as write_CNTvtk (src/IO.cpp:38-76). */

@@ -165,6 +165,24 @@ class StateHeader(C.Structure):
                 "n_rods": self.n_rods, "tol": self.tol}
 
 
+class ConductanceInfo(C.Structure):
+    """nf_conductance_info (include/nanofiller.h)."""
+
+    _fields_ = [
+        ("conductance", C.c_double),
+        ("current_sink", C.c_double),
+        ("residual", C.c_double),
+        ("solve_ms", C.c_double),
+        ("unknowns", C.c_uint32),
+        ("iterations", C.c_uint32),
+        ("converged", C.c_int32),
+        ("pad", C.c_uint32),
+    ]
+
+    def to_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
+
+
 _lib_handle = None
 
 # (name, restype, argtypes) of every exported entry point
@@ -196,6 +214,8 @@ _SIGS = [
     ("nf_get_network", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_write_vtk", C.c_int, [C.c_void_p, C.c_char_p]),
     ("nf_write_coo", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("nf_conductance", C.c_int, [C.c_void_p, C.c_double, C.c_uint32, C.POINTER(ConductanceInfo),
+                                 C.c_void_p]),
     ("nf_save_state", C.c_int, [C.c_void_p, C.c_char_p]),
     ("nf_read_state_header", C.c_int, [C.c_char_p, C.POINTER(StateHeader)]),
     ("nf_load_state", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint32)]),
@@ -405,6 +425,17 @@ class Packing:
         _check(lib().nf_get_network(self._h, labels.ctypes.data, row.ctypes.data, col.ctypes.data,
                                     val.ctypes.data, info.nnz), self._h)
         return info.to_dict(), {"labels": labels, "row_ptr": row, "col": col, "val": val}
+
+    def conductance(self, tol: float = 1e-10, max_iter: int = 20000, potentials: bool = False):
+        """DC conductance between the electrodes of the last network():
+        (info dict, rod potentials or None)."""
+        info = ConductanceInfo()
+        n = C.c_uint32()
+        _check(lib().nf_get_rods(self._h, 0, None, 0, C.byref(n)), self._h)
+        v = np.zeros(max(n.value, 1), np.float64) if potentials else None
+        _check(lib().nf_conductance(self._h, tol, max_iter, C.byref(info),
+                                    v.ctypes.data if v is not None else None), self._h)
+        return info.to_dict(), (v[: n.value] if v is not None else None)
 
     def write_vtk(self, path: str) -> None:
         _check(lib().nf_write_vtk(self._h, path.encode()), self._h)

@@ -29,11 +29,13 @@ SOURCES = [
     "nf_pair.hip",
     "nf_post.hip",
     "nf_net.hip",
+    "nf_solve.hip",
     "nf_api.cpp",
     "nf_contacts.cpp",
     "nf_network.cpp",
     "nf_io.cpp",
     "nf_state.cpp",
+    "nf_conduct.cpp",
 ]
 HEADERS = ["nf_common.h", "nf_internal.h", "nf_host.h"]
 ARCH = os.environ.get("NF_OFFLOAD_ARCH", "gfx950")

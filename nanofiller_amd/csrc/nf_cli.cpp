@@ -16,7 +16,9 @@
 //                   src/statistics_collection.cu:53-154)
 // stat_conduct d    contacts, network along axis d, COO files internal.srt.coo
 //                   / source.elec.coo / sink.elec.coo (collect_cunduct_data,
-//                   src/conductivity.cu:140-423; "internal" is accepted)
+//                   src/conductivity.cu:140-423; "internal" is accepted), and
+//                   the DC conductance between the electrodes (the solve of
+//                   tools/kirchhoff/kirchhoff.py:256-272)
 //
 // As in the reference, generate saves output.bin when WRT_BIN is set
 // (save_state_BIN, src/stateSR.cpp:54-88), and without generate the rods
@@ -233,6 +235,17 @@ int main(int argc, char** argv) {
                     (unsigned long long)nc2, (unsigned long long)ni.n_edges, ni.n_components,
                     ni.largest, ni.percolating, ni.n_source, ni.n_sink,
                     (unsigned long long)ni.nnz);
+        // DC solve between the electrodes (kirchhoff.py's role, real conductances)
+        nf_conductance_info ci;
+        const int rc2 = nf_conductance(h2, 1e-10, 20000, &ci, nullptr);
+        if (rc2 != NF_OK) {
+          status = die(h2, "conductance", rc2);
+        } else {
+          std::printf("effective conductance %.10e (units of TUNNEL_G0) over %u rods, "
+                      "%u CG iterations, residual %.2e%s\n",
+                      ci.conductance, ci.unknowns, ci.iterations, ci.residual,
+                      ci.converged ? "" : " (not converged)");
+        }
       }
       if (h2) nf_destroy(h2);
     } else {

@@ -253,6 +253,41 @@ void launch_vdw(const float4* P, const float4* D, uint32_t n, const uint32_t* ro
                 double B, double* rowsum, double* bsum, double* out, unsigned long long* pairs,
                 hipStream_t s);
 
+// ---- conductance solve (nf_solve.hip) ----
+constexpr int CG_MAX_BLOCKS = 1024;
+// Device scalars of the conjugate-gradient solve.
+struct CgScal {
+  double rz, pq, alpha, beta, rr, bb;  // r.z, p.Ap, step, direction update, r.r, b.b
+  double isrc, isnk;                   // currents into the source and out to the sink
+};
+struct CgArgs {
+  uint32_t m;  // unknowns
+  const uint32_t* crow;
+  const uint32_t* ccol;
+  const double* cval;
+  const double* dinv;
+  const double* bsrc;  // source couplings (right-hand side)
+  const double* bsnk;  // sink couplings
+  double *x, *r, *z, *p, *q;
+  double* part;  // 2 * CG_MAX_BLOCKS block partials
+  CgScal* scal;
+};
+int cg_grid(uint64_t m);
+void launch_cs_flag(const uint32_t* nrow, const uint32_t* ncol, const uint32_t* labels, uint32_t n,
+                    uint32_t* flag, hipStream_t s);
+void launch_cs_active(const uint32_t* nrow, const uint32_t* ncol, const uint32_t* labels,
+                      const uint32_t* flag, uint32_t n, uint32_t* act, uint32_t* rl, hipStream_t s);
+void launch_cs_fill(const uint32_t* nrow, const uint32_t* ncol, const double* nval, uint32_t n,
+                    const uint32_t* act, const uint32_t* idx, const uint32_t* off, uint32_t* crow,
+                    uint32_t* ccol, double* cval, double* dinv, double* bsrc, double* bsnk,
+                    hipStream_t s);
+void launch_cs_potential(uint32_t n, const uint32_t* labels, const uint32_t* flag,
+                         const uint32_t* act, const uint32_t* idx, const double* x, double* v,
+                         hipStream_t s);
+void launch_cg_init(const CgArgs& a, hipStream_t s);
+void launch_cg_iteration(const CgArgs& a, hipStream_t s);
+void launch_cg_current(const CgArgs& a, hipStream_t s);
+
 // ---- network extraction (nf_net.hip) ----
 enum NetSummary {
   SUM_EDGES = 0,

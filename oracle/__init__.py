@@ -241,3 +241,43 @@ def centre_distance(p, rods, bins, width):
     n = lib().orc_centre_distance(C.byref(p), r.ctypes.data, r.shape[0], bins, width,
                                   counts.ctypes.data, C.byref(s1), C.byref(s2))
     return {"counts": counts, "pairs": n, "sum_d_q24": s1.value, "sum_d2_q16": s2.value}
+
+
+def conductance(net: dict):
+    """DC conductance between the electrodes of a Kirchhoff network (source at
+    potential 1, sink at 0), restating tools/kirchhoff/kirchhoff.py:256-272
+    (solver_type 0: scipy spsolve of the Laplacian with the source couplings
+    as right-hand side, current I = sum (1 - V_i) B_i) for real conductances.
+
+    Only components touching both electrodes are solved; rods of components
+    touching only the source / only the sink sit at 1 / 0, floating ones are
+    NaN.  Returns the source and sink currents, the unknown count and every
+    rod's potential.
+    """
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import spsolve
+
+    labels = np.asarray(net["labels"], np.int64)
+    n = len(labels)
+    row = np.asarray(net["row_ptr"], np.int64)
+    m = sp.csr_matrix((net["val"], net["col"], row), shape=(n + 2, n + 2))
+    flag = np.zeros(n, np.int64)
+    for e, bit in ((n, 1), (n + 1, 2)):
+        cols = net["col"][row[e]:row[e + 1]]
+        for j in cols[cols < n]:
+            flag[labels[j]] |= bit
+    f = flag[labels] if n else flag
+    act = np.flatnonzero(f == 3)
+    v = np.full(n, np.nan)
+    v[f == 1] = 1.0
+    v[f == 2] = 0.0
+    isrc = isnk = 0.0
+    if len(act):
+        a = m[act][:, act].tocsc()
+        b = -m[act][:, [n]].toarray().ravel()
+        bk = -m[act][:, [n + 1]].toarray().ravel()
+        x = spsolve(a, b) if len(act) > 1 else np.array([b[0] / a.toarray()[0, 0]])
+        v[act] = x
+        isrc = float(np.dot(b, 1.0 - x))
+        isnk = float(np.dot(bk, x))
+    return {"conductance": isrc, "current_sink": isnk, "unknowns": len(act), "potentials": v}

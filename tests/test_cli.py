@@ -59,3 +59,4 @@ def test_cli_end_to_end(tmp_path):
     rv = np.array([int(x) for x in rho.replace(",", " ").split()])
     np.testing.assert_array_equal(rv, h["phitheta"].ravel())
     assert f"contacts (gap <= 0.01 um): {c.n};" in r.stdout
+    assert "effective conductance" in r.stdout
