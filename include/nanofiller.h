@@ -38,7 +38,7 @@ extern "C" {
 /* ---- laws --------------------------------------------------------------- */
 enum { NF_LEN_CONST = 0, NF_LEN_LOGNORMAL = 1 };
 enum { NF_ORT_ISO = 0, NF_ORT_VMF = 1 };
-enum { NF_DEN_UNIFORM = 0, NF_DEN_AGGLOM = 1 };
+enum { NF_DEN_UNIFORM = 0, NF_DEN_AGGLOM = 1, NF_DEN_MESH = 2 };
 enum { NF_PREC_FP64 = 0, NF_PREC_FP32_RECHECK = 1 };
 enum { NF_STATUS_RUNNING = 0, NF_STATUS_TARGET = 1, NF_STATUS_MAX_ROUNDS = 2,
        NF_STATUS_STALLED = 3, NF_STATUS_OVERFLOW = 4 };
@@ -82,6 +82,10 @@ typedef struct nf_params {
   double tunnel_lambda; /* tunnelling decay length (um), g = g0 exp(-gap/l)*/
   double tunnel_g0;     /* contact conductance prefactor                   */
   int32_t write_state;  /* WRT_BIN: the CLI saves output.bin after generate */
+  int32_t den_ext[3];   /* NF_DEN_MESH: mesh points per axis                 */
+  const float* den_mesh;/* NF_DEN_MESH: ext0*ext1*ext2 values in [0, 1], C
+                           order (x slowest, as the reference's file); not
+                           owned, read by nf_create                          */
 } nf_params;
 
 /* Generation statistics (reference prints these; cudaMain.cu:98-157). */
@@ -269,6 +273,21 @@ int nf_write_vtk(nf_handle* h, const char* path);
 /* Contact lists as save_coo (src/conductivity.cu:838-879): <prefix>internal.srt.coo,
  * <prefix>sink.elec.coo, <prefix>source.elec.coo. */
 int nf_write_coo(nf_handle* h, const char* prefix);
+
+/* ---- density mesh files ------------------------------------------------ */
+
+/* Reads a density map in the reference's text format (read_density,
+ * src/IO.cpp:11-36): physDim[3], ext[3], then ext0*ext1*ext2 values in C
+ * order ([ix][iy][iz], x slowest).  values may be NULL (header only); cap
+ * is its length.  The density at a point is the trilinear interpolation of
+ * the map with periodic wrap and mesh points at the centres of ext cells of
+ * the box, as the reference's normalised, linearly filtered, wrapping 3-D
+ * texture (src/den2tex.h:5-53) - evaluated exactly in FP64. */
+int nf_read_density_mesh(const char* path, double dim[3], int32_t ext[3], float* values,
+                         uint64_t cap);
+/* Writes a density map in the same format. */
+int nf_write_density_mesh(const char* path, const double dim[3], const int32_t ext[3],
+                          const float* values);
 
 /* ---- state files ------------------------------------------------------- */
 

@@ -26,7 +26,7 @@ NF_E_INVALID, NF_E_NOMEM, NF_E_HIP, NF_E_IO = -1, -2, -3, -4
 NF_E_OVERFLOW, NF_E_STATE, NF_E_NODEVICE, NF_E_CAPACITY = -5, -6, -7, -8
 NF_LEN_CONST, NF_LEN_LOGNORMAL = 0, 1
 NF_ORT_ISO, NF_ORT_VMF = 0, 1
-NF_DEN_UNIFORM, NF_DEN_AGGLOM = 0, 1
+NF_DEN_UNIFORM, NF_DEN_AGGLOM, NF_DEN_MESH = 0, 1, 2
 NF_PREC_FP64, NF_PREC_FP32_RECHECK = 0, 1
 NF_STATUS_RUNNING, NF_STATUS_TARGET, NF_STATUS_MAX_ROUNDS, NF_STATUS_STALLED, NF_STATUS_OVERFLOW = 0, 1, 2, 3, 4
 STATUS = {0: "running", 1: "target", 2: "max_rounds", 3: "stalled", 4: "overflow"}
@@ -75,16 +75,32 @@ class Params(C.Structure):
         ("tunnel_lambda", C.c_double),
         ("tunnel_g0", C.c_double),
         ("write_state", C.c_int32),
+        ("den_ext", C.c_int32 * 3),
+        ("den_mesh", C.POINTER(C.c_float)),
     ]
 
     def copy(self) -> "Params":
         p = Params()
         C.pointer(p)[0] = self
+        if hasattr(self, "_mesh"):
+            p._mesh = self._mesh  # keeps the density map the pointer refers to alive
         return p
+
+    def set_density_mesh(self, values: np.ndarray) -> None:
+        """Density map law: values[ix, iy, iz] in [0, 1] over the box."""
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        assert v.ndim == 3
+        self._mesh = v
+        self.den_law = NF_DEN_MESH
+        for i in range(3):
+            self.den_ext[i] = v.shape[i]
+        self.den_mesh = v.ctypes.data_as(C.POINTER(C.c_float))
 
     def to_dict(self) -> dict:
         out = {}
         for name, _ in self._fields_:
+            if name == "den_mesh":
+                continue
             v = getattr(self, name)
             out[name] = list(v) if hasattr(v, "__len__") else v
         return out
@@ -216,6 +232,8 @@ _SIGS = [
     ("nf_write_coo", C.c_int, [C.c_void_p, C.c_char_p]),
     ("nf_conductance", C.c_int, [C.c_void_p, C.c_double, C.c_uint32, C.POINTER(ConductanceInfo),
                                  C.c_void_p]),
+    ("nf_read_density_mesh", C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("nf_write_density_mesh", C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("nf_save_state", C.c_int, [C.c_void_p, C.c_char_p]),
     ("nf_read_state_header", C.c_int, [C.c_char_p, C.POINTER(StateHeader)]),
     ("nf_load_state", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint32)]),
@@ -257,13 +275,32 @@ def default_params(**kw) -> Params:
     p = Params()
     lib().nf_params_default(C.byref(p))
     for k, v in kw.items():
-        if k in ("box", "axis"):
+        if k == "den_mesh":
+            p.set_density_mesh(v)
+        elif k in ("box", "axis"):
             arr = getattr(p, k)
             for i in range(3):
                 arr[i] = float(v[i])
         else:
             setattr(p, k, v)
     return p
+
+
+def read_density_mesh(path: str):
+    """Density map file (read_density format): (dims, values[ix, iy, iz])."""
+    dim = (C.c_double * 3)()
+    ext = (C.c_int32 * 3)()
+    _check(lib().nf_read_density_mesh(path.encode(), dim, ext, None, 0))
+    v = np.zeros(tuple(ext), np.float32)
+    _check(lib().nf_read_density_mesh(path.encode(), dim, ext, v.ctypes.data, v.size))
+    return tuple(dim), v
+
+
+def write_density_mesh(path: str, dims, values: np.ndarray) -> None:
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    dim = (C.c_double * 3)(*[float(x) for x in dims])
+    ext = (C.c_int32 * 3)(*v.shape)
+    _check(lib().nf_write_density_mesh(path.encode(), dim, ext, v.ctypes.data))
 
 
 def load_params(path: str, base: Params | None = None) -> Params:

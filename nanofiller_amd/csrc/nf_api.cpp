@@ -173,6 +173,22 @@ extern "C" int nf_params_normalize(nf_params* p) {
   if (p->den_law == NF_DEN_AGGLOM) {
     if (p->n_agg < 1 || !(p->agg_sigma > 0.0) || !(p->agg_peak >= 1.0))
       return fail(nullptr, NF_E_INVALID, "agglomerates need n_agg >= 1, sigma > 0, peak >= 1");
+  } else if (p->den_law == NF_DEN_MESH) {
+    if (!p->den_mesh) return fail(nullptr, NF_E_INVALID, "the mesh density law needs den_mesh");
+    uint64_t nv = 1;
+    for (int d = 0; d < 3; ++d) {
+      if (p->den_ext[d] < 1 || p->den_ext[d] > 4096)
+        return fail(nullptr, NF_E_INVALID, "density map extents must be 1..4096");
+      nv *= (uint64_t)p->den_ext[d];
+    }
+    double sum = 0.0;
+    for (uint64_t i = 0; i < nv; ++i) {
+      const float v = p->den_mesh[i];
+      if (!(v >= 0.0f && v <= 1.0f))
+        return fail(nullptr, NF_E_INVALID, "density map values must lie in [0, 1]");
+      sum += v;
+    }
+    if (!(sum > 0.0)) return fail(nullptr, NF_E_INVALID, "density map is zero everywhere");
   } else if (p->den_law != NF_DEN_UNIFORM) {
     return fail(nullptr, NF_E_INVALID, "unknown density law");
   }
@@ -287,6 +303,8 @@ static int plan_instance(const nf_params& p, InstDesc& d, std::vector<double>& a
   s.den_law = p.den_law;
   s.n_agg = p.den_law == NF_DEN_AGGLOM ? p.n_agg : 0;
   s.agg = nullptr;
+  s.mesh = nullptr;  // device copy, set at upload
+  for (int a = 0; a < 3; ++a) s.mext[a] = p.den_law == NF_DEN_MESH ? p.den_ext[a] : 0;
   agg.resize(3 * (size_t)s.n_agg);
   for (int k = 0; k < s.n_agg; ++k) agg_centre(p.seed, p.stream, p.box, (uint32_t)k, &agg[3 * k]);
 
@@ -327,7 +345,17 @@ static int plan_instance(const nf_params& p, InstDesc& d, std::vector<double>& a
   stencil.insert(stencil.end(), half.begin(), half.end());
   // per-cell capacity for accepted rods
   const double cellV = V / (double)ncells;
-  const double peak = p.den_law == NF_DEN_AGGLOM ? p.agg_peak : 1.0;
+  double peak = p.den_law == NF_DEN_AGGLOM ? p.agg_peak : 1.0;
+  if (p.den_law == NF_DEN_MESH) {
+    // densest point relative to the mean of the map
+    const uint64_t nv = (uint64_t)p.den_ext[0] * p.den_ext[1] * p.den_ext[2];
+    double sum = 0.0, mx = 0.0;
+    for (uint64_t i = 0; i < nv; ++i) {
+      sum += p.den_mesh[i];
+      mx = std::max(mx, (double)p.den_mesh[i]);
+    }
+    peak = mx * (double)nv / sum;
+  }
   const double lam = (double)p.n_rods * cellV / V * peak;
   double cap = std::ceil(lam * 1.25 + 8.0 * std::sqrt(lam) + 16.0);
   if (cap > (double)p.n_rods + 16.0) cap = (double)p.n_rods + 16.0;
@@ -348,7 +376,7 @@ static int plan_instance(const nf_params& p, InstDesc& d, std::vector<double>& a
 
 static void release_gen(nf_handle* h) {
   GenBuffers& b = h->gb;
-  void* ptrs[] = {b.desc, b.st, b.ri, b.slot_base, b.stencil, b.agg, b.cell_inst, b.oP, b.oD,
+  void* ptrs[] = {b.desc, b.st, b.ri, b.slot_base, b.stencil, b.agg, b.mesh, b.cell_inst, b.oP, b.oD,
                   b.oid, b.ostart, b.ocount, b.ohl, b.bP, b.bD, b.sP, b.sD, b.scell, b.srk,
                   b.surv, b.srank, b.pP, b.pD, b.pslot, b.palive, b.pcount, b.pstart, b.phl,
                   b.queues, b.scan_tmp};
@@ -394,7 +422,8 @@ extern "C" int nf_create_batch(const nf_params* ps, int n, nf_handle** out) {
   std::vector<std::vector<int4>> sts(n);
   GenBuffers& b = h->gb;
   b.n_inst = n;
-  uint64_t cell_base = 0, old_off = 0, rod_off = 0, T_cap = 0, agg_total = 0, st_total = 0;
+  uint64_t cell_base = 0, old_off = 0, rod_off = 0, T_cap = 0, agg_total = 0, st_total = 0,
+           mesh_total = 0;
   for (int i = 0; i < n; ++i) {
     InstDesc& d = h->desc[i];
     if (plan_instance(h->params[i], d, aggs[i], sts[i]) != NF_OK) {
@@ -411,6 +440,8 @@ extern "C" int nf_create_batch(const nf_params* ps, int n, nf_handle** out) {
     rod_off += d.n_target;
     T_cap += d.max_batch;
     agg_total += aggs[i].size();
+    if (h->params[i].den_law == NF_DEN_MESH)
+      mesh_total += (uint64_t)d.smp.mext[0] * d.smp.mext[1] * d.smp.mext[2];
     st_total += sts[i].size();
     if (cell_base > 0xFFFFFFF0ull || old_off > 0xFFFFFFF0ull || rod_off > 0xFFFFFFF0ull ||
         T_cap > 0xFFFFFFF0ull) {
@@ -431,6 +462,7 @@ extern "C" int nf_create_batch(const nf_params* ps, int n, nf_handle** out) {
   ALLOC(b.slot_base, n + 1);
   ALLOC(b.stencil, st_total);
   ALLOC(b.agg, agg_total);
+  ALLOC(b.mesh, mesh_total);
   ALLOC(b.cell_inst, b.total_cells);
   ALLOC(b.oP, b.total_old);
   ALLOC(b.oD, b.total_old);
@@ -463,12 +495,19 @@ extern "C" int nf_create_batch(const nf_params* ps, int n, nf_handle** out) {
   }
   // upload descriptors, agglomerates, stencils, per-cell tables
   std::vector<double> agg_all;
+  std::vector<float> mesh_all;
   std::vector<int4> st_all;
   std::vector<uint32_t> cinst(b.total_cells), ostart(b.total_cells);
   for (int i = 0; i < n; ++i) {
     InstDesc& d = h->desc[i];
     d.smp.agg = aggs[i].empty() ? nullptr : b.agg + agg_all.size();
     agg_all.insert(agg_all.end(), aggs[i].begin(), aggs[i].end());
+    if (h->params[i].den_law == NF_DEN_MESH) {
+      const uint64_t nv = (uint64_t)d.smp.mext[0] * d.smp.mext[1] * d.smp.mext[2];
+      d.smp.mesh = b.mesh + mesh_all.size();
+      mesh_all.insert(mesh_all.end(), h->params[i].den_mesh, h->params[i].den_mesh + nv);
+      h->params[i].den_mesh = nullptr;  // the caller's buffer: not kept past nf_create
+    }
     st_all.insert(st_all.end(), sts[i].begin(), sts[i].end());
     for (uint32_t c = 0; c < d.ncells; ++c) {
       cinst[d.cell_base + c] = (uint32_t)i;
@@ -479,6 +518,8 @@ extern "C" int nf_create_batch(const nf_params* ps, int n, nf_handle** out) {
   if (e == hipSuccess) e = hipMemcpy(b.desc, h->desc.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice);
   if (e == hipSuccess && !agg_all.empty())
     e = hipMemcpy(b.agg, agg_all.data(), sizeof(double) * agg_all.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !mesh_all.empty())
+    e = hipMemcpy(b.mesh, mesh_all.data(), sizeof(float) * mesh_all.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess && !st_all.empty())
     e = hipMemcpy(b.stencil, st_all.data(), sizeof(int4) * st_all.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)

@@ -173,7 +173,43 @@ struct SampleDesc {
   uint32_t seed, stream;
   int32_t len_law, ort_law, den_law, n_agg;
   const double* agg;  // 3 doubles per agglomerate centre
+  const float* mesh;  // density map, C order (x slowest), NF_DEN_MESH
+  int32_t mext[3];
 };
+
+// Density map value at (x, y, z): trilinear interpolation with periodic
+// wrap, mesh points at the centres of the map's cells - the reference's
+// normalised, linearly filtered, wrapping 3-D texture (src/den2tex.h:5-53,
+// read at inc/cuda_CNTgenerate.h:228), in FP64 with a fixed evaluation
+// order (z, then y, then x).
+NF_HD double mesh_density(const float* v, const int32_t e[3], const double box[3], double x, double y,
+                          double z) {
+  const double pos[3] = {x, y, z};
+  int i0[3], i1[3];
+  double f[3];
+  for (int a = 0; a < 3; ++a) {
+    const double t = (pos[a] / box[a]) * (double)e[a] - 0.5;
+    const double fl = floor(t);
+    f[a] = t - fl;
+    int k = (int)fl;  // -1 .. e - 1 for positions in [0, box)
+    if (k < 0) k += e[a];
+    if (k >= e[a]) k -= e[a];
+    i0[a] = k;
+    i1[a] = k + 1 == e[a] ? 0 : k + 1;
+  }
+  const uint64_t sy = (uint64_t)e[2], sx = (uint64_t)e[1] * (uint64_t)e[2];
+  const uint64_t x0 = (uint64_t)i0[0] * sx, x1 = (uint64_t)i1[0] * sx;
+  const uint64_t y0 = (uint64_t)i0[1] * sy, y1 = (uint64_t)i1[1] * sy;
+  const uint64_t z0 = (uint64_t)i0[2], z1 = (uint64_t)i1[2];
+  const double gz = 1.0 - f[2], gy = 1.0 - f[1], gx = 1.0 - f[0];
+  const double c00 = (double)v[x0 + y0 + z0] * gz + (double)v[x0 + y0 + z1] * f[2];
+  const double c01 = (double)v[x0 + y1 + z0] * gz + (double)v[x0 + y1 + z1] * f[2];
+  const double c10 = (double)v[x1 + y0 + z0] * gz + (double)v[x1 + y0 + z1] * f[2];
+  const double c11 = (double)v[x1 + y1 + z0] * gz + (double)v[x1 + y1 + z1] * f[2];
+  const double c0 = c00 * gy + c01 * f[1];
+  const double c1 = c10 * gy + c11 * f[1];
+  return c0 * gx + c1 * f[0];
+}
 
 // Basis perpendicular to the vMF axis mu: the least-aligned coordinate axis
 // projected out of mu, normalised, and mu x e1.
@@ -586,6 +622,10 @@ NF_HD int sample_candidate(const SampleDesc& d, uint32_t round, uint32_t slot, R
     const double inv2s2 = 1.0 / (2.0 * d.agg_sigma * d.agg_sigma);
     const double rho = 1.0 + (d.agg_peak - 1.0) * dexp(-(d2min * inv2s2));
     if (!(ud * d.agg_peak < rho)) return 0;
+  } else if (d.den_law == NF_DEN_MESH) {
+    // the map's values are at most 1: accept iff u < rho(x) (cuda_CNTgenerate.h:227-232)
+    const double rho = mesh_density(d.mesh, d.mext, d.box, (double)xf, (double)yf, (double)zf);
+    if (!(ud < rho)) return 0;
   }
 
   double cx = 0.0, cy = 0.0, cz = 0.0;

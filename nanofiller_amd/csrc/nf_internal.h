@@ -89,6 +89,7 @@ struct GenBuffers {
   uint32_t* slot_base = nullptr;  // n_inst+1, exclusive prefix of M
   int4* stencil = nullptr;
   double* agg = nullptr;
+  float* mesh = nullptr;          // density maps of the NF_DEN_MESH instances
   uint32_t* cell_inst = nullptr;  // instance of every global cell
   // accepted rods in fixed-capacity cell slots
   float4* oP = nullptr;

@@ -7,6 +7,8 @@
 //   INC_LEN_LAW CONST | LOGNORMAL median sigma min max
 //   ORT_LAW ISO | VMF kappa ax ay az
 //   DENSITY_LAW UNIFORM | AGGLOM n sigma peak
+//   SPAT_DISTR_FILE file             density map (read_density format); sets
+//                                    the box and the mesh density law
 //   CELL_SIZE edge                   generation cell edge (um)
 //   PRECISION FP64 | FP32            FP32 = FP32 with FP64 re-check band
 //   ELECTRODE_AXIS 0|1|2|x|y|z, TUNNEL_LAMBDA l, TUNNEL_G0 g0
@@ -14,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <fstream>
 #include <iomanip>
 #include <sstream>
@@ -70,10 +73,29 @@ extern "C" int nf_params_load(const char* path, nf_params* p) {
     } else if (key == "SPAT_DISTR_FILE") {
       std::string fn;
       is >> fn;
-      if (fn != "NONE") return bad("density-map files are not supported; use DENSITY_LAW");
-      int ext[3];
-      if (!(is >> p->box[0] >> p->box[1] >> p->box[2])) return bad("expected NONE Lx Ly Lz ex ey ez");
-      is >> ext[0] >> ext[1] >> ext[2];
+      if (fn == "NONE") {
+        int ext[3];
+        if (!(is >> p->box[0] >> p->box[1] >> p->box[2])) return bad("expected NONE Lx Ly Lz ex ey ez");
+        is >> ext[0] >> ext[1] >> ext[2];
+      } else {
+        // density map file: box from its dimensions, values kept by the
+        // library for the life of the process (the params point at them)
+        double dim[3];
+        int32_t ext[3];
+        if (nf_read_density_mesh(fn.c_str(), dim, ext, nullptr, 0) != NF_OK)
+          return bad(("cannot read density map: " + g_last_error).c_str());
+        std::vector<float> v((size_t)ext[0] * ext[1] * ext[2]);
+        if (nf_read_density_mesh(fn.c_str(), dim, ext, v.data(), v.size()) != NF_OK)
+          return bad(("cannot read density map: " + g_last_error).c_str());
+        static std::deque<std::vector<float>> kept;
+        kept.push_back(std::move(v));
+        for (int d = 0; d < 3; ++d) {
+          p->box[d] = dim[d];
+          p->den_ext[d] = ext[d];
+        }
+        p->den_mesh = kept.back().data();
+        p->den_law = NF_DEN_MESH;
+      }
     } else if (key == "PREF_ORT_PHI") {
       is >> pref_phi;
       ort_dev_given = true;
@@ -212,6 +234,49 @@ extern "C" int nf_params_load(const char* path, nf_params* p) {
   (void)pref_theta;
   (void)ort_dev_given;
   g_last_error = warn;
+  return NF_OK;
+}
+
+// ===========================================================================
+// Density maps: read_density (src/IO.cpp:11-36) - text: physDim[3], ext[3],
+// then the values in C order ([ix][iy][iz], x slowest).
+// ===========================================================================
+extern "C" int nf_read_density_mesh(const char* path, double dim[3], int32_t ext[3], float* values,
+                                    uint64_t cap) {
+  if (!path || !dim || !ext) return fail(nullptr, NF_E_INVALID, "null argument");
+  std::ifstream f(path);
+  if (!f.is_open()) return fail(nullptr, NF_E_IO, std::string("cannot open density map ") + path);
+  if (!(f >> dim[0] >> dim[1] >> dim[2] >> ext[0] >> ext[1] >> ext[2]))
+    return fail(nullptr, NF_E_INVALID, std::string(path) + ": expected Lx Ly Lz ex ey ez");
+  for (int d = 0; d < 3; ++d)
+    if (!(dim[d] > 0.0) || ext[d] < 1 || ext[d] > 4096)
+      return fail(nullptr, NF_E_INVALID, std::string(path) + ": bad dimensions or extents");
+  const uint64_t n = (uint64_t)ext[0] * ext[1] * ext[2];
+  if (!values) return NF_OK;
+  if (cap < n) return fail(nullptr, NF_E_CAPACITY, std::string(path) + ": value buffer too small");
+  for (uint64_t i = 0; i < n; ++i) {
+    double v;
+    if (!(f >> v))
+      return fail(nullptr, NF_E_INVALID, std::string(path) + ": expected " + std::to_string(n) +
+                                             " values, found " + std::to_string(i));
+    if (!(v >= 0.0 && v <= 1.0))
+      return fail(nullptr, NF_E_INVALID, std::string(path) + ": density values must lie in [0, 1] "
+                                         "(von Neumann acceptance)");
+    values[i] = (float)v;
+  }
+  return NF_OK;
+}
+
+extern "C" int nf_write_density_mesh(const char* path, const double dim[3], const int32_t ext[3],
+                                     const float* values) {
+  if (!path || !dim || !ext || !values) return fail(nullptr, NF_E_INVALID, "null argument");
+  std::ofstream f(path, std::ios::out | std::ios::trunc);
+  if (!f.is_open()) return fail(nullptr, NF_E_IO, std::string("cannot write ") + path);
+  f << std::setprecision(9) << dim[0] << " " << dim[1] << " " << dim[2] << "\n";
+  f << ext[0] << " " << ext[1] << " " << ext[2] << "\n";
+  const uint64_t n = (uint64_t)ext[0] * ext[1] * ext[2];
+  for (uint64_t i = 0; i < n; ++i) f << values[i] << ((i + 1) % (uint64_t)ext[2] ? " " : "\n");
+  if (!f.good()) return fail(nullptr, NF_E_IO, std::string("write failed: ") + path);
   return NF_OK;
 }
 

@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
             ("orc_agg_centres", None, [P(Params), C.c_void_p]),
             ("orc_vmf_basis", None, [C.c_void_p, C.c_void_p, C.c_void_p]),
             ("orc_candidate", C.c_int, [P(Params), C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+            ("orc_mesh_density", C.c_double, [P(Params), C.c_double, C.c_double, C.c_double]),
             ("orc_pair_d2", C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, P(C.c_int)]),
             ("orc_pair_batch", None, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                                       C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -116,6 +117,11 @@ def pair_batch(ra, rb, shifts, contact: bool, sep_or_gap: float, soft_frac: floa
                          float(sep_or_gap), float(soft_frac), d2.ctypes.data, w.ctypes.data,
                          cut.ctypes.data)
     return d2, w.astype(bool), cut.astype(bool)
+
+
+def mesh_density(p, x: float, y: float, z: float) -> float:
+    """Density map value at a point (trilinear, periodic; params' map)."""
+    return lib().orc_mesh_density(C.byref(p), x, y, z)
 
 
 def candidate(p, rnd: int, slot: int, agg=None):

@@ -286,6 +286,41 @@ static int self_image_overlap(const nf_params* p, const float* r) {
   return 0;
 }
 
+/* Density map value: trilinear interpolation with periodic wrap, mesh points
+ * at the centres of the map's cells, as the reference's normalised, linearly
+ * filtered, wrapping texture (src/den2tex.h:5-53; read at
+ * inc/cuda_CNTgenerate.h:228), in FP64; interpolation along z, then y,
+ * then x. */
+double orc_mesh_density(const nf_params* p, double x, double y, double z) {
+  const float* v = p->den_mesh;
+  const int32_t* e = p->den_ext;
+  double pos[3] = {x, y, z};
+  int i0[3], i1[3];
+  double f[3];
+  for (int a = 0; a < 3; ++a) {
+    double t = (pos[a] / p->box[a]) * (double)e[a] - 0.5;
+    double fl = floor(t);
+    f[a] = t - fl;
+    int k = (int)fl;
+    if (k < 0) k += e[a];
+    if (k >= e[a]) k -= e[a];
+    i0[a] = k;
+    i1[a] = k + 1 == e[a] ? 0 : k + 1;
+  }
+  uint64_t sy = (uint64_t)e[2], sx = (uint64_t)e[1] * (uint64_t)e[2];
+  uint64_t x0 = (uint64_t)i0[0] * sx, x1 = (uint64_t)i1[0] * sx;
+  uint64_t y0 = (uint64_t)i0[1] * sy, y1 = (uint64_t)i1[1] * sy;
+  uint64_t z0 = (uint64_t)i0[2], z1 = (uint64_t)i1[2];
+  double gz = 1.0 - f[2], gy = 1.0 - f[1], gx = 1.0 - f[0];
+  double c00 = (double)v[x0 + y0 + z0] * gz + (double)v[x0 + y0 + z1] * f[2];
+  double c01 = (double)v[x0 + y1 + z0] * gz + (double)v[x0 + y1 + z1] * f[2];
+  double c10 = (double)v[x1 + y0 + z0] * gz + (double)v[x1 + y0 + z1] * f[2];
+  double c11 = (double)v[x1 + y1 + z0] * gz + (double)v[x1 + y1 + z1] * f[2];
+  double c0 = c00 * gy + c01 * f[1];
+  double c1 = c10 * gy + c11 * f[1];
+  return c0 * gx + c1 * f[0];
+}
+
 int orc_candidate(const nf_params* p, const double* agg, uint32_t round, uint32_t slot,
                   float out[8]) {
   uint32_t o[4];
@@ -313,6 +348,9 @@ int orc_candidate(const nf_params* p, const double* agg, uint32_t round, uint32_
     double inv2s2 = 1.0 / (2.0 * p->agg_sigma * p->agg_sigma);
     double rho = 1.0 + (p->agg_peak - 1.0) * orc_exp(-(d2min * inv2s2));
     if (!(ud * p->agg_peak < rho)) return 0;
+  } else if (p->den_law == NF_DEN_MESH) {
+    double rho = orc_mesh_density(p, (double)xf, (double)yf, (double)zf);
+    if (!(ud < rho)) return 0;
   }
 
   double cx = 0.0, cy = 0.0, cz = 0.0;

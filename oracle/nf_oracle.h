@@ -36,6 +36,7 @@ double orc_log(double x);
 
 /* Agglomerate centres (3 doubles each) of instance p. */
 void orc_agg_centres(const nf_params* p, double* out);
+double orc_mesh_density(const nf_params* p, double x, double y, double z);
 /* Orthonormal basis e1, e2 perpendicular to the (unit) vMF axis. */
 void orc_vmf_basis(const double mu[3], double e1[3], double e2[3]);
 
