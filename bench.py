@@ -240,14 +240,23 @@ def leg_network(a, out: dict) -> None:
         t0 = time.perf_counter()
         info, _ = pk.network()
         t_n = time.perf_counter() - t0
+        pk.node_network()  # warm-up
+        t0 = time.perf_counter()
+        ninfo, _ = pk.node_network()
+        t_nd = time.perf_counter() - t0
     log(f"network: {c.n} contacts, {info['n_components']} components, largest {info['largest']}, "
         f"nnz {info['nnz']}, {t_n * 1e3:.1f} ms")
+    log(f"network: node level {ninfo['n_nodes']} nodes, {ninfo['n_segments']} segments, "
+        f"{ninfo['n_screened']} screened contacts, built in {ninfo['device_ms']:.1f} ms on the device "
+        f"({t_nd * 1e3:.1f} ms with every table copied to the host)")
     out["extra"]["network"] = {
         "workload": "8,000,000 rods, 40 um cube, l 1 um, d 10 nm, 16 agglomerates, vMF(+x, 3), "
                     "gap <= 2 nm, electrodes x = 0 / 40 um, g0 exp(-gap / 0.1 nm)",
         "generate_ms": t_gen * 1e3, "generate_rods_per_s": st.n_accepted / t_gen,
         "rounds": st.rounds, "contacts": int(c.n), "contacts_call_ms": t_c * 1e3,
         "network_call_ms": t_n * 1e3, "info": info,
+        "node_network": {"info": ninfo, "build_device_ms": ninfo["device_ms"],
+                         "call_with_host_copies_ms": t_nd * 1e3},
     }
 
 

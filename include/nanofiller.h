@@ -266,6 +266,42 @@ typedef struct nf_conductance_info {
 int nf_conductance(nf_handle* h, double tol, uint32_t max_iter, nf_conductance_info* info,
                    double* potentials);
 
+/* Node-level segment network: the internal-resistance branch of
+ * collect_cunduct_data (src/conductivity.cu:213-410): contact points as
+ * nodes, segments along the rods, screened contacts removed
+ * (check_screened_contacts :430-528), nodes within one radius merged
+ * (remove_redundant_nodes / reduce_node_tbl, src/adjacency_graph.cu:57-219).
+ * Built from the contacts of the last nf_contacts call and the electrode
+ * rules of nf_network (params.elec_axis, contact_gap). */
+typedef struct nf_node_info {
+  uint32_t n_points;    /* contact and electrode points before merging      */
+  uint32_t n_nodes;     /* nodes after merging                              */
+  uint32_t n_segments;  /* segments between consecutive nodes of a rod      */
+  uint32_t n_edges;     /* rod-rod contacts kept (network, not screened)   */
+  uint32_t n_screened;  /* contacts removed: a third rod blocks them       */
+  uint32_t n_source;    /* electrode points on the source plane            */
+  uint32_t n_sink;      /* electrode points on the sink plane              */
+  uint32_t pad;
+  double device_ms;     /* device time of the build                        */
+} nf_node_info;
+
+int nf_node_network(nf_handle* h, nf_node_info* info);
+/* Nodes in rod order, then t: rod[n_nodes], t[n_nodes] (axial position of
+ * the node, length units, mean of its merged points). */
+int nf_get_nodes(nf_handle* h, uint32_t* rod, double* t, uint64_t cap);
+/* Segments: node a < node b on one rod, L = (t_b - t_a) / radius, A = radius
+ * (get_segment_length, src/get_contact_data.cu:235-297). */
+int nf_get_segments(nf_handle* h, uint32_t* na, uint32_t* nb, double* L, double* A, uint64_t cap);
+/* Kept rod-rod contacts as node pairs, in contact-list order; contact[]
+ * gives each one's index in the nf_get_contacts arrays.  All pointers NULL:
+ * count only. */
+int nf_get_node_edges(nf_handle* h, uint32_t* na, uint32_t* nb, uint32_t* contact, uint64_t cap,
+                      uint64_t* count);
+/* Electrode couplings as (node, surface gap, side 0 = source / 1 = sink):
+ * all source couplings in rod order, then the sink ones. */
+int nf_get_node_electrodes(nf_handle* h, uint32_t* node, double* gap, int32_t* side, uint64_t cap,
+                           uint64_t* count);
+
 /* ---- output writers ---------------------------------------------------- */
 
 /* Legacy-ASCII VTK of instance 0 as write_CNTvtk (src/IO.cpp:38-76). */
@@ -273,6 +309,11 @@ int nf_write_vtk(nf_handle* h, const char* path);
 /* Contact lists as save_coo (src/conductivity.cu:838-879): <prefix>internal.srt.coo,
  * <prefix>sink.elec.coo, <prefix>source.elec.coo. */
 int nf_write_coo(nf_handle* h, const char* prefix);
+/* Node network files of the internal-resistance branch (save_coo,
+ * src/conductivity.cu:881-930): <prefix>Nodes.tbl.dat (node, rod, t),
+ * segment.LA.coo (a, b, L, A), internal.DS.coo (a, b, D, S), sink.DS.coo
+ * and source.DS.coo (electrode id, node, D, S). */
+int nf_write_node_coo(nf_handle* h, const char* prefix);
 
 /* ---- density mesh files ------------------------------------------------ */
 

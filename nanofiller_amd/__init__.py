@@ -181,6 +181,25 @@ class StateHeader(C.Structure):
                 "n_rods": self.n_rods, "tol": self.tol}
 
 
+class NodeInfo(C.Structure):
+    """nf_node_info (include/nanofiller.h)."""
+
+    _fields_ = [
+        ("n_points", C.c_uint32),
+        ("n_nodes", C.c_uint32),
+        ("n_segments", C.c_uint32),
+        ("n_edges", C.c_uint32),
+        ("n_screened", C.c_uint32),
+        ("n_source", C.c_uint32),
+        ("n_sink", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("device_ms", C.c_double),
+    ]
+
+    def to_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
+
+
 class ConductanceInfo(C.Structure):
     """nf_conductance_info (include/nanofiller.h)."""
 
@@ -234,6 +253,15 @@ _SIGS = [
                                  C.c_void_p]),
     ("nf_read_density_mesh", C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_write_density_mesh", C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("nf_node_network", C.c_int, [C.c_void_p, C.POINTER(NodeInfo)]),
+    ("nf_get_nodes", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("nf_get_segments", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_uint64]),
+    ("nf_get_node_edges", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                    C.POINTER(C.c_uint64)]),
+    ("nf_get_node_electrodes", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                         C.POINTER(C.c_uint64)]),
+    ("nf_write_node_coo", C.c_int, [C.c_void_p, C.c_char_p]),
     ("nf_save_state", C.c_int, [C.c_void_p, C.c_char_p]),
     ("nf_read_state_header", C.c_int, [C.c_char_p, C.POINTER(StateHeader)]),
     ("nf_load_state", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint32)]),
@@ -462,6 +490,40 @@ class Packing:
         _check(lib().nf_get_network(self._h, labels.ctypes.data, row.ctypes.data, col.ctypes.data,
                                     val.ctypes.data, info.nnz), self._h)
         return info.to_dict(), {"labels": labels, "row_ptr": row, "col": col, "val": val}
+
+    def node_network(self) -> tuple[dict, dict]:
+        """Node-level segment network of the last contacts(): (info, arrays)."""
+        info = NodeInfo()
+        _check(lib().nf_node_network(self._h, C.byref(info)), self._h)
+        nn, ns = info.n_nodes, info.n_segments
+        rod = np.zeros(nn, np.uint32)
+        t = np.zeros(nn, np.float64)
+        _check(lib().nf_get_nodes(self._h, rod.ctypes.data, t.ctypes.data, nn), self._h)
+        sa = np.zeros(ns, np.uint32)
+        sb = np.zeros(ns, np.uint32)
+        sl = np.zeros(ns, np.float64)
+        sarea = np.zeros(ns, np.float64)
+        _check(lib().nf_get_segments(self._h, sa.ctypes.data, sb.ctypes.data, sl.ctypes.data,
+                                     sarea.ctypes.data, ns), self._h)
+        cnt = C.c_uint64()
+        _check(lib().nf_get_node_edges(self._h, None, None, None, 0, C.byref(cnt)), self._h)
+        ea = np.zeros(cnt.value, np.uint32)
+        eb = np.zeros(cnt.value, np.uint32)
+        ek = np.zeros(cnt.value, np.uint32)
+        _check(lib().nf_get_node_edges(self._h, ea.ctypes.data, eb.ctypes.data, ek.ctypes.data,
+                                       cnt.value, C.byref(cnt)), self._h)
+        _check(lib().nf_get_node_electrodes(self._h, None, None, None, 0, C.byref(cnt)), self._h)
+        en = np.zeros(cnt.value, np.uint32)
+        eg = np.zeros(cnt.value, np.float64)
+        es = np.zeros(cnt.value, np.int32)
+        _check(lib().nf_get_node_electrodes(self._h, en.ctypes.data, eg.ctypes.data, es.ctypes.data,
+                                            cnt.value, C.byref(cnt)), self._h)
+        return info.to_dict(), {"node_rod": rod, "node_t": t, "seg_a": sa, "seg_b": sb, "seg_L": sl,
+                                "seg_A": sarea, "edge_a": ea, "edge_b": eb, "edge_contact": ek,
+                                "elec_node": en, "elec_gap": eg, "elec_side": es}
+
+    def write_node_coo(self, prefix: str = "") -> None:
+        _check(lib().nf_write_node_coo(self._h, prefix.encode()), self._h)
 
     def conductance(self, tol: float = 1e-10, max_iter: int = 20000, potentials: bool = False):
         """DC conductance between the electrodes of the last network():

@@ -30,12 +30,14 @@ SOURCES = [
     "nf_post.hip",
     "nf_net.hip",
     "nf_solve.hip",
+    "nf_nodes.hip",
     "nf_api.cpp",
     "nf_contacts.cpp",
     "nf_network.cpp",
     "nf_io.cpp",
     "nf_state.cpp",
     "nf_conduct.cpp",
+    "nf_nodes.cpp",
 ]
 HEADERS = ["nf_common.h", "nf_internal.h", "nf_host.h"]
 ARCH = os.environ.get("NF_OFFLOAD_ARCH", "gfx950")

@@ -33,15 +33,44 @@ int fail(nf_handle* h, int code, const std::string& msg) {
   return code;
 }
 
+// Size classes: 256 B granules below 64 KiB, then four classes per power of
+// two, so a reused block wastes at most a quarter of its size.
+static size_t size_class(size_t bytes) {
+  if (bytes <= 65536) return (bytes + 255) & ~(size_t)255;
+  size_t p = 65536;
+  while (p < bytes) p <<= 1;
+  const size_t q = p >> 3;  // p/2 + k*p/8, k = 1..4
+  size_t c = p >> 1;
+  while (c < bytes) c += q;
+  return c;
+}
+
+constexpr size_t POOL_CAP = (size_t)16 << 30;  // freed bytes kept per handle
+
 int dalloc_bytes(nf_handle* h, void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 16;
+  const size_t cls = size_class(bytes);
+  auto it = h->pool.find(cls);
   void* q = nullptr;
-  hipError_t e = hipMalloc(&q, bytes);
-  if (e != hipSuccess)
-    return fail(h, NF_E_NOMEM,
-                "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
-  h->allocs.push_back(q);
+  if (it != h->pool.end()) {
+    q = it->second;
+    h->pool.erase(it);
+    h->pool_bytes -= cls;
+  } else {
+    hipError_t e = hipMalloc(&q, cls);
+    if (e != hipSuccess && !h->pool.empty()) {
+      // give the cached blocks back and try once more
+      for (auto& kv : h->pool) (void)hipFree(kv.second);
+      h->pool.clear();
+      h->pool_bytes = 0;
+      e = hipMalloc(&q, cls);
+    }
+    if (e != hipSuccess)
+      return fail(h, NF_E_NOMEM,
+                  "hipMalloc of " + std::to_string(cls) + " bytes failed: " + hipGetErrorString(e));
+  }
+  h->allocs[q] = cls;
   *p = q;
   return NF_OK;
 }
@@ -87,11 +116,25 @@ hipError_t sync_stream(hipStream_t s, const char* what) {
   }
 }
 
+// Returns a block to the handle's pool.  The handle's stream is drained
+// first (usually already idle), so no queued kernel can still use a block
+// that a later call receives.
 void dfree(nf_handle* h, void* p) {
   if (!p) return;
-  auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
-  if (it != h->allocs.end()) h->allocs.erase(it);
-  (void)hipFree(p);
+  auto it = h->allocs.find(p);
+  if (it == h->allocs.end()) {
+    (void)hipFree(p);
+    return;
+  }
+  const size_t cls = it->second;
+  h->allocs.erase(it);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->pool_bytes + cls > POOL_CAP) {
+    (void)hipFree(p);
+    return;
+  }
+  h->pool.emplace(cls, p);
+  h->pool_bytes += cls;
 }
 
 }  // namespace nf
@@ -387,7 +430,10 @@ static void release_gen(nf_handle* h) {
 void nf::release_contacts(nf_handle* h) {
   ContactBuffers& c = h->cb;
   void* ptrs[] = {c.cP, c.cD, c.cid, c.cstart, c.ccount, c.chl, c.ccur, c.stencil, c.cnt,
-                  c.rowptr, c.fill, c.J, c.gap, c.gap64, c.img, c.labels, c.nrow, c.ncol, c.nval};
+                  c.rowptr, c.fill, c.J, c.gap, c.gap64, c.img, c.labels, c.nrow, c.ncol, c.nval,
+                  c.nd_pflag, c.nd_edge_a, c.nd_edge_b, c.nd_esrc, c.nd_esnk, c.nd_src_node,
+                  c.nd_snk_node, c.nd_egs, c.nd_egk, c.nd_node_rod, c.nd_node_t, c.nd_seg_a,
+                  c.nd_seg_b, c.nd_seg_L, c.nd_seg_A};
   for (void* p : ptrs) dfree(h, p);
   c = ContactBuffers();
 }
@@ -543,7 +589,8 @@ extern "C" void nf_destroy(nf_handle* h) {
   if (h->graph) (void)hipGraphExecDestroy(h->graph);
   release_contacts(h);
   release_gen(h);
-  for (void* p : h->allocs) (void)hipFree(p);
+  for (auto& kv : h->allocs) (void)hipFree(kv.first);
+  for (auto& kv : h->pool) (void)hipFree(kv.second);
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -708,6 +755,7 @@ extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
   h->generated = true;
   h->cb.valid = false;
   h->cb.net_valid = false;
+  h->cb.nd_valid = false;
   if (status_rc != NF_OK) return fail(h, status_rc, "a cell exceeded its capacity during generation");
   return NF_OK;
 }
@@ -761,6 +809,7 @@ extern "C" int nf_set_rods(nf_handle* h, int inst, const float* rods, uint32_t n
   h->generated = false;
   h->cb.valid = false;
   h->cb.net_valid = false;
+  h->cb.nd_valid = false;
   return NF_OK;
 }
 

@@ -16,9 +16,11 @@
 //                   src/statistics_collection.cu:53-154)
 // stat_conduct d    contacts, network along axis d, COO files internal.srt.coo
 //                   / source.elec.coo / sink.elec.coo (collect_cunduct_data,
-//                   src/conductivity.cu:140-423; "internal" is accepted), and
-//                   the DC conductance between the electrodes (the solve of
-//                   tools/kirchhoff/kirchhoff.py:256-272)
+//                   src/conductivity.cu:140-423), the DC conductance between
+//                   the electrodes (the solve of tools/kirchhoff/kirchhoff.py:
+//                   256-272); with "internal" also the node-level files
+//                   Nodes.tbl.dat, segment.LA.coo, internal.DS.coo,
+//                   sink.DS.coo, source.DS.coo (:213-410)
 //
 // As in the reference, generate saves output.bin when WRT_BIN is set
 // (save_state_BIN, src/stateSR.cpp:54-88), and without generate the rods
@@ -208,7 +210,11 @@ int main(int argc, char** argv) {
         break;
       }
       const int dir = std::atoi(argv[++ai]);
-      if (ai + 1 < argc && !std::strcmp(argv[ai + 1], "internal")) ++ai;
+      bool internal = false;
+      if (ai + 1 < argc && !std::strcmp(argv[ai + 1], "internal")) {
+        internal = true;
+        ++ai;
+      }
       if (dir < 0 || dir > 2) {
         std::fprintf(stderr, "nanofiller: stat_conduct axis must be 0, 1 or 2\n");
         status = 2;
@@ -235,6 +241,20 @@ int main(int argc, char** argv) {
                     (unsigned long long)nc2, (unsigned long long)ni.n_edges, ni.n_components,
                     ni.largest, ni.percolating, ni.n_source, ni.n_sink,
                     (unsigned long long)ni.nnz);
+        if (internal && status == 0) {
+          // node level: contact points, segments along the rods, screened
+          // contacts removed (the reference's internal-resistance files)
+          nf_node_info nd;
+          int rc3 = nf_node_network(h2, &nd);
+          if (rc3 == NF_OK) rc3 = nf_write_node_coo(h2, "");
+          if (rc3 != NF_OK) {
+            status = die(h2, "node network", rc3);
+          } else {
+            std::printf("node network: %u nodes (%u contact and electrode points), %u segments, "
+                        "%u contacts kept, %u screened\n",
+                        nd.n_nodes, nd.n_points, nd.n_segments, nd.n_edges, nd.n_screened);
+          }
+        }
         // DC solve between the electrodes (kirchhoff.py's role, real conductances)
         nf_conductance_info ci;
         const int rc2 = nf_conductance(h2, 1e-10, 20000, &ci, nullptr);

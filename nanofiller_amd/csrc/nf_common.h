@@ -259,9 +259,11 @@ struct Rod {
 // --------------------------------------------------------------------------
 constexpr double REACH_PAD = 1.000001;
 
-NF_HD double pair_d2_f64(const float4 P1, const float4 D1, const float4 P2, const float4 D2,
-                         const double sx, const double sy, const double sz, const double R,
-                         bool* within) {
+// As pair_d2_f64, also returning the closest points' axial parameters in
+// units of the half lengths (s on rod 1, t on rod 2, both in [-1, 1]).
+NF_HD double pair_closest_f64(const float4 P1, const float4 D1, const float4 P2, const float4 D2,
+                              const double sx, const double sy, const double sz, const double R,
+                              bool* within, double* s_out, double* t_out) {
   const double wx = ((double)P1.x - (double)P2.x) - sx;
   const double wy = ((double)P1.y - (double)P2.y) - sy;
   const double wz = ((double)P1.z - (double)P2.z) - sz;
@@ -290,6 +292,47 @@ NF_HD double pair_d2_f64(const float4 P1, const float4 D1, const float4 P2, cons
   } else if (t > 1.0) {
     t = 1.0;
     s = (b - c) / a;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  }
+  const double vx = (wx + s * h1x) - t * h2x;
+  const double vy = (wy + s * h1y) - t * h2y;
+  const double vz = (wz + s * h1z) - t * h2z;
+  if (s_out) *s_out = s;
+  if (t_out) *t_out = t;
+  return (vx * vx + vy * vy) + vz * vz;
+}
+
+NF_HD double pair_d2_f64(const float4 P1, const float4 D1, const float4 P2, const float4 D2,
+                         const double sx, const double sy, const double sz, const double R,
+                         bool* within) {
+  return pair_closest_f64(P1, D1, P2, D2, sx, sy, sz, R, within, nullptr, nullptr);
+}
+
+// Squared distance between two FP64 segments given by centre and half
+// vector, with the clamped closest-point solve of pair_closest_f64; a
+// zero-length first segment is a point.
+NF_HD double seg_seg_d2(double c1x, double c1y, double c1z, double h1x, double h1y, double h1z,
+                        double c2x, double c2y, double c2z, double h2x, double h2y, double h2z) {
+  const double wx = c1x - c2x, wy = c1y - c2y, wz = c1z - c2z;
+  const double a = (h1x * h1x + h1y * h1y) + h1z * h1z;
+  const double e = (h2x * h2x + h2y * h2y) + h2z * h2z;
+  const double b = (h1x * h2x + h1y * h2y) + h1z * h2z;
+  const double c = (h1x * wx + h1y * wy) + h1z * wz;
+  const double f = (h2x * wx + h2y * wy) + h2z * wz;
+  const double denom = a * e - b * b;
+  double s = 0.0;
+  if (denom > 0.0) {
+    s = (b * f - c * e) / denom;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  }
+  double t = (b * s + f) / e;
+  if (t < -1.0) {
+    t = -1.0;
+    s = a > 0.0 ? (-b - c) / a : 0.0;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  } else if (t > 1.0) {
+    t = 1.0;
+    s = a > 0.0 ? (b - c) / a : 0.0;
     s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
   }
   const double vx = (wx + s * h1x) - t * h2x;

@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "nf_internal.h"
@@ -24,7 +26,11 @@ struct nf_handle {
   int device = 0;
   std::vector<uint32_t> n_rods;  // current rod count per instance
   bool generated = false;
-  std::vector<void*> allocs;
+  // device memory: live blocks (pointer -> size-class bytes) and a pool of
+  // freed blocks by size class, reused by later calls of the same handle
+  std::unordered_map<void*, size_t> allocs;
+  std::multimap<size_t, void*> pool;
+  size_t pool_bytes = 0;
   unsigned long long cb_stats[4] = {0, 0, 0, 0};  // contact pair tests, rechecks, max dd bits
   float cb_ms[4] = {0, 0, 0, 0};                  // contact kernel times: bin, pair pass, CSR build, row sort
   uint64_t contact_hint = 0;                      // contacts found by the last call (append capacity)

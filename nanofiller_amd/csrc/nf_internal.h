@@ -155,6 +155,16 @@ struct ContactBuffers {
   double* nval = nullptr;
   uint64_t nnz = 0;
   bool net_valid = false;
+  // node-level segment network (nf_node_network)
+  uint32_t *nd_pflag = nullptr, *nd_edge_a = nullptr, *nd_edge_b = nullptr;
+  uint32_t *nd_esrc = nullptr, *nd_esnk = nullptr, *nd_src_node = nullptr, *nd_snk_node = nullptr;
+  double *nd_egs = nullptr, *nd_egk = nullptr;
+  uint32_t* nd_node_rod = nullptr;
+  double* nd_node_t = nullptr;
+  uint32_t *nd_seg_a = nullptr, *nd_seg_b = nullptr;
+  double *nd_seg_L = nullptr, *nd_seg_A = nullptr;
+  uint64_t nd_nodes = 0, nd_segments = 0, nd_points = 0;
+  bool nd_valid = false;
 };
 
 // Arguments of the cell-pair engine.
@@ -288,6 +298,52 @@ void launch_cs_potential(uint32_t n, const uint32_t* labels, const uint32_t* fla
 void launch_cg_init(const CgArgs& a, hipStream_t s);
 void launch_cg_iteration(const CgArgs& a, hipStream_t s);
 void launch_cg_current(const CgArgs& a, hipStream_t s);
+
+// ---- node-level segment network (nf_nodes.hip) ----
+enum NodeFlag { NODE_KEPT = 0, NODE_SCREENED = 1, NODE_CROSSING = 2 };
+enum PointKind { PT_CONTACT_I = 0, PT_CONTACT_J = 1, PT_SOURCE = 2, PT_SINK = 3 };
+struct NodeArgs {
+  uint32_t n;
+  // contacts: CSR over the lower id, J > row, image applies to J
+  const uint32_t* rowptr;
+  const uint32_t* J;
+  const int32_t* img;
+  const float4* P;  // rods by id
+  const float4* D;
+  int axis;
+  double box[3];
+  double L, gcut, hl_max, a_max;
+  // contact cell grid (rods sorted by cell)
+  const float4* cP;
+  const float4* cD;
+  const uint32_t* cid;
+  const uint32_t* cstart;
+  const uint32_t* ccount;
+  int32_t ncell[3];
+  double cell_scale[3];
+  // per contact
+  uint32_t* pflag;
+  double *ptI, *ptJ;
+  uint32_t *edge_a, *edge_b;
+  // per rod
+  uint32_t *npt, *pstart, *pfill, *ngrp, *nseg, *gstart, *sstart;
+  uint32_t *esrc, *esnk, *src_node, *snk_node;
+  double *egs, *egk;
+  // points
+  double* pt_t;
+  unsigned long long* pt_key;  // kind << 32 | contact or rod index
+  uint32_t* pt_grp;
+  // nodes and segments
+  uint32_t* node_rod;
+  double* node_t;
+  uint32_t *seg_a, *seg_b;
+  double *seg_L, *seg_A;
+};
+void launch_node_points(const NodeArgs& a, hipStream_t s);
+void launch_node_lists(const NodeArgs& a, hipStream_t s);
+void launch_node_nodes(const NodeArgs& a, hipStream_t s);
+// out[4] (zeroed by the caller): kept contacts, screened contacts, source and sink points.
+void launch_node_count(const NodeArgs& a, uint32_t nc, unsigned long long* out, hipStream_t s);
 
 // ---- network extraction (nf_net.hip) ----
 enum NetSummary {

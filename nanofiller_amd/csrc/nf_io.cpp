@@ -398,3 +398,72 @@ extern "C" int nf_write_coo(nf_handle* h, const char* prefix) {
   }
   return NF_OK;
 }
+
+// Node network files: save_coo's node-table and two-column overloads
+// (src/conductivity.cu:881-930): "#\tI\tJ\tData" / "#\tI\tJ\tData1\tData2"
+// headers, "\tI\tJ\t..." rows.  Electrode rows carry the electrode id
+// (4294967295 source, 4294967294 sink) in the first column, as the
+// reference's SRC_ELEC / SNK_ELEC node ids.
+extern "C" int nf_write_node_coo(nf_handle* h, const char* prefix) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  const ContactBuffers& c = h->cb;
+  if (!c.nd_valid) return fail(h, NF_E_STATE, "nf_node_network must run before nf_write_node_coo");
+  const std::string pre = prefix ? prefix : "";
+  const uint64_t nn = c.nd_nodes, ns = c.nd_segments;
+  std::vector<uint32_t> rod(nn), sa(ns), sb(ns);
+  std::vector<double> t(nn), L(ns), A(ns);
+  int rc = nf_get_nodes(h, rod.data(), t.data(), nn);
+  if (rc == NF_OK) rc = nf_get_segments(h, sa.data(), sb.data(), L.data(), A.data(), ns);
+  uint64_t ne = 0, nel = 0;
+  if (rc == NF_OK) rc = nf_get_node_edges(h, nullptr, nullptr, nullptr, 0, &ne);
+  if (rc == NF_OK) rc = nf_get_node_electrodes(h, nullptr, nullptr, nullptr, 0, &nel);
+  if (rc != NF_OK) return rc;
+  std::vector<uint32_t> ea(ne), eb(ne), ek(ne), en(nel);
+  std::vector<double> eg(nel);
+  std::vector<int32_t> es(nel);
+  rc = nf_get_node_edges(h, ea.data(), eb.data(), ek.data(), ne, &ne);
+  if (rc == NF_OK) rc = nf_get_node_electrodes(h, en.data(), eg.data(), es.data(), nel, &nel);
+  if (rc != NF_OK) return rc;
+  const uint32_t n = c.n_rods;
+  const uint64_t C = c.n_contacts;
+  std::vector<uint32_t> row(n + 1), col(C);
+  std::vector<float> gap(C);
+  rc = nf_get_contacts(h, row.data(), col.data(), gap.data(), nullptr, C);
+  if (rc != NF_OK) return rc;
+  std::vector<uint32_t> rowof(C);
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t k = row[i]; k < row[i + 1]; ++k) rowof[k] = i;
+  std::vector<float> rods((size_t)n * 8);
+  uint32_t got = 0;
+  rc = nf_get_rods(h, 0, rods.data(), n, &got);
+  if (rc != NF_OK) return rc;
+  const double rlim = c.gap_cut;
+  std::ofstream fn(pre + "Nodes.tbl.dat"), fsg(pre + "segment.LA.coo"), fi(pre + "internal.DS.coo"),
+      fk(pre + "sink.DS.coo"), fs(pre + "source.DS.coo");
+  if (!fn.is_open() || !fsg.is_open() || !fi.is_open() || !fk.is_open() || !fs.is_open())
+    return fail(h, NF_E_IO, "cannot write node network files");
+  fn << "#\tI\tJ\tData\n";
+  for (uint64_t g = 0; g < nn; ++g) fn << "\t" << g << "\t" << rod[g] << "\t" << (float)t[g] << "\n";
+  fsg << "#\tI\tJ\tData1\tData2\n";
+  for (uint64_t k = 0; k < ns; ++k)
+    fsg << "\t" << sa[k] << "\t" << sb[k] << "\t" << (float)L[k] << "\t" << (float)A[k] << "\n";
+  fi << "#\tI\tJ\tData1\tData2\n";
+  for (uint64_t e = 0; e < ne; ++e) {
+    const uint32_t k = ek[e];
+    const double g = gap[k] > 0.0f ? (double)gap[k] : 0.0;
+    fi << "\t" << ea[e] << "\t" << eb[e] << "\t" << (float)g << "\t"
+       << (float)point_area(g, rods[8 * (size_t)rowof[k] + 7], rods[8 * (size_t)col[k] + 7], rlim) << "\n";
+  }
+  fk << "#\tI\tJ\tData1\tData2\n";
+  fs << "#\tI\tJ\tData1\tData2\n";
+  for (uint64_t e = 0; e < nel; ++e) {
+    const double g = eg[e] > 0.0 ? eg[e] : 0.0;
+    const double a = rods[8 * (size_t)rod[en[e]] + 7];
+    std::ofstream& f = es[e] == 0 ? fs : fk;
+    f << "\t" << (es[e] == 0 ? 4294967295u : 4294967294u) << "\t" << en[e] << "\t" << (float)g << "\t"
+      << (float)point_area(g, a, a, rlim) << "\n";
+  }
+  if (!fn.good() || !fsg.good() || !fi.good() || !fk.good() || !fs.good())
+    return fail(h, NF_E_IO, "write failed: node network files");
+  return NF_OK;
+}

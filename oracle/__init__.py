@@ -57,6 +57,9 @@ def lib() -> C.CDLL:
             ("orc_vmf_basis", None, [C.c_void_p, C.c_void_p, C.c_void_p]),
             ("orc_candidate", C.c_int, [P(Params), C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
             ("orc_mesh_density", C.c_double, [P(Params), C.c_double, C.c_double, C.c_double]),
+            ("orc_node_points", None, [P(Params), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_int]),
             ("orc_pair_d2", C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, P(C.c_int)]),
             ("orc_pair_batch", None, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                                       C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -287,3 +290,101 @@ def conductance(net: dict):
         isrc = float(np.dot(b, 1.0 - x))
         isnk = float(np.dot(bk, x))
     return {"conductance": isrc, "current_sink": isnk, "unknowns": len(act), "potentials": v}
+
+
+def node_network(p, rods, I, J, image, threads: int = 8):
+    """Node-level segment network of a contact list (the internal-resistance
+    branch of collect_cunduct_data, src/conductivity.cu:213-410): contact
+    points and screening from orc_node_points, electrode points at the rod
+    ends facing the planes (the couplings of the network matrix), per-rod
+    points ordered by (t, kind, index), chains of points within one radius
+    merged (remove_redundant_nodes / reduce_node_tbl, src/adjacency_graph.cu:
+    57-155; node t = mean of its points), segments L = dt / a between
+    consecutive nodes (get_segment_length / recalculate_segments)."""
+    r = np.ascontiguousarray(rods, np.float32)
+    n = r.shape[0]
+    I = np.ascontiguousarray(I, np.uint32)
+    J = np.ascontiguousarray(J, np.uint32)
+    im = np.ascontiguousarray(image, np.int32)
+    nc = len(I)
+    tI = np.zeros(max(nc, 1))
+    tJ = np.zeros(max(nc, 1))
+    flag = np.zeros(max(nc, 1), np.uint32)
+    lib().orc_node_points(C.byref(p), r.ctypes.data, n, I.ctypes.data, J.ctypes.data, im.ctypes.data,
+                          nc, tI.ctypes.data, tJ.ctypes.data, flag.ctypes.data, threads)
+    ax = p.elec_axis
+    L = float(p.box[ax])
+    gcut = float(p.contact_gap)
+    pts = [[] for _ in range(n)]  # (t, kind, index)
+    for k in range(nc):
+        if flag[k] != 0:
+            continue
+        pts[int(I[k])].append((float(tI[k]), 0, k))
+        pts[int(J[k])].append((float(tJ[k]), 1, k))
+    esrc, esnk = np.zeros(n, bool), np.zeros(n, bool)
+    egs, egk = np.zeros(n), np.zeros(n)
+    for i in range(n):
+        hl = float(r[i, 6]) * 0.5
+        ca = r[i, 3 + ax]
+        ext = abs(hl * float(ca))
+        pos, rad = float(r[i, ax]), float(r[i, 7])
+        gs = (pos - ext) - rad
+        gk = (L - (pos + ext)) - rad
+        egs[i], egk[i] = gs, gk
+        if gs <= gcut:
+            esrc[i] = True
+            pts[i].append((-hl if ca >= 0 else hl, 2, i))
+        if gk <= gcut:
+            esnk[i] = True
+            pts[i].append((hl if ca >= 0 else -hl, 3, i))
+    node_rod, node_t, sa, sb, sL = [], [], [], [], []
+    edge_a = np.zeros(nc, np.uint32)
+    edge_b = np.zeros(nc, np.uint32)
+    src_node = np.zeros(n, np.uint32)
+    snk_node = np.zeros(n, np.uint32)
+    for i in range(n):
+        ps = sorted(pts[i])
+        if not ps:
+            continue
+        rad = float(r[i, 7])
+        groups = [[ps[0]]]
+        for q in range(1, len(ps)):
+            if ps[q][0] - ps[q - 1][0] <= rad:
+                groups[-1].append(ps[q])
+            else:
+                groups.append([ps[q]])
+        prev = None
+        for g in groups:
+            tot = 0.0
+            for x in g:
+                tot += x[0]
+            tm = tot / float(len(g))
+            node = len(node_rod)
+            node_rod.append(i)
+            node_t.append(tm)
+            if prev is not None:
+                sa.append(node - 1)
+                sb.append(node)
+                sL.append((tm - prev) / rad)
+            prev = tm
+            for (_, kind, idx) in g:
+                if kind == 0:
+                    edge_a[idx] = node
+                elif kind == 1:
+                    edge_b[idx] = node
+                elif kind == 2:
+                    src_node[idx] = node
+                else:
+                    snk_node[idx] = node
+    kept = np.flatnonzero(flag[:nc] == 0)
+    elec = [(int(src_node[i]), egs[i], 0) for i in range(n) if esrc[i]] + \
+           [(int(snk_node[i]), egk[i], 1) for i in range(n) if esnk[i]]
+    return {
+        "flag": flag[:nc].copy(), "tI": tI[:nc].copy(), "tJ": tJ[:nc].copy(),
+        "node_rod": np.array(node_rod, np.uint32), "node_t": np.array(node_t),
+        "seg_a": np.array(sa, np.uint32), "seg_b": np.array(sb, np.uint32), "seg_L": np.array(sL),
+        "edge_a": edge_a[kept], "edge_b": edge_b[kept], "edge_contact": kept.astype(np.uint32),
+        "elec_node": np.array([e[0] for e in elec], np.uint32),
+        "elec_gap": np.array([e[1] for e in elec]), "elec_side": np.array([e[2] for e in elec], np.int32),
+        "n_points": sum(len(x) for x in pts),
+    }

@@ -1305,3 +1305,149 @@ uint64_t orc_centre_distance(const nf_params* p, const float* rods, uint32_t n, 
   *sum2_q = s2q;
   return cnt;
 }
+
+/* ===================================================================== */
+/* Node-level segment network: contact points and screening              */
+/* ===================================================================== */
+/* FP64 squared distance between two segments (centre, half vector), the
+ * clamped closest-point solve of orc_pair_d2; a zero-length first segment
+ * is a point. */
+static double orc_seg_seg_d2(const double c1[3], const double h1[3], const double c2[3],
+                             const double h2[3]) {
+  double wx = c1[0] - c2[0], wy = c1[1] - c2[1], wz = c1[2] - c2[2];
+  double a = (h1[0] * h1[0] + h1[1] * h1[1]) + h1[2] * h1[2];
+  double e = (h2[0] * h2[0] + h2[1] * h2[1]) + h2[2] * h2[2];
+  double b = (h1[0] * h2[0] + h1[1] * h2[1]) + h1[2] * h2[2];
+  double c = (h1[0] * wx + h1[1] * wy) + h1[2] * wz;
+  double f = (h2[0] * wx + h2[1] * wy) + h2[2] * wz;
+  double denom = a * e - b * b;
+  double s = 0.0;
+  if (denom > 0.0) {
+    s = (b * f - c * e) / denom;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  }
+  double t = (b * s + f) / e;
+  if (t < -1.0) {
+    t = -1.0;
+    s = a > 0.0 ? (-b - c) / a : 0.0;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  } else if (t > 1.0) {
+    t = 1.0;
+    s = a > 0.0 ? (b - c) / a : 0.0;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  }
+  double vx = (wx + s * h1[0]) - t * h2[0];
+  double vy = (wy + s * h1[1]) - t * h2[1];
+  double vz = (wz + s * h1[2]) - t * h2[2];
+  return (vx * vx + vy * vy) + vz * vz;
+}
+
+/* Closest-point parameters of one contact (I, J, image on J) in length
+ * units, as the closest points of get_contact_arrays' (I, J, tI, tJ)
+ * records (src/get_contact_arrays.cu:71-140). */
+static void orc_contact_params(const nf_params* p, const float* ri, const float* rj, int32_t code,
+                               double* tI, double* tJ) {
+  double sh[3];
+  for (int d = 0; d < 3; ++d) sh[d] = (double)(int8_t)((code >> (8 * d)) & 0xFF) * p->box[d];
+  double wx = ((double)ri[0] - (double)rj[0]) - sh[0];
+  double wy = ((double)ri[1] - (double)rj[1]) - sh[1];
+  double wz = ((double)ri[2] - (double)rj[2]) - sh[2];
+  double hli = (double)ri[6] * 0.5, hlj = (double)rj[6] * 0.5;
+  double h1x = hli * (double)ri[3], h1y = hli * (double)ri[4], h1z = hli * (double)ri[5];
+  double h2x = hlj * (double)rj[3], h2y = hlj * (double)rj[4], h2z = hlj * (double)rj[5];
+  double a = (h1x * h1x + h1y * h1y) + h1z * h1z;
+  double e = (h2x * h2x + h2y * h2y) + h2z * h2z;
+  double b = (h1x * h2x + h1y * h2y) + h1z * h2z;
+  double c = (h1x * wx + h1y * wy) + h1z * wz;
+  double f = (h2x * wx + h2y * wy) + h2z * wz;
+  double denom = a * e - b * b;
+  double s = 0.0;
+  if (denom > 0.0) {
+    s = (b * f - c * e) / denom;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  }
+  double t = (b * s + f) / e;
+  if (t < -1.0) {
+    t = -1.0;
+    s = (-b - c) / a;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  } else if (t > 1.0) {
+    t = 1.0;
+    s = (b - c) / a;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+  }
+  *tI = s * hli;
+  *tJ = t * hlj;
+}
+
+/* Contact points and screening of every contact (flag 0 kept, 1 screened,
+ * 2 crossing the electrode axis): the segment between the contact points,
+ * with the radius min(a_I, a_J) of generate_virt_inclusions
+ * (inc/conductKernels.h:289-361), is screened by any periodic image of any
+ * third rod K whose axis comes closer than that radius + a_K
+ * (check_screened_contacts, src/conductivity.cu:430-528).  Brute force over
+ * all rods and the images within reach. */
+void orc_node_points(const nf_params* p, const float* rods, uint32_t n, const uint32_t* I,
+                     const uint32_t* J, const int32_t* img, uint64_t nc, double* tI, double* tJ,
+                     uint32_t* flag, int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  const int ax = p->elec_axis;
+  double hlmax = 0.0, amax = 0.0;
+  for (uint32_t k = 0; k < n; ++k) {
+    if ((double)rods[8 * (size_t)k + 6] * 0.5 > hlmax) hlmax = (double)rods[8 * (size_t)k + 6] * 0.5;
+    if ((double)rods[8 * (size_t)k + 7] > amax) amax = (double)rods[8 * (size_t)k + 7];
+  }
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t q = 0; q < (int64_t)nc; ++q) {
+    const float* ri = rods + 8 * (size_t)I[q];
+    const float* rj = rods + 8 * (size_t)J[q];
+    const int32_t code = img[q];
+    if ((int)(int8_t)((code >> (8 * ax)) & 0xFF) != 0) {
+      flag[q] = 2;
+      tI[q] = tJ[q] = 0.0;
+      continue;
+    }
+    double ti, tj;
+    orc_contact_params(p, ri, rj, code, &ti, &tj);
+    tI[q] = ti;
+    tJ[q] = tj;
+    double u[3];
+    for (int d = 0; d < 3; ++d) u[d] = (double)(int8_t)((code >> (8 * d)) & 0xFF) * p->box[d];
+    double A[3], B[3];
+    for (int d = 0; d < 3; ++d) {
+      A[d] = (double)ri[d] + ti * (double)ri[3 + d];
+      B[d] = ((double)rj[d] + u[d]) + tj * (double)rj[3 + d];
+    }
+    double av = (double)ri[7] < (double)rj[7] ? (double)ri[7] : (double)rj[7];
+    double m[3] = {0.5 * (A[0] + B[0]), 0.5 * (A[1] + B[1]), 0.5 * (A[2] + B[2])};
+    double hv[3] = {0.5 * (B[0] - A[0]), 0.5 * (B[1] - A[1]), 0.5 * (B[2] - A[2])};
+    double hs = sqrt((hv[0] * hv[0] + hv[1] * hv[1]) + hv[2] * hv[2]);
+    double reach = hlmax + hs + av + amax;
+    int K[3];
+    for (int d = 0; d < 3; ++d) K[d] = (int)floor(reach / p->box[d]) + 2;
+    int scr = 0;
+    for (uint32_t k = 0; k < n && !scr; ++k) {
+      if (k == I[q] || k == J[q]) continue;
+      const float* rk = rods + 8 * (size_t)k;
+      double hl = (double)rk[6] * 0.5;
+      double h2[3] = {hl * (double)rk[3], hl * (double)rk[4], hl * (double)rk[5]};
+      double R = av + (double)rk[7];
+      /* images of K around the midpoint: start from the nearest one */
+      int n0[3];
+      for (int d = 0; d < 3; ++d) n0[d] = (int)floor((m[d] - (double)rk[d]) / p->box[d] + 0.5);
+      for (int iz = n0[2] - K[2]; iz <= n0[2] + K[2] && !scr; ++iz)
+        for (int iy = n0[1] - K[1]; iy <= n0[1] + K[1] && !scr; ++iy)
+          for (int ix = n0[0] - K[0]; ix <= n0[0] + K[0] && !scr; ++ix) {
+            double c2[3] = {(double)rk[0] + (double)ix * p->box[0], (double)rk[1] + (double)iy * p->box[1],
+                            (double)rk[2] + (double)iz * p->box[2]};
+            double d2 = orc_seg_seg_d2(m, hv, c2, h2);
+            if (d2 < R * R) scr = 1;
+          }
+    }
+    flag[q] = scr ? 1u : 0u;
+  }
+}

@@ -37,6 +37,9 @@ double orc_log(double x);
 /* Agglomerate centres (3 doubles each) of instance p. */
 void orc_agg_centres(const nf_params* p, double* out);
 double orc_mesh_density(const nf_params* p, double x, double y, double z);
+void orc_node_points(const nf_params* p, const float* rods, uint32_t n, const uint32_t* I,
+                     const uint32_t* J, const int32_t* img, uint64_t nc, double* tI, double* tJ,
+                     uint32_t* flag, int threads);
 /* Orthonormal basis e1, e2 perpendicular to the (unit) vMF axis. */
 void orc_vmf_basis(const double mu[3], double e1[3], double e2[3]);
 

@@ -40,7 +40,8 @@ def test_cli_end_to_end(tmp_path):
     assert r.returncode == 0, r.stderr
     assert "2000 inclusions generated!" in r.stdout
     for f in ("output.vtk", "rho.dat", "gamma.dat", "internal.srt.coo", "source.elec.coo",
-              "sink.elec.coo"):
+              "sink.elec.coo", "Nodes.tbl.dat", "segment.LA.coo", "internal.DS.coo", "sink.DS.coo",
+              "source.DS.coo"):
         assert (tmp_path / f).exists(), f
     # same packing as the library call with the same parameters
     p = nf.normalize(nf.load_params(str(tmp_path / "params.txt")))
@@ -60,3 +61,4 @@ def test_cli_end_to_end(tmp_path):
     np.testing.assert_array_equal(rv, h["phitheta"].ravel())
     assert f"contacts (gap <= 0.01 um): {c.n};" in r.stdout
     assert "effective conductance" in r.stdout
+    assert "node network:" in r.stdout
