@@ -53,7 +53,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   // longest rods and per-cell length bounds prune the rest
   double edge = (p.len_law == NF_LEN_LOGNORMAL ? p.len : (double)hlmax * 2.0) * REACH_PAD + Rmax;
   if (edge > reach) edge = reach;
-  edge = default_cell_edge(edge, (double)n / (p.box[0] * p.box[1] * p.box[2]));
+  edge = default_cell_edge(edge, (double)n / (p.box[0] * p.box[1] * p.box[2]), 1.0);
   uint64_t ncells = 1;
   double scale[3];
   for (int a = 0; a < 3; ++a) {
@@ -386,7 +386,7 @@ extern "C" int nf_centre_distance(nf_handle* h, int32_t bins, double width, int6
   const float4* P = h->gb.bP + h->desc[0].rod_off;
   const float4* D = h->gb.bD + h->desc[0].rod_off;
   // grid: edge about rmax (half stencil of +-1 cells), finer when crowded
-  const double edge = default_cell_edge(rmax, (double)n / (p.box[0] * p.box[1] * p.box[2]));
+  const double edge = default_cell_edge(rmax, (double)n / (p.box[0] * p.box[1] * p.box[2]), 1.0);
   int32_t nc[3];
   double scale[3];
   uint64_t ncells = 1;

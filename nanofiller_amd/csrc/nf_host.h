@@ -51,7 +51,7 @@ int dalloc_bytes(nf_handle* h, void** p, size_t bytes);
 void dfree(nf_handle* h, void* p);
 void release_contacts(nf_handle* h);
 // Default cell edge for a typical pair reach and a mean rod density.
-double default_cell_edge(double reach_typ, double density);
+double default_cell_edge(double reach_typ, double density, double factor);
 std::vector<int4> build_stencil(const int32_t nc[3], const double box[3], double reach, bool half);
 
 template <class T>

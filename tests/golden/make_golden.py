@@ -25,17 +25,17 @@ import oracle  # noqa: E402
 
 CASES = {
     "iso_fp64": dict(box=(3, 3, 3), n_rods=400, len=1.0, radius=0.005, seed=7, contact_gap=0.05,
-                     tunnel_lambda=0.01, elec_axis=0, precision=nf.NF_PREC_FP64),
+                     tunnel_lambda=0.01, elec_axis=0, precision=nf.NF_PREC_FP64, max_batch=413),
     "lognormal_vmf_agglom_fp32": dict(box=(4, 4, 4), n_rods=1500, len=1.0,
                                       len_law=nf.NF_LEN_LOGNORMAL, len_sigma=0.3, len_min=0.3,
                                       len_max=3.0, radius=0.005, seed=2, den_law=nf.NF_DEN_AGGLOM,
                                       n_agg=4, agg_sigma=1.0, agg_peak=3.0, ort_law=nf.NF_ORT_VMF,
                                       kappa=5.0, axis=(0, 0, 1), contact_gap=0.02,
                                       tunnel_lambda=0.005, elec_axis=2,
-                                      precision=nf.NF_PREC_FP32_RECHECK),
+                                      precision=nf.NF_PREC_FP32_RECHECK, max_batch=979),
     "tiny_box_images": dict(box=(1.5, 1.5, 1.5), n_rods=40, len=2.0, radius=0.01, seed=11,
                             contact_gap=0.05, tunnel_lambda=0.01, elec_axis=1,
-                            precision=nf.NF_PREC_FP64),
+                            precision=nf.NF_PREC_FP64, max_batch=256),
 }
 HIST = dict(bins=16, centre_axis=0, pol_axis=(0.0, 0.0, 1.0), phi_bins=8, theta_bins=8)
 
