@@ -230,10 +230,10 @@ void launch_ctl_init(GenBuffers& b, hipStream_t s);
 // ---- launchers (nf_pair.hip) ----
 void launch_pair(int mode, int prec, const PairArgs& a, hipStream_t s);
 // Persistent grid of a pair pass (waves): 16 per CU for the
-// reject-vs-accepted and contact passes, 6 per CU for the same-round pass,
+// reject-vs-accepted and contact passes, 12 per CU for the same-round pass,
 // whose light, sparse cells run faster with fewer waves competing for the
-// memory system (measured on MI355X: 130 us per launch at 6 per CU against
-// 210 us at 16 on the 500k-rod workload).
+// memory system (measured on MI355X, 500k-rod workload: 300 us per launch at
+// 12 per CU, 311 us at 16, 391 us at 6).
 int pair_grid(int mode);
 
 // ---- launchers (nf_post.hip) ----

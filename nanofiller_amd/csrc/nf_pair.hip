@@ -775,7 +775,7 @@ int pair_grid(int mode) {
       cus = 256;
     g_cus = cus;
   }
-  return g_cus * (mode == PM_GEN_NEW ? 6 : 16);
+  return g_cus * (mode == PM_GEN_NEW ? 12 : 16);
 }
 
 void launch_pair(int mode, int prec, const PairArgs& a, hipStream_t s) {
