@@ -412,12 +412,13 @@ __device__ __forceinline__ void next_cell(const PairArgs& a, const CellGeom& g, 
 // distance test drops them.  Keeps every pair the full test could count or
 // cut (see process_cell).
 template <int MODE>
-__device__ __forceinline__ bool prefilter(const float4 P1, float rr2, uint32_t X1, uint32_t qprobe,
+__device__ __forceinline__ bool prefilter(const float4 P1, float rp, uint32_t X1, uint32_t qprobe,
                                           const float4 Q2, uint32_t xj, uint32_t qa, uint32_t qb) {
   constexpr bool CNT = (MODE == PM_CNT);
   const float wx = P1.x - Q2.x, wy = P1.y - Q2.y, wz = P1.z - Q2.z;
   const float w2 = fmaf(wz, wz, fmaf(wy, wy, wx * wx));
-  bool f = w2 <= rr2;
+  const float rr = rp + Q2.w;  // the pair's own reach bound
+  bool f = w2 <= rr * rr;
   // same-round pass: the slot must have survived the reject-vs-accepted
   // pass (dead slots carry id 0xFFFFFFFF)
   if (MODE == PM_GEN_NEW) f = f && xj != 0xFFFFFFFFu;
@@ -453,12 +454,12 @@ __device__ __forceinline__ void push_pairs(const PairArgs& a, const CellGeom& g,
 
 // Prefilter of every (probe, slot) pair of one staged batch, looping over
 // the probes of the mask (read from their lanes' registers); TWO: the batch
-// fills the second slot of the lanes too.  rr2_l is the lane's own probe's
-// squared reach bound for this batch.
+// fills the second slot of the lanes too.  rp_l is the probe part of the
+// lane's own probe's reach bound; the slot part sits in Q.w.
 template <int MODE, int PREC, bool TWO>
 __device__ __forceinline__ void probe_loop(const PairArgs& a, const CellGeom& g, PairLds& L,
                                            unsigned long long pmask, const float4 P1, uint32_t X1,
-                                           float rr2_l, uint32_t qbase, const float4 Q0, uint32_t x0,
+                                           float rp_l, uint32_t qbase, const float4 Q0, uint32_t x0,
                                            uint32_t qa0, uint32_t qb0, const float4 Q1, uint32_t x1,
                                            uint32_t qa1, uint32_t qb1, uint32_t& qn, uint32_t& q2n,
                                            Acc& acc) {
@@ -469,10 +470,10 @@ __device__ __forceinline__ void probe_loop(const PairArgs& a, const CellGeom& g,
                                   __int_as_float(__builtin_amdgcn_readlane(__float_as_int(P1.z), pl)),
                                   0.0f);
     const uint32_t Xp = (uint32_t)__builtin_amdgcn_readlane((int)X1, pl);
-    const float rr2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rr2_l), pl));
+    const float rp = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rp_l), pl));
     const uint32_t qprobe = qbase + (uint32_t)pl;
-    const bool f0 = prefilter<MODE>(Pp, rr2, Xp, qprobe, Q0, x0, qa0, qb0);
-    const bool f1 = TWO && prefilter<MODE>(Pp, rr2, Xp, qprobe, Q1, x1, qa1, qb1);
+    const bool f0 = prefilter<MODE>(Pp, rp, Xp, qprobe, Q0, x0, qa0, qb0);
+    const bool f1 = TWO && prefilter<MODE>(Pp, rp, Xp, qprobe, Q1, x1, qa1, qb1);
     push_pairs<MODE, PREC, TWO>(a, g, L, f0, f1, (uint32_t)pl, qn, q2n, acc);
   }
 }
@@ -489,11 +490,15 @@ __device__ __forceinline__ void probe_loop(const PairArgs& a, const CellGeom& g,
 //
 // Prefilter: the full test first rejects pairs whose FP32 centre distance
 // w2 exceeds rr^2 by more than its error band, rr = (hl1 + hl2) * 1.000001
-// + R.  The prefilter keeps every pair with w2' <= rr'^2 * 1.00001, where
-// rr' = hl1 * 1.000001 + hl2 * 1.000001 + R_max + 16 ew >= rr + 16 ew and
-// w' = P1 - (P2 + shift) differs from the full test's ((P1 - P2) - shift)
-// by a few roundings of coordinates bounded by box + reach, i.e. by less
-// than 6 ew; so every pair the full test would count or cut is queued.
+// + R.  The prefilter keeps every pair with w2' <= rr''^2, where
+// rr'' = (hl1 * 1.000001 + R_max + 16 ew) * 1.000006 + (hl2 * 1.000001) * 1.000006
+// is rr' = hl1 * 1.000001 + hl2 * 1.000001 + R_max + 16 ew >= rr + 16 ew
+// scaled by 1.000006 less a few float roundings (below 1e-7 relative), so
+// rr''^2 >= rr'^2 * 1.00001; and w' = P1 - (P2 + shift) differs from the
+// full test's ((P1 - P2) - shift) by a few roundings of coordinates bounded
+// by box + reach, i.e. by less than 6 ew.  So every pair the full test would
+// count or cut is queued.  The probe's part of rr'' is computed per lane,
+// the slot's part is staged with the slot (Q.w).
 template <int MODE, int PREC>
 __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, PairLds& L,
                              Acc& acc) {
@@ -595,12 +600,12 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
       L.bM[lane + WAVE] = m1;
       const float3 s0 = shift_f32(g, m0), s1 = shift_f32(g, m1);
       const float far = 1e30f;  // unfilled slots: out of every reach
-      const float4 Q0 = v0 ? make_float4(p0.x + s0.x, p0.y + s0.y, p0.z + s0.z, 0.0f)
+      // w: the slot's part of the pair reach bound (see the prefilter note)
+      const float4 Q0 = v0 ? make_float4(p0.x + s0.x, p0.y + s0.y, p0.z + s0.z, (p0.w * 1.000001f) * 1.000006f)
                            : make_float4(far, far, far, 0.0f);
-      const float4 Q1 = v1 ? make_float4(p1.x + s1.x, p1.y + s1.y, p1.z + s1.z, 0.0f)
+      const float4 Q1 = v1 ? make_float4(p1.x + s1.x, p1.y + s1.y, p1.z + s1.z, (p1.w * 1.000001f) * 1.000006f)
                            : make_float4(far, far, far, 0.0f);
       // batch-wide half-length bound for the prefilter reach
-      const float bmax = unif(wave_max(fmaxf(v0 ? p0.w : 0.0f, v1 ? p1.w : 0.0f))) * 1.000001f;
       // contacts: keep slot iff qprobe < qa || qprobe > qb (home cell: only
       // later indices; elsewhere: anything but the probe itself)
       const uint32_t qa0 = src0, qa1 = src1;
@@ -608,14 +613,13 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
       const uint32_t qb1 = (m1 & 0x80000000u) ? 0xFFFFFFFFu : src1;
       __syncthreads();
       // ---- prefilter every (probe, slot) pair, queue the survivors
-      // each lane's own probe reach bound for this batch
-      const float rr_l = P1.w * 1.000001f + pad_r + bmax;
-      const float rr2_l = rr_l * rr_l * 1.00001f;
+      // each lane's own probe part of the pair reach bound
+      const float rp_l = (P1.w * 1.000001f + pad_r) * 1.000006f;
       if (fill > (uint32_t)WAVE)
-        probe_loop<MODE, PREC, true>(a, g, L, pmask, P1, X1, rr2_l, ps + pb, Q0, x0, qa0, qb0, Q1,
+        probe_loop<MODE, PREC, true>(a, g, L, pmask, P1, X1, rp_l, ps + pb, Q0, x0, qa0, qb0, Q1,
                                      x1, qa1, qb1, qn, q2n, acc);
       else
-        probe_loop<MODE, PREC, false>(a, g, L, pmask, P1, X1, rr2_l, ps + pb, Q0, x0, qa0, qb0, Q1,
+        probe_loop<MODE, PREC, false>(a, g, L, pmask, P1, X1, rp_l, ps + pb, Q0, x0, qa0, qb0, Q1,
                                       x1, qa1, qb1, qn, q2n, acc);
       // ---- drain both queues before the slots are restaged
       if (qn > 0) {
