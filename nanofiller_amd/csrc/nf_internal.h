@@ -71,7 +71,6 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
   return v;
 }
 
-// Queue counters: 8 per pair kernel launch (one per XCD chunk).
 // Work queues of the pair kernels: 64 per launch, each counter on its own
 // 128-byte line (QSTRIDE words apart) so that grabs on different queues do
 // not serialise on one line.  Queue q covers the cell range q of 64 equal
@@ -174,6 +173,8 @@ struct PairArgs {
   InstState* st;
   const uint32_t* cell_inst;
   uint32_t ncells_total;
+  uint32_t part_shift;  // work items per cell: 1 << part_shift (pair_part_shift)
+  uint32_t split_min;   // probes per part of a split cell, at least (pair_split_min)
   const int4* stencil;
   uint32_t st_off, st_n;  // used when desc == nullptr (contacts)
   // probes
@@ -235,6 +236,12 @@ void launch_pair(int mode, int prec, const PairArgs& a, hipStream_t s);
 // memory system (measured on MI355X, 500k-rod workload: 300 us per launch at
 // 12 per CU, 311 us at 16, 391 us at 6).
 int pair_grid(int mode);
+// Work items per cell of a pair pass, as a shift: each dense cell's stencil
+// walk is dealt over 1 << shift items, so that dense cells do not hold a
+// launch back.  launch_pair applies it to launches with fewer than 16 cells
+// per wave of the grid.
+int pair_part_shift(int mode);
+int pair_split_min(int mode);
 
 // ---- launchers (nf_post.hip) ----
 void launch_cell_count(const float4* P, uint32_t n, const int32_t ncell[3], const double scale[3],

@@ -144,9 +144,8 @@ __device__ __forceinline__ void flush(const PairArgs& a, const CellGeom& g, Acc&
   acc.bytes = 0;
 }
 
-// LDS of one wave (about 10 KB, so 16 waves fit a CU): the probe chunk,
-// the staged neighbour rods, the two pair queues and the list of relevant
-// neighbour cells of the current stencil block.
+// LDS of one wave (under 10 KB, so 16 waves fit a CU): the probe chunk, the
+// staged neighbour rods and the two pair queues.
 constexpr int SLOTS = 2 * WAVE;
 constexpr int QCAP = 3 * WAVE;
 struct PairLds {
@@ -159,7 +158,6 @@ struct PairLds {
   uint32_t bM[SLOTS];           // image code | same cell << 31
   uint32_t qE[QCAP];            // queued pairs: probe | slot << 8
   uint32_t q2[QCAP];            // pairs for the full test: probe | slot << 8 | fp64 << 16
-  uint32_t lS[WAVE], lN[WAVE], lM[WAVE];  // relevant neighbour cells: start, count, meta
 };
 
 __device__ __forceinline__ uint32_t lane_prefix(unsigned long long mask) {
@@ -324,17 +322,20 @@ __device__ __forceinline__ void screen_batch(const PairArgs& a, const CellGeom& 
 
 // Walk state over the sorted stencil of one probe chunk.  The stencil is
 // taken 64 offsets at a time: every lane looks up one neighbour cell (count,
-// start and longest rod in parallel, one round trip), and the relevant cells
-// are compacted in stencil order into the LDS list.  All fields are
+// start and longest rod in parallel, one round trip) and keeps it in
+// registers (cS, cN, cM); the relevant ones form a wave-uniform lane mask,
+// taken in stencil (lane) order with readlane.  All other fields are
 // wave-uniform.
 struct StencilCursor {
-  uint32_t sb;      // next stencil block
-  uint32_t li, ln;  // position in / length of the cell list
-  uint32_t start;   // current neighbour cell: first index, rod count, staged so far
+  uint32_t sb;              // next stencil block (first offset)
+  uint32_t step;            // offsets from one block of this work item to its next
+  unsigned long long mask;  // relevant cells of the block not yet taken
+  uint32_t cS, cN, cM;      // per lane: the lane's cell of the block
+  uint32_t start;           // current neighbour cell: first index, rod count, staged so far
   uint32_t n;
   uint32_t off;
-  uint32_t meta;    // image code | same cell << 31
-  bool last;        // no stencil block left after the current list
+  uint32_t meta;            // image code | same cell << 31
+  bool last;                // no stencil block left after the current one
   bool done;
 };
 
@@ -369,19 +370,14 @@ __device__ void refill_cells(const PairArgs& a, const CellGeom& g, PairLds& L, S
     const bool rel = valid && !beyond && n > 0u && md2 <= rr_c * rr_c;
     const unsigned long long mk = __ballot(rel);
     const bool same = so.x == 0 && so.y == 0 && so.z == 0;
-    if (rel) {
-      const uint32_t pos = lane_prefix(mk);
-      L.lS[pos] = st0;
-      L.lN[pos] = n;
-      L.lM[pos] = (uint32_t)pack_img(img[0], img[1], img[2]) | (same ? 0x80000000u : 0u);
-    }
-    const bool stop = (__ballot(beyond) != 0ull) || (c.sb + (uint32_t)WAVE >= g.st_n);
-    c.sb += (uint32_t)WAVE;
-    c.ln = (uint32_t)__popcll(mk);
-    c.li = 0u;
+    c.cS = st0;
+    c.cN = n;
+    c.cM = (uint32_t)pack_img(img[0], img[1], img[2]) | (same ? 0x80000000u : 0u);
+    const bool stop = (__ballot(beyond) != 0ull) || (c.sb + c.step >= g.st_n);
+    c.sb += c.step;
+    c.mask = mk;
     if (stop) c.last = true;
-    __syncthreads();
-    if (c.ln > 0u) return;
+    if (mk != 0ull) return;
   }
 }
 
@@ -392,18 +388,19 @@ template <int MODE>
 __device__ __forceinline__ void next_cell(const PairArgs& a, const CellGeom& g, PairLds& L,
                                           StencilCursor& c, int c0, int c1, int c2,
                                           float chunk_hl, float rr_chunk2) {
-  if (c.li >= c.ln) {
+  if (c.mask == 0ull) {
     if (!c.last) refill_cells<MODE>(a, g, L, c, c0, c1, c2, chunk_hl, rr_chunk2);
-    if (c.li >= c.ln) {
+    if (c.mask == 0ull) {
       c.done = true;
       return;
     }
   }
-  c.start = uni(L.lS[c.li]);
-  c.n = uni(L.lN[c.li]);
-  c.meta = uni(L.lM[c.li]);
+  const int i = (int)__builtin_ctzll(c.mask);
+  c.mask &= c.mask - 1ull;
+  c.start = (uint32_t)__builtin_amdgcn_readlane((int)c.cS, i);
+  c.n = (uint32_t)__builtin_amdgcn_readlane((int)c.cN, i);
+  c.meta = (uint32_t)__builtin_amdgcn_readlane((int)c.cM, i);
   c.off = 0u;
-  c.li += 1u;
 }
 
 // Prefilter of one (probe, slot) pair, the slot's data in registers:
@@ -499,14 +496,24 @@ __device__ __forceinline__ void probe_loop(const PairArgs& a, const CellGeom& g,
 // by box + reach, i.e. by less than 6 ew.  So every pair the full test would
 // count or cut is queued.  The probe's part of rr'' is computed per lane,
 // the slot's part is staged with the slot (Q.w).
+//
+// A cell is 1 << part_shift work items (parts).  A cell with more than
+// split_min probes uses np = min(parts, ceil(probes / split_min)) of them:
+// part k < np walks the stencil blocks k, k + np, k + 2 np, ... (64 offsets
+// each), so the parts partition the pairs of the cell and a dense cell is
+// worked on by several waves at once; parts k >= np return at once.
+// Per-probe effects that are not idempotent (the same-round pass's count of
+// probes already cut, the probe bytes) belong to part 0.
 template <int MODE, int PREC>
-__device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, PairLds& L,
-                             Acc& acc) {
+__device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, uint32_t part,
+                             PairLds& L, Acc& acc) {
   constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
   const int lane = threadIdx.x;
   uint32_t pc = uni(g.hCount[gc]);
   if (MODE == PM_GEN_OLD && g.inv) pc = min(pc, g.cap);  // accepted cells hold at most cap rods
   if (pc == 0) return;
+  const uint32_t np = min(1u << a.part_shift, (pc + a.split_min - 1u) / a.split_min);
+  if (part >= np) return;
   const uint32_t ps = uni(g.hStart[gc]);
   const uint32_t lc = gc - g.cell_base;
   const int c0 = (int)(lc % (uint32_t)g.nc[0]);
@@ -525,7 +532,7 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
     bool have = inrange;
     if (MODE == PM_GEN_NEW) {
       const bool alive = a.prAlive[pidx] != 0u;
-      const bool doa = inrange && !alive;  // already cut by an accepted rod
+      const bool doa = inrange && !alive && part == 0u;  // already cut by an accepted rod
       acc.kills_old += doa ? 1u : 0u;
       uint32_t* dst = doa ? &a.surv[X1] : &L.dummy[lane];
       *dst = 0u;
@@ -540,12 +547,13 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
     const unsigned long long pmask = __ballot(have);
     if (pmask == 0ull) continue;  // every probe of the chunk already cut: nothing to stage
     const float chunk_hl = unif(wave_max(have ? P1.w : 0.0f));
-    acc.bytes += (unsigned long long)G * 36ull;
+    if (part == 0u) acc.bytes += (unsigned long long)G * 36ull;
     const float rr_chunk = (chunk_hl + g.hl_max) * 1.000001f + g.R_max + CELL_MARGIN;
     const float rr_chunk2 = rr_chunk * rr_chunk;
     uint32_t qn = 0;   // wave-uniform queue lengths: prefiltered pairs,
     uint32_t q2n = 0;  // and pairs waiting for the full test
-    StencilCursor cur{0u, 0u, 0u, 0u, 0u, 0u, 0u, false, false};
+    StencilCursor cur{part * (uint32_t)WAVE, np * (uint32_t)WAVE, 0ull, 0u, 0u, 0u, 0u, 0u, 0u, 0u,
+                      false, false};
     __syncthreads();
 
     while (!cur.done) {
@@ -649,7 +657,7 @@ template <int MODE, int PREC>
 __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
   __shared__ PairLds L;
   constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
-  const uint32_t W = a.ncells_total;
+  const uint32_t W = a.ncells_total << a.part_shift;  // work items: cell parts
   const int xcd = blockIdx.x & 7;
   // rounds after every instance finished plan no candidates: nothing to do
   if (GEN && uni(a.ri->T) == 0u) return;
@@ -714,7 +722,8 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
       if (l0) old = atomicAdd(&a.qctr[q * QSTRIDE], 1u);
       const uint32_t item = uni(old);
       if (item >= qe - qb) break;
-      const uint32_t gc = qb + item;
+      const uint32_t gc = (qb + item) >> a.part_shift;
+      const uint32_t part = (qb + item) & ((1u << a.part_shift) - 1u);
       if (GEN) {
         const uint32_t inst = uni(a.cell_inst[gc]);
         if (inst != cur_inst) {
@@ -762,13 +771,29 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
           }
         }
       }
-      process_cell<MODE, PREC>(a, g, gc, L, acc);
+      process_cell<MODE, PREC>(a, g, gc, part, L, acc);
     }
   }
   if (!GEN || cur_inst != 0xFFFFFFFFu) flush<GEN>(a, g, acc, MODE);
 }
 
 static int g_cus = 0;
+
+// Two parts per cell for the generation passes, whose agglomerate-dense
+// cells otherwise end each launch alone; none for the contact pass.  Used
+// only when a launch has fewer than 16 cells per wave (launch_pair): with
+// more, the dense cells are spread over many items per wave anyway, and the
+// extra grabs of empty parts cost more than they save.  Measured on MI355X:
+// reject-vs-accepted pass 3.43 -> 3.05 ms per 500k-rod generation (12k
+// cells), same-round pass 1.92 -> 1.89 ms, four parts or splits of 16 or 48
+// probes no better; the 256-packing batch 301 -> 312 ms with parts; the
+// uniform 4M-rod contact pass 2.45 ms unsplit, 2.49 ms split.
+int pair_part_shift(int mode) { return mode == PM_CNT ? 0 : 1; }
+
+int pair_split_min(int mode) {
+  (void)mode;
+  return 32;
+}
 
 int pair_grid(int mode) {
   if (g_cus == 0) {
@@ -782,8 +807,11 @@ int pair_grid(int mode) {
   return g_cus * (mode == PM_GEN_NEW ? 12 : 16);
 }
 
-void launch_pair(int mode, int prec, const PairArgs& a, hipStream_t s) {
+void launch_pair(int mode, int prec, const PairArgs& args, hipStream_t s) {
   const dim3 grid(pair_grid(mode)), block(PAIR_THREADS);
+  PairArgs a = args;
+  a.part_shift = a.ncells_total < 16u * grid.x ? (uint32_t)pair_part_shift(mode) : 0u;
+  a.split_min = (uint32_t)pair_split_min(mode);
 #define NF_LAUNCH(M, P) hipLaunchKernelGGL((k_pair<M, P>), grid, block, 0, s, a)
   if (prec == NF_PREC_FP64) {
     switch (mode) {
