@@ -288,6 +288,7 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   a.ncells_total = (uint32_t)b.total_cells;
   a.stencil = b.stencil;
   a.diag = diag;
+  a.order = b.order;
   // stage 1: proposals vs accepted rods
   a.prP = b.pP;
   a.prD = b.pD;

@@ -114,6 +114,7 @@ struct GenBuffers {
   uint32_t* palive = nullptr;
   uint32_t* pcount = nullptr;  // total_cells
   uint32_t* pstart = nullptr;  // total_cells + 1
+  uint32_t* order = nullptr;   // total_cells: work order of a pair pass
   float* phl = nullptr;
   uint32_t* queues = nullptr;  // Q_N * QWORDS
   uint32_t* scan_tmp = nullptr;
@@ -175,6 +176,7 @@ struct PairArgs {
   uint32_t ncells_total;
   uint32_t part_shift;  // work items per cell: 1 << part_shift (pair_part_shift)
   uint32_t split_min;   // probes per part of a split cell, at least (pair_split_min)
+  uint32_t* order;      // ncells_total: work order of the cells (filled by launch_pair; nullptr: none)
   const int4* stencil;
   uint32_t st_off, st_n;  // used when desc == nullptr (contacts)
   // probes

@@ -657,7 +657,7 @@ template <int MODE, int PREC>
 __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
   __shared__ PairLds L;
   constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
-  const uint32_t W = a.ncells_total << a.part_shift;  // work items: cell parts
+  const uint32_t W = a.ncells_total;
   const int xcd = blockIdx.x & 7;
   // rounds after every instance finished plan no candidates: nothing to do
   if (GEN && uni(a.ri->T) == 0u) return;
@@ -704,8 +704,9 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
   // vector load of the 64 counters finds the first queue with work left.
   const int sub0 = (blockIdx.x >> 3) & 7;
   const int my_q = (((xcd + (threadIdx.x >> 3)) & 7) << 3) | ((sub0 + (int)threadIdx.x) & 7);
-  const uint32_t my_qb = (uint32_t)(((uint64_t)W * (uint32_t)my_q) / QUEUES_PER_KERNEL);
-  const uint32_t my_qe = (uint32_t)(((uint64_t)W * (uint32_t)(my_q + 1)) / QUEUES_PER_KERNEL);
+  // queue q: the parts of cells [W q / 64, W (q + 1) / 64) (k_order's ranges)
+  const uint32_t my_qb = (uint32_t)(((uint64_t)W * (uint32_t)my_q) / QUEUES_PER_KERNEL) << a.part_shift;
+  const uint32_t my_qe = (uint32_t)(((uint64_t)W * (uint32_t)(my_q + 1)) / QUEUES_PER_KERNEL) << a.part_shift;
   for (;;) {
     const uint32_t seen =
         __hip_atomic_load(&a.qctr[my_q * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -722,7 +723,8 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
       if (l0) old = atomicAdd(&a.qctr[q * QSTRIDE], 1u);
       const uint32_t item = uni(old);
       if (item >= qe - qb) break;
-      const uint32_t gc = (qb + item) >> a.part_shift;
+      const uint32_t ci = (qb + item) >> a.part_shift;
+      const uint32_t gc = a.order ? uni(a.order[ci]) : ci;
       const uint32_t part = (qb + item) & ((1u << a.part_shift) - 1u);
       if (GEN) {
         const uint32_t inst = uni(a.cell_inst[gc]);
@@ -777,6 +779,44 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
   if (!GEN || cur_inst != 0xFFFFFFFFu) flush<GEN>(a, g, acc, MODE);
 }
 
+// Work order of a pair pass: within each of the 64 queue ranges of cells,
+// the cells by decreasing rod count (probes plus neighbour rods, in buckets
+// of four), so that each queue starts with its densest cells and the launch
+// ends on light ones.  One workgroup per range; the order inside a bucket
+// is arbitrary, as every result of a pair pass is independent of the order
+// of its work.
+constexpr int ORDER_BUCKETS = 32;
+constexpr int ORDER_THREADS = 256;
+__global__ void __launch_bounds__(ORDER_THREADS) k_order(const uint32_t* __restrict__ ca,
+                                                         const uint32_t* __restrict__ cb,
+                                                         uint32_t ncells, uint32_t* __restrict__ order) {
+  __shared__ uint32_t slot[ORDER_BUCKETS];
+  const uint32_t r = blockIdx.x;
+  const uint32_t c0 = (uint32_t)(((uint64_t)ncells * r) / QUEUES_PER_KERNEL);
+  const uint32_t c1 = (uint32_t)(((uint64_t)ncells * (r + 1u)) / QUEUES_PER_KERNEL);
+  if (threadIdx.x < ORDER_BUCKETS) slot[threadIdx.x] = 0u;
+  __syncthreads();
+  // bucket 0: the densest cells
+  for (uint32_t c = c0 + threadIdx.x; c < c1; c += ORDER_THREADS) {
+    const uint32_t k = min((ca[c] + cb[c]) >> 2, (uint32_t)ORDER_BUCKETS - 1u);
+    atomicAdd(&slot[ORDER_BUCKETS - 1 - k], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t sum = c0;
+    for (int k = 0; k < ORDER_BUCKETS; ++k) {
+      const uint32_t t = slot[k];
+      slot[k] = sum;
+      sum += t;
+    }
+  }
+  __syncthreads();
+  for (uint32_t c = c0 + threadIdx.x; c < c1; c += ORDER_THREADS) {
+    const uint32_t k = min((ca[c] + cb[c]) >> 2, (uint32_t)ORDER_BUCKETS - 1u);
+    order[atomicAdd(&slot[ORDER_BUCKETS - 1 - k], 1u)] = c;
+  }
+}
+
 static int g_cus = 0;
 
 // Two parts per cell for the generation passes, whose agglomerate-dense
@@ -810,7 +850,14 @@ int pair_grid(int mode) {
 void launch_pair(int mode, int prec, const PairArgs& args, hipStream_t s) {
   const dim3 grid(pair_grid(mode)), block(PAIR_THREADS);
   PairArgs a = args;
-  a.part_shift = a.ncells_total < 16u * grid.x ? (uint32_t)pair_part_shift(mode) : 0u;
+  // launches with few cells per wave: dense cells split, and taken first
+  const bool small = a.ncells_total < 16u * grid.x;
+  a.part_shift = small ? (uint32_t)pair_part_shift(mode) : 0u;
+  if (small && a.order != nullptr && a.part_shift > 0u)
+    hipLaunchKernelGGL(k_order, dim3(QUEUES_PER_KERNEL), dim3(ORDER_THREADS), 0, s, a.prCount,
+                       a.nbCount, a.ncells_total, a.order);
+  else
+    a.order = nullptr;
   a.split_min = (uint32_t)pair_split_min(mode);
 #define NF_LAUNCH(M, P) hipLaunchKernelGGL((k_pair<M, P>), grid, block, 0, s, a)
   if (prec == NF_PREC_FP64) {
