@@ -819,16 +819,17 @@ __global__ void __launch_bounds__(ORDER_THREADS) k_order(const uint32_t* __restr
 
 static int g_cus = 0;
 
-// Two parts per cell for the generation passes, whose agglomerate-dense
-// cells otherwise end each launch alone; none for the contact pass.  Used
-// only when a launch has fewer than 16 cells per wave (launch_pair): with
-// more, the dense cells are spread over many items per wave anyway, and the
-// extra grabs of empty parts cost more than they save.  Measured on MI355X:
-// reject-vs-accepted pass 3.43 -> 3.05 ms per 500k-rod generation (12k
-// cells), same-round pass 1.92 -> 1.89 ms, four parts or splits of 16 or 48
-// probes no better; the 256-packing batch 301 -> 312 ms with parts; the
-// uniform 4M-rod contact pass 2.45 ms unsplit, 2.49 ms split.
-int pair_part_shift(int mode) { return mode == PM_CNT ? 0 : 1; }
+// Two parts per cell for the reject-vs-accepted pass, whose
+// agglomerate-dense cells otherwise end a launch alone; none for the
+// same-round and contact passes.  Used only when a launch has fewer than 16
+// cells per wave (launch_pair): with more, dense cells are spread over many
+// items per wave anyway, and the grabs of empty parts cost more than they
+// save.  Measured on MI355X (500k-rod generation, 12k cells, with the
+// dense-first order of k_order): reject-vs-accepted pass 2.72 ms with two
+// parts of at least 32 probes, 2.77 with 24, 2.97 with four parts;
+// same-round pass 1.80 ms unsplit, 1.91-2.06 split.  The 256-packing batch
+// (1M+ cells): 301 ms unsplit, 312 split.
+int pair_part_shift(int mode) { return mode == PM_GEN_OLD ? 1 : 0; }
 
 int pair_split_min(int mode) {
   (void)mode;
@@ -853,7 +854,7 @@ void launch_pair(int mode, int prec, const PairArgs& args, hipStream_t s) {
   // launches with few cells per wave: dense cells split, and taken first
   const bool small = a.ncells_total < 16u * grid.x;
   a.part_shift = small ? (uint32_t)pair_part_shift(mode) : 0u;
-  if (small && a.order != nullptr && a.part_shift > 0u)
+  if (small && a.order != nullptr)
     hipLaunchKernelGGL(k_order, dim3(QUEUES_PER_KERNEL), dim3(ORDER_THREADS), 0, s, a.prCount,
                        a.nbCount, a.ncells_total, a.order);
   else
