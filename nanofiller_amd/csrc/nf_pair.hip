@@ -780,27 +780,32 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
 }
 
 // Work order of a pair pass: within each of the 64 queue ranges of cells,
-// the cells by decreasing rod count (probes plus neighbour rods, in buckets
-// of four), so that each queue starts with its densest cells and the launch
-// ends on light ones.  One workgroup per range; the order inside a bucket
-// is arbitrary, as every result of a pair pass is independent of the order
-// of its work.
+// the cells by decreasing estimated cost, so that each queue starts with its
+// costliest cells and the launch ends on light ones.  The cost of a cell is
+// the rods around (its probe and neighbour counts) times the volume its
+// stencil walk covers, (h + H)^3, h the cell's longest rod (half length) and
+// H the instance's longest half length plus a cell edge; two buckets per
+// octave.  One workgroup per range; the order inside a bucket is arbitrary,
+// as every result of a pair pass is independent of the order of its work.
 constexpr int ORDER_BUCKETS = 32;
 constexpr int ORDER_THREADS = 256;
-__global__ void __launch_bounds__(ORDER_THREADS) k_order(const uint32_t* __restrict__ ca,
-                                                         const uint32_t* __restrict__ cb,
-                                                         uint32_t ncells, uint32_t* __restrict__ order) {
+__device__ __forceinline__ uint32_t order_bucket(const PairArgs& a, uint32_t c) {
+  const InstDesc& d = a.desc[a.cell_inst[c]];
+  const float h = fmaxf(a.prHl[c], a.nbHl[c]) + d.hl_max + d.edge[0];
+  const float w = (float)(a.prCount[c] + a.nbCount[c]) * h * h * h;
+  // (bits >> 22) = 2 * biased exponent + first mantissa bit; bucket 0: w >= 2^13
+  const int k = 283 - (int)(__float_as_uint(w) >> 22);
+  return (uint32_t)min(max(k, 0), ORDER_BUCKETS - 1);
+}
+
+__global__ void __launch_bounds__(ORDER_THREADS) k_order(PairArgs a) {
   __shared__ uint32_t slot[ORDER_BUCKETS];
   const uint32_t r = blockIdx.x;
-  const uint32_t c0 = (uint32_t)(((uint64_t)ncells * r) / QUEUES_PER_KERNEL);
-  const uint32_t c1 = (uint32_t)(((uint64_t)ncells * (r + 1u)) / QUEUES_PER_KERNEL);
+  const uint32_t c0 = (uint32_t)(((uint64_t)a.ncells_total * r) / QUEUES_PER_KERNEL);
+  const uint32_t c1 = (uint32_t)(((uint64_t)a.ncells_total * (r + 1u)) / QUEUES_PER_KERNEL);
   if (threadIdx.x < ORDER_BUCKETS) slot[threadIdx.x] = 0u;
   __syncthreads();
-  // bucket 0: the densest cells
-  for (uint32_t c = c0 + threadIdx.x; c < c1; c += ORDER_THREADS) {
-    const uint32_t k = min((ca[c] + cb[c]) >> 2, (uint32_t)ORDER_BUCKETS - 1u);
-    atomicAdd(&slot[ORDER_BUCKETS - 1 - k], 1u);
-  }
+  for (uint32_t c = c0 + threadIdx.x; c < c1; c += ORDER_THREADS) atomicAdd(&slot[order_bucket(a, c)], 1u);
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t sum = c0;
@@ -811,10 +816,8 @@ __global__ void __launch_bounds__(ORDER_THREADS) k_order(const uint32_t* __restr
     }
   }
   __syncthreads();
-  for (uint32_t c = c0 + threadIdx.x; c < c1; c += ORDER_THREADS) {
-    const uint32_t k = min((ca[c] + cb[c]) >> 2, (uint32_t)ORDER_BUCKETS - 1u);
-    order[atomicAdd(&slot[ORDER_BUCKETS - 1 - k], 1u)] = c;
-  }
+  for (uint32_t c = c0 + threadIdx.x; c < c1; c += ORDER_THREADS)
+    a.order[atomicAdd(&slot[order_bucket(a, c)], 1u)] = c;
 }
 
 static int g_cus = 0;
@@ -855,8 +858,7 @@ void launch_pair(int mode, int prec, const PairArgs& args, hipStream_t s) {
   const bool small = a.ncells_total < 16u * grid.x;
   a.part_shift = small ? (uint32_t)pair_part_shift(mode) : 0u;
   if (small && a.order != nullptr)
-    hipLaunchKernelGGL(k_order, dim3(QUEUES_PER_KERNEL), dim3(ORDER_THREADS), 0, s, a.prCount,
-                       a.nbCount, a.ncells_total, a.order);
+    hipLaunchKernelGGL(k_order, dim3(QUEUES_PER_KERNEL), dim3(ORDER_THREADS), 0, s, a);
   else
     a.order = nullptr;
   a.split_min = (uint32_t)pair_split_min(mode);
