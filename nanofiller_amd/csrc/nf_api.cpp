@@ -714,7 +714,9 @@ extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
     trace("generate: %u rounds launched, controller rounds %u, running %u, last T %u", launched,
           ri.rounds, ri.running, ri.T);
     if (ri.running == 0 || launched > max_rounds + 2) break;
-    if (batch < 8) batch *= 2;
+    // next batch: the rounds the controller expects to need at the last
+    // round's acceptance (at most 8 between host checks)
+    batch = (int)std::min<uint32_t>(std::max<uint32_t>(ri.est, 1u), 8u);
   }
   const auto t1 = std::chrono::steady_clock::now();
   (void)hipHostFree(ri_host);

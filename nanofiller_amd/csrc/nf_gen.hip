@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(WAVE) k_ctl_end(const InstDesc* __restrict__ d
                                                   const uint32_t* __restrict__ srank,
                                                   const uint32_t* __restrict__ pstart, int n_inst) {
   const int lane = threadIdx.x;
-  uint32_t running = 0;
+  uint32_t running = 0, est = 0;
   for (int i = lane; i < n_inst; i += WAVE) {
     InstState s = st[i];
     if (s.status != NF_STATUS_RUNNING) continue;
@@ -245,13 +245,23 @@ __global__ void __launch_bounds__(WAVE) k_ctl_end(const InstDesc* __restrict__ d
     st[i].stall = s.stall;
     st[i].status = s.status;
     running += (s.status == NF_STATUS_RUNNING) ? 1u : 0u;
+    if (s.status == NF_STATUS_RUNNING) {
+      const uint32_t left = d.n_target - s.n_acc;
+      const uint32_t e = acc ? (left + acc - 1u) / acc : 1u;
+      est = e > est ? e : est;
+    }
   }
   uint32_t v = running;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  for (int o = 32; o > 0; o >>= 1) {
+    v += __shfl_xor(v, o, WAVE);
+    const uint32_t w = __shfl_xor(est, o, WAVE);
+    est = w > est ? w : est;
+  }
   if (lane == 0) {
     ri->running = v;
     ri->rounds += 1;
+    ri->est = est;
   }
 }
 

@@ -52,7 +52,7 @@ struct RoundInfo {
   uint32_t T;         // candidate slots this round over all instances
   uint32_t running;   // instances still running after the last round
   uint32_t rounds;    // rounds executed by the controller
-  uint32_t pad;
+  uint32_t est;       // rounds still needed if the last round's acceptance holds (max over instances)
 };
 
 // Wave reductions for the end of a kernel (every lane converged): sum and
