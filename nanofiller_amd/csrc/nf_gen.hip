@@ -44,13 +44,25 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) 
   return inc - v;
 }
 
-// Plans the round: M per running instance, slot prefix, queue reset.  One
-// wave; lane l owns a contiguous chunk of instances.
-__global__ void __launch_bounds__(WAVE) k_ctl_begin(const InstDesc* __restrict__ desc,
-                                                    InstState* __restrict__ st,
-                                                    RoundInfo* __restrict__ ri,
-                                                    uint32_t* __restrict__ slot_base,
-                                                    uint32_t* __restrict__ queues, int n_inst) {
+// Plans the round: M per running instance, slot prefix, queue reset (the
+// first wave; lane l owns a contiguous chunk of instances), and clears the
+// per-cell proposal counts and longest proposals (the whole grid).
+__global__ void __launch_bounds__(GEN_THREADS) k_ctl_begin(const InstDesc* __restrict__ desc,
+                                                           InstState* __restrict__ st,
+                                                           RoundInfo* __restrict__ ri,
+                                                           uint32_t* __restrict__ slot_base,
+                                                           uint32_t* __restrict__ queues, int n_inst,
+                                                           uint32_t* __restrict__ pcount,
+                                                           uint32_t* __restrict__ phl,
+                                                           uint64_t ncells) {
+  // every thread: clear the per-cell proposal counts and longest proposals
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncells;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    pcount[c] = 0u;
+    phl[c] = 0u;
+  }
+  // the first wave: the controller
+  if (blockIdx.x != 0 || threadIdx.x >= WAVE) return;
   const int lane = threadIdx.x;
   const int chunk = (n_inst + WAVE - 1) / WAVE;
   const int b = lane * chunk;
@@ -276,10 +288,9 @@ static int gen_grid(uint64_t n) {
 void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* ev) {
   const int n = b.n_inst;
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(WAVE), 0, s, b.desc, b.st, b.ri,
-                     b.slot_base, b.queues, n);
-  fill_u32(b.pcount, 0u, b.total_cells, s);
-  fill_u32((uint32_t*)b.phl, 0u, b.total_cells, s);
+  hipLaunchKernelGGL(k_ctl_begin, dim3(gen_grid(b.total_cells)), dim3(GEN_THREADS), 0, s, b.desc,
+                     b.st, b.ri, b.slot_base, b.queues, n, b.pcount, (uint32_t*)b.phl,
+                     (uint64_t)b.total_cells);
   if (ev) (void)hipEventRecord(ev[1], s);
   hipLaunchKernelGGL(k_gen, dim3(gen_grid(b.T_cap + 1)), dim3(GEN_THREADS), 0, s, b.desc, b.st,
                      b.ri, b.slot_base, n, b.T_cap, b.sP, b.sD, b.scell, b.srk, b.surv, b.pcount,

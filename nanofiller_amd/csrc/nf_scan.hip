@@ -4,8 +4,9 @@
 // reference (src/get_contact_arrays.cu:101, src/conductivity.cu:495,
 // src/adjacency_graph.cu:79) and its in-block work-efficient scan
 // (inc/sort.h:103-154).  Three passes (tile sums, scan of tile sums, tile
-// rescan) with 2048-element tiles, 256 threads x 8 items; fixed launch shape
-// for a given n, so the passes can be captured in a HIP graph.
+// rescan) with 2048-element tiles, 256 threads x 8 items, or for arrays of
+// at most 16384 one pass of one workgroup; fixed launch shape for a given
+// n, so the passes can be captured in a HIP graph.
 #include "nf_internal.h"
 
 namespace nf {
@@ -103,10 +104,60 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __r
   }
 }
 
+// Short arrays (the cells of a single packing): one workgroup of 1024
+// threads walks the array in tiles of 4096 with a running carry, one launch
+// instead of three.
+constexpr int SCAN1_THREADS = 1024;
+constexpr int SCAN1_ITEMS = 4;
+constexpr uint64_t SCAN1_MAX = 16384;
+__global__ void __launch_bounds__(SCAN1_THREADS) k_scan_one(const uint32_t* __restrict__ in,
+                                                            uint64_t n, uint32_t* __restrict__ out) {
+  __shared__ uint32_t sm[SCAN1_THREADS / WAVE];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  uint32_t carry = 0;
+  for (uint64_t t0 = 0; t0 < n; t0 += (uint64_t)SCAN1_THREADS * SCAN1_ITEMS) {
+    const uint64_t base = t0 + (uint64_t)threadIdx.x * SCAN1_ITEMS;
+    uint32_t v[SCAN1_ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN1_ITEMS; ++k) {
+      const uint64_t i = base + k;
+      const uint32_t x = in[i < n ? i : n - 1];  // clamped load, masked below
+      v[k] = i < n ? x : 0u;
+      sum += v[k];
+    }
+    const uint32_t inc = wave_incl_scan(sum);
+    if (lane == WAVE - 1) sm[wv] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN1_THREADS / WAVE; ++w) {
+      const uint32_t x = sm[w];
+      off += w < wv ? x : 0u;
+      tot += x;
+    }
+    __syncthreads();
+    uint32_t run = carry + off + inc - sum;
+#pragma unroll
+    for (int k = 0; k < SCAN1_ITEMS; ++k) {
+      const uint64_t i = base + k;
+      if (i < n) out[i] = run;
+      run += v[k];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
 size_t scan_tmp_words(uint64_t n) { return (size_t)((n + SCAN_TILE - 1) / SCAN_TILE) + 1; }
 
 // out[0..n) = exclusive prefix of in, out[n] = total.  in and out may alias.
 void scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s) {
+  if (n > 0 && n <= SCAN1_MAX) {
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(SCAN1_THREADS), 0, s, in, n, out);
+    return;
+  }
   const uint32_t nt = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
   if (nt == 0) {
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_THREADS), 0, s, tmp, 0u, out, (uint64_t)0);
