@@ -404,8 +404,9 @@ __device__ __forceinline__ void next_cell(const PairArgs& a, const CellGeom& g, 
 }
 
 // Prefilter of one (probe, slot) pair, the slot's data in registers:
-// centre distance against the batch-wide reach bound rr2 of the probe, and
-// the pass-specific pair filters.  Unfilled slots sit at 1e30, so the
+// centre distance against the pair's own reach bound rp + Q2.w (the probe's
+// part from its lane, the slot's staged with the slot), and the
+// pass-specific pair filters.  Unfilled slots sit at 1e30, so the
 // distance test drops them.  Keeps every pair the full test could count or
 // cut (see process_cell).
 template <int MODE>
@@ -613,7 +614,6 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
                            : make_float4(far, far, far, 0.0f);
       const float4 Q1 = v1 ? make_float4(p1.x + s1.x, p1.y + s1.y, p1.z + s1.z, (p1.w * 1.000001f) * 1.000006f)
                            : make_float4(far, far, far, 0.0f);
-      // batch-wide half-length bound for the prefilter reach
       // contacts: keep slot iff qprobe < qa || qprobe > qb (home cell: only
       // later indices; elsewhere: anything but the probe itself)
       const uint32_t qa0 = src0, qa1 = src1;
