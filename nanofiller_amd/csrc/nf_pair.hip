@@ -854,10 +854,11 @@ int pair_grid(int mode) {
 void launch_pair(int mode, int prec, const PairArgs& args, hipStream_t s) {
   const dim3 grid(pair_grid(mode)), block(PAIR_THREADS);
   PairArgs a = args;
-  // launches with few cells per wave: dense cells split, and taken first
+  // launches with few cells per wave: costliest cells first (under 64 per
+  // wave), dense cells split (under 16 per wave)
   const bool small = a.ncells_total < 16u * grid.x;
   a.part_shift = small ? (uint32_t)pair_part_shift(mode) : 0u;
-  if (small && a.order != nullptr)
+  if (a.ncells_total < 64u * grid.x && a.order != nullptr)
     hipLaunchKernelGGL(k_order, dim3(QUEUES_PER_KERNEL), dim3(ORDER_THREADS), 0, s, a);
   else
     a.order = nullptr;
