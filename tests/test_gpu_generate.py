@@ -113,6 +113,26 @@ def test_cell_size_does_not_change_packing(cell_size):
     assert s0.pair_tests == s1.pair_tests and s0.rounds == s1.rounds
 
 
+def test_work_schedule_does_not_change_packing():
+    """A launch with few cells per wave splits dense cells over two work
+    items and takes each queue's costliest cells first; one with many cells
+    takes one item per cell in index order.  Both give the same packing and
+    pair-test count: the dense packing alone (about 180 cells) against the
+    same packing in a batch with a sparse 40 um companion (about 800k cells)."""
+    p = cases.small_cfg2(nf.NF_PREC_FP32_RECHECK, n_rods=6000)
+    big = nf.normalize(nf.default_params(box=(40, 40, 40), n_rods=1000, len=0.5, radius=0.005, seed=8,
+                                         stream=3, precision=nf.NF_PREC_FP32_RECHECK))
+    with nf.Packing(p) as pk:
+        s0 = pk.generate()[0]
+        r0 = pk.rods()
+    with nf.Packing([p, big]) as pk:
+        s1 = pk.generate()[0]
+        r1 = pk.rods(0)
+    np.testing.assert_array_equal(r0.view(np.uint32), r1.view(np.uint32))
+    assert s0.pair_tests == s1.pair_tests and s0.rounds == s1.rounds
+    assert s0.rejected_old == s1.rejected_old and s0.rejected_new == s1.rejected_new
+
+
 def test_long_rods_in_small_box_default_cells():
     """Rods almost as long as the box: the default cell edge keeps several
     cells per axis; the packing still equals the oracle's."""
