@@ -76,6 +76,7 @@ struct CellGeom {
   const uint32_t* nCount;
   const float* nHl;
   bool inv;
+  bool idle;  // reject-vs-accepted pass: the instance has no proposals this round
 };
 
 // Per-lane statistics, flushed to the instance (or the contact counters)
@@ -510,6 +511,9 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
                              PairLds& L, Acc& acc) {
   constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
   const int lane = threadIdx.x;
+  // an instance without proposals (finished, or planned none) has no pairs:
+  // its accepted cells would walk their stencils for nothing
+  if (MODE == PM_GEN_OLD && g.idle) return;
   uint32_t pc = uni(g.hCount[gc]);
   if (MODE == PM_GEN_OLD && g.inv) pc = min(pc, g.cap);  // accepted cells hold at most cap rods
   if (pc == 0) return;
@@ -685,6 +689,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
     g.cap = 0xFFFFFFFFu;
     g.state = nullptr;
     g.inv = false;
+    g.idle = false;
   }
   // role pointers of the usual orientation: probes from pr*, staged
   // neighbours from nb*
@@ -757,6 +762,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
             const uint32_t nacc = uni(a.st[inst].n_acc);
             const uint32_t nprop = uni(a.prStart[d.cell_base + d.ncells]) - uni(a.prStart[d.cell_base]);
             g.inv = nacc > nprop;
+            g.idle = nprop == 0u;
             g.hP = g.inv ? a.nbP : a.prP;
             g.hD = g.inv ? a.nbD : a.prD;
             g.hX = g.inv ? a.nbX : a.prX;
@@ -770,6 +776,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
             g.nHl = g.inv ? a.prHl : a.nbHl;
           } else {
             g.inv = false;
+            g.idle = false;
           }
         }
       }
