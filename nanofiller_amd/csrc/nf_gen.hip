@@ -54,12 +54,17 @@ __global__ void __launch_bounds__(GEN_THREADS) k_ctl_begin(const InstDesc* __res
                                                            uint32_t* __restrict__ queues, int n_inst,
                                                            uint32_t* __restrict__ pcount,
                                                            uint32_t* __restrict__ phl,
+                                                           uint32_t* __restrict__ acount,
+                                                           uint32_t* __restrict__ ahl,
                                                            uint64_t ncells) {
-  // every thread: clear the per-cell proposal counts and longest proposals
+  // every thread: clear the per-cell proposal and survivor counts and
+  // longest rods
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncells;
        c += (uint64_t)gridDim.x * blockDim.x) {
     pcount[c] = 0u;
     phl[c] = 0u;
+    acount[c] = 0u;
+    ahl[c] = 0u;
   }
   // the first wave: the controller
   if (blockIdx.x != 0 || threadIdx.x >= WAVE) return;
@@ -165,7 +170,58 @@ __global__ void __launch_bounds__(GEN_THREADS) k_scatter(const RoundInfo* __rest
     pD[dst] = sD[g];
     pslot[dst] = g;
     palive[dst] = 1u;
-    surv[g] = 1u;  // cleared by the same-round pass for the proposals it rejects
+    surv[g] = 1u;  // cleared for the proposals that either pass rejects
+  }
+}
+
+// Survivors of the reject-vs-accepted pass: rank within their cell (any
+// order: no result of the same-round pass depends on it), count and longest
+// rod per cell.  One thread per proposal, like k_gen's binning.
+__global__ void __launch_bounds__(GEN_THREADS) k_alive_count(const RoundInfo* __restrict__ ri,
+                                                             const int32_t* __restrict__ scell,
+                                                             const uint32_t* __restrict__ pstart,
+                                                             const uint32_t* __restrict__ pslot,
+                                                             const uint32_t* __restrict__ palive,
+                                                             const float4* __restrict__ pP,
+                                                             uint32_t* __restrict__ arank,
+                                                             uint32_t* __restrict__ acount,
+                                                             float* __restrict__ ahl,
+                                                             uint64_t ncells) {
+  const uint32_t P = pstart[ncells];  // proposals of this round, all instances
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < P; k += gridDim.x * blockDim.x) {
+    if (!palive[k]) continue;
+    const int32_t c = scell[pslot[k]];
+    arank[k] = atomicAdd(&acount[c], 1u);
+    atomicMax((unsigned int*)&ahl[c], __float_as_uint(pP[k].w));
+  }
+}
+
+// Moves the survivors into their compacted cell ranges; the proposals the
+// reject-vs-accepted pass cut clear their survivor flag.
+__global__ void __launch_bounds__(GEN_THREADS) k_alive_scatter(const int32_t* __restrict__ scell,
+                                                               const uint32_t* __restrict__ pstart,
+                                                               const float4* __restrict__ pP,
+                                                               const float4* __restrict__ pD,
+                                                               const uint32_t* __restrict__ pslot,
+                                                               const uint32_t* __restrict__ palive,
+                                                               const uint32_t* __restrict__ arank,
+                                                               const uint32_t* __restrict__ astart,
+                                                               float4* __restrict__ aP,
+                                                               float4* __restrict__ aD,
+                                                               uint32_t* __restrict__ aslot,
+                                                               uint32_t* __restrict__ surv,
+                                                               uint64_t ncells) {
+  const uint32_t P = pstart[ncells];
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < P; k += gridDim.x * blockDim.x) {
+    const uint32_t g = pslot[k];
+    if (!palive[k]) {
+      surv[g] = 0u;
+      continue;
+    }
+    const uint32_t dst = astart[scell[g]] + arank[k];
+    aP[dst] = pP[k];
+    aD[dst] = pD[k];
+    aslot[dst] = g;
   }
 }
 
@@ -221,7 +277,8 @@ __global__ void __launch_bounds__(WAVE) k_ctl_end(const InstDesc* __restrict__ d
                                                   RoundInfo* __restrict__ ri,
                                                   const uint32_t* __restrict__ slot_base,
                                                   const uint32_t* __restrict__ srank,
-                                                  const uint32_t* __restrict__ pstart, int n_inst) {
+                                                  const uint32_t* __restrict__ pstart,
+                                                  const uint32_t* __restrict__ astart, int n_inst) {
   const int lane = threadIdx.x;
   uint32_t running = 0, est = 0;
   for (int i = lane; i < n_inst; i += WAVE) {
@@ -233,8 +290,11 @@ __global__ void __launch_bounds__(WAVE) k_ctl_end(const InstDesc* __restrict__ d
     const uint32_t deficit = d.n_target - s.n_acc;
     const uint32_t acc = surv < deficit ? surv : deficit;
     const uint32_t P = pstart[d.cell_base + d.ncells] - pstart[d.cell_base];
+    const uint32_t A = astart[d.cell_base + d.ncells] - astart[d.cell_base];
     s.candidates += s.M;
     s.proposals += P;
+    s.rej_old += P - A;  // cut by an accepted rod
+    st[i].rej_old = s.rej_old;
     s.n_acc += acc;
     s.round += 1;
     s.mprev = s.M;
@@ -289,8 +349,8 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   const int n = b.n_inst;
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_ctl_begin, dim3(gen_grid(b.total_cells)), dim3(GEN_THREADS), 0, s, b.desc,
-                     b.st, b.ri, b.slot_base, b.queues, n, b.pcount, (uint32_t*)b.phl,
-                     (uint64_t)b.total_cells);
+                     b.st, b.ri, b.slot_base, b.queues, n, b.pcount, (uint32_t*)b.phl, b.acount,
+                     (uint32_t*)b.ahl, (uint64_t)b.total_cells);
   if (ev) (void)hipEventRecord(ev[1], s);
   hipLaunchKernelGGL(k_gen, dim3(gen_grid(b.T_cap + 1)), dim3(GEN_THREADS), 0, s, b.desc, b.st,
                      b.ri, b.slot_base, n, b.T_cap, b.sP, b.sD, b.scell, b.srk, b.surv, b.pcount,
@@ -316,7 +376,6 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   a.prX = b.pslot;
   a.prStart = b.pstart;
   a.prCount = b.pcount;
-  a.prAlive = nullptr;
   a.prHl = b.phl;
   a.nbP = b.oP;
   a.nbD = b.oD;
@@ -324,20 +383,30 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   a.nbStart = b.ostart;
   a.nbCount = b.ocount;
   a.nbHl = b.ohl;
-  a.nbAlive = nullptr;
   a.qctr = b.queues + Q_OLD * QWORDS;
   a.alive_out = b.palive;
   launch_pair(PM_GEN_OLD, prec, a, s);
   if (ev) (void)hipEventRecord(ev[4], s);
-  // stage 2: survivors vs higher-priority survivors
-  a.prAlive = b.palive;
-  a.nbP = b.pP;
-  a.nbD = b.pD;
-  a.nbX = b.pslot;
-  a.nbStart = b.pstart;
-  a.nbCount = b.pcount;
-  a.nbHl = b.phl;
-  a.nbAlive = b.palive;
+  // stage 2: survivors vs higher-priority survivors, over the survivors
+  // compacted per cell
+  hipLaunchKernelGGL(k_alive_count, dim3(gen_grid(b.T_cap)), dim3(GEN_THREADS), 0, s, b.ri, b.scell,
+                     b.pstart, b.pslot, b.palive, b.pP, b.arank, b.acount, b.ahl, (uint64_t)b.total_cells);
+  scan_u32(b.acount, b.astart, b.total_cells, b.scan_tmp, s);
+  hipLaunchKernelGGL(k_alive_scatter, dim3(gen_grid(b.T_cap)), dim3(GEN_THREADS), 0, s, b.scell,
+                     b.pstart, b.pP, b.pD, b.pslot, b.palive, b.arank, b.astart, b.aP, b.aD, b.aslot,
+                     b.surv, (uint64_t)b.total_cells);
+  a.prP = b.aP;
+  a.prD = b.aD;
+  a.prX = b.aslot;
+  a.prStart = b.astart;
+  a.prCount = b.acount;
+  a.prHl = b.ahl;
+  a.nbP = b.aP;
+  a.nbD = b.aD;
+  a.nbX = b.aslot;
+  a.nbStart = b.astart;
+  a.nbCount = b.acount;
+  a.nbHl = b.ahl;
   a.qctr = b.queues + Q_NEW * QWORDS;
   a.alive_out = nullptr;
   a.surv = b.surv;
@@ -349,7 +418,7 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
                      b.ostart, b.ocount, b.ohl, b.oP, b.oD, b.oid);
   if (ev) (void)hipEventRecord(ev[6], s);
   hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(WAVE), 0, s, b.desc, b.st, b.ri,
-                     b.slot_base, b.srank, b.pstart, n);
+                     b.slot_base, b.srank, b.pstart, b.astart, n);
   if (ev) (void)hipEventRecord(ev[7], s);
 }
 

@@ -115,6 +115,14 @@ struct GenBuffers {
   uint32_t* pcount = nullptr;  // total_cells
   uint32_t* pstart = nullptr;  // total_cells + 1
   uint32_t* order = nullptr;   // total_cells: work order of a pair pass
+  // survivors of the reject-vs-accepted pass, compacted per cell
+  float4* aP = nullptr;
+  float4* aD = nullptr;
+  uint32_t* aslot = nullptr;   // T_cap
+  uint32_t* arank = nullptr;   // T_cap: rank of a proposal among its cell's survivors
+  uint32_t* acount = nullptr;  // total_cells
+  uint32_t* astart = nullptr;  // total_cells + 1
+  float* ahl = nullptr;        // total_cells: longest survivor
   float* phl = nullptr;
   uint32_t* queues = nullptr;  // Q_N * QWORDS
   uint32_t* scan_tmp = nullptr;
@@ -185,7 +193,6 @@ struct PairArgs {
   const uint32_t* prX;
   const uint32_t* prStart;
   const uint32_t* prCount;
-  const uint32_t* prAlive;
   const float* prHl;  // longest probe per cell (reject-vs-accepted pass run accepted-major)
   // neighbours
   const float4* nbP;
@@ -194,7 +201,6 @@ struct PairArgs {
   const uint32_t* nbStart;
   const uint32_t* nbCount;
   const float* nbHl;
-  const uint32_t* nbAlive;
   uint32_t* qctr;
   // outputs
   uint32_t* alive_out;

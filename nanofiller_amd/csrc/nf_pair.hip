@@ -19,8 +19,9 @@
 //                 round's proposals)
 //   PM_GEN_NEW    new vs new, priority   cuda_InternalIsect_noLck / cuda_NewExtIsectLB_parted
 //                                        (inc/intersection_kern.h:12-241)
-//                 (half stencil: each pair of surviving proposals once; a
-//                 cut clears the later slot's survivor flag)
+//                 (over the first pass's survivors, compacted per cell;
+//                 half stencil: each pair of them once; a cut clears the
+//                 later slot's survivor flag)
 //   PM_CNT        contact records        core_cell_interact_kernel CONTACT_COUNT and
 //                                        CONTACT_RECORD in one pass (appended, unsorted)
 //                                        (src/core_cell_interaction_kernel.cu:70-315)
@@ -83,7 +84,6 @@ struct CellGeom {
 // by every lane together; lanes other than 0 add zero.
 struct Acc {
   uint32_t tests, rech, kills;
-  uint32_t kills_old;  // same-round pass: probes that arrive already cut by an accepted rod
   double maxdd;
   unsigned long long bytes;  // wave-uniform: probe and tile bytes streamed
 };
@@ -104,7 +104,6 @@ __device__ __forceinline__ float unif(float v) {
 template <bool GEN>
 __device__ __forceinline__ void flush(const PairArgs& a, const CellGeom& g, Acc& acc, int mode) {
   const uint32_t wt = wave_sum(acc.tests), wr = wave_sum(acc.rech), wk = wave_sum(acc.kills);
-  const uint32_t wo = wave_sum(acc.kills_old);
   double v = acc.maxdd;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
@@ -131,16 +130,13 @@ __device__ __forceinline__ void flush(const PairArgs& a, const CellGeom& g, Acc&
   if (l0) {
     atomicAdd(c0, (unsigned long long)wt);
     atomicAdd(c1, (unsigned long long)wr);
-    // rejections: both kinds are counted by the same-round pass, which
-    // reads every proposal's alive flag after the reject-vs-accepted pass
-    if (GEN && mode == PM_GEN_NEW) {
-      atomicAdd(c2, (unsigned long long)wk);
-      atomicAdd(&g.state->rej_old, (unsigned long long)wo);
-    }
+    // rejections by earlier proposals of the round (the round controller
+    // counts those by accepted rods from the survivor counts)
+    if (GEN && mode == PM_GEN_NEW) atomicAdd(c2, (unsigned long long)wk);
     atomicMax(c3, (unsigned long long)__double_as_longlong(v));
     atomicAdd(cb, acc.bytes);
   }
-  acc.tests = acc.rech = acc.kills = acc.kills_old = 0;
+  acc.tests = acc.rech = acc.kills = 0;
   acc.maxdd = 0.0;
   acc.bytes = 0;
 }
@@ -418,9 +414,6 @@ __device__ __forceinline__ bool prefilter(const float4 P1, float rp, uint32_t X1
   const float w2 = fmaf(wz, wz, fmaf(wy, wy, wx * wx));
   const float rr = rp + Q2.w;  // the pair's own reach bound
   bool f = w2 <= rr * rr;
-  // same-round pass: the slot must have survived the reject-vs-accepted
-  // pass (dead slots carry id 0xFFFFFFFF)
-  if (MODE == PM_GEN_NEW) f = f && xj != 0xFFFFFFFFu;
   // half-stencil passes (contacts, same-round): a slot of the home cell only
   // after the probe's index; another cell (an image may map back to the home
   // cell) never the probe itself
@@ -504,8 +497,7 @@ __device__ __forceinline__ void probe_loop(const PairArgs& a, const CellGeom& g,
 // part k < np walks the stencil blocks k, k + np, k + 2 np, ... (64 offsets
 // each), so the parts partition the pairs of the cell and a dense cell is
 // worked on by several waves at once; parts k >= np return at once.
-// Per-probe effects that are not idempotent (the same-round pass's count of
-// probes already cut, the probe bytes) belong to part 0.
+// The probe bytes, counted per probe, belong to part 0.
 template <int MODE, int PREC>
 __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, uint32_t part,
                              PairLds& L, Acc& acc) {
@@ -534,15 +526,7 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
     const float4 P1 = g.hP[pidx];
     const float4 D1 = g.hD[pidx];
     const uint32_t X1 = g.hX[pidx];
-    bool have = inrange;
-    if (MODE == PM_GEN_NEW) {
-      const bool alive = a.prAlive[pidx] != 0u;
-      const bool doa = inrange && !alive && part == 0u;  // already cut by an accepted rod
-      acc.kills_old += doa ? 1u : 0u;
-      uint32_t* dst = doa ? &a.surv[X1] : &L.dummy[lane];
-      *dst = 0u;
-      have = have && alive;
-    }
+    const bool have = inrange;
     L.sP[lane] = P1;
     L.sD[lane] = D1;
     L.sX[lane] = X1;
@@ -598,10 +582,6 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
       if (!(MODE == PM_GEN_OLD && g.inv)) {
         x0 = g.nX[src0];
         x1 = g.nX[src1];
-      }
-      if (MODE == PM_GEN_NEW) {
-        x0 = (a.nbAlive[src0] != 0u) ? x0 : 0xFFFFFFFFu;
-        x1 = (a.nbAlive[src1] != 0u) ? x1 : 0xFFFFFFFFu;
       }
       L.bP[lane] = p0;
       L.bD[lane] = d0;
@@ -667,7 +647,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
   if (GEN && uni(a.ri->T) == 0u) return;
   const bool l0 = threadIdx.x == 0;
   CellGeom g;
-  Acc acc = {0u, 0u, 0u, 0u, 0.0, 0ull};
+  Acc acc = {0u, 0u, 0u, 0.0, 0ull};
   uint32_t cur_inst = 0xFFFFFFFFu;
   if (!GEN) {
     for (int d = 0; d < 3; ++d) {
