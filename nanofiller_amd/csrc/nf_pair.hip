@@ -77,7 +77,7 @@ struct CellGeom {
   const uint32_t* nCount;
   const float* nHl;
   bool inv;
-  bool idle;  // reject-vs-accepted pass: the instance has no proposals this round
+  bool idle;  // reject-vs-accepted pass: the instance has no proposals, or no accepted rods
 };
 
 // Per-lane statistics, flushed to the instance (or the contact counters)
@@ -503,8 +503,9 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
                              PairLds& L, Acc& acc) {
   constexpr bool GEN = (MODE == PM_GEN_OLD || MODE == PM_GEN_NEW);
   const int lane = threadIdx.x;
-  // an instance without proposals (finished, or planned none) has no pairs:
-  // its accepted cells would walk their stencils for nothing
+  // an instance without proposals (finished, or planned none) or without
+  // accepted rods (its first round) has no pairs: its cells would walk their
+  // stencils for nothing
   if (MODE == PM_GEN_OLD && g.idle) return;
   uint32_t pc = uni(g.hCount[gc]);
   if (MODE == PM_GEN_OLD && g.inv) pc = min(pc, g.cap);  // accepted cells hold at most cap rods
@@ -742,7 +743,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
             const uint32_t nacc = uni(a.st[inst].n_acc);
             const uint32_t nprop = uni(a.prStart[d.cell_base + d.ncells]) - uni(a.prStart[d.cell_base]);
             g.inv = nacc > nprop;
-            g.idle = nprop == 0u;
+            g.idle = nprop == 0u || nacc == 0u;
             g.hP = g.inv ? a.nbP : a.prP;
             g.hD = g.inv ? a.nbD : a.prD;
             g.hX = g.inv ? a.nbX : a.prX;
