@@ -72,20 +72,23 @@ __global__ void __launch_bounds__(GEN_THREADS) k_ctl_begin(const InstDesc* __res
   const int chunk = (n_inst + WAVE - 1) / WAVE;
   const int b = lane * chunk;
   const int e = min(b + chunk, n_inst);
-  uint32_t sum = 0;
+  uint32_t sum = 0, old = 0;
   for (int i = b; i < e; ++i) {
     uint32_t M = 0;
     if (st[i].status == NF_STATUS_RUNNING) M = schedule(desc[i], st[i]);
     st[i].M = M;
     slot_base[i] = sum;
     sum += M;
+    old += (M > 0u && st[i].n_acc > 0u) ? 1u : 0u;
   }
-  uint32_t total;
+  uint32_t total, old_total;
   const uint32_t off = wave_excl_scan(sum, &total);
+  (void)wave_excl_scan(old, &old_total);
   for (int i = b; i < e; ++i) slot_base[i] += off;
   if (lane == 0) {
     slot_base[n_inst] = total;
     ri->T = total;
+    ri->old = old_total;
   }
   for (int q = lane; q < Q_N * QUEUES_PER_KERNEL; q += WAVE) queues[q * QSTRIDE] = 0;
 }

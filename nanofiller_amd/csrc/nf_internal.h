@@ -53,6 +53,8 @@ struct RoundInfo {
   uint32_t running;   // instances still running after the last round
   uint32_t rounds;    // rounds executed by the controller
   uint32_t est;       // rounds still needed if the last round's acceptance holds (max over instances)
+  uint32_t old;       // instances with candidates and accepted rods this round (0: no first-pass pairs)
+  uint32_t pad[3];
 };
 
 // Wave reductions for the end of a kernel (every lane converged): sum and

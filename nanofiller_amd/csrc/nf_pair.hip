@@ -646,6 +646,8 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
   const int xcd = blockIdx.x & 7;
   // rounds after every instance finished plan no candidates: nothing to do
   if (GEN && uni(a.ri->T) == 0u) return;
+  // no instance has both candidates and accepted rods: no first-pass pairs
+  if (MODE == PM_GEN_OLD && uni(a.ri->old) == 0u) return;
   const bool l0 = threadIdx.x == 0;
   CellGeom g;
   Acc acc = {0u, 0u, 0u, 0.0, 0ull};
