@@ -240,11 +240,10 @@ void launch_ctl_init(GenBuffers& b, hipStream_t s);
 
 // ---- launchers (nf_pair.hip) ----
 void launch_pair(int mode, int prec, const PairArgs& a, hipStream_t s);
-// Persistent grid of a pair pass (waves): 16 per CU for the
-// reject-vs-accepted and contact passes, 12 per CU for the same-round pass,
-// whose light, sparse cells run faster with fewer waves competing for the
-// memory system (measured on MI355X, 500k-rod workload: 300 us per launch at
-// 12 per CU, 311 us at 16, 391 us at 6).
+// Persistent grid of a pair pass (waves): 16 per CU, the most that the
+// register budget lets reside (measured on MI355X, 500k-rod workload in 3
+// rounds: same-round pass 1.72 ms per generation at 16 per CU, 1.82 at 12;
+// with the earlier, smaller rounds 12 per CU had been best).
 int pair_grid(int mode);
 // Work items per cell of a pair pass, as a shift: each dense cell's stencil
 // walk is dealt over 1 << shift items, so that dense cells do not hold a

@@ -838,7 +838,8 @@ int pair_grid(int mode) {
       cus = 256;
     g_cus = cus;
   }
-  return g_cus * (mode == PM_GEN_NEW ? 12 : 16);
+  (void)mode;
+  return g_cus * 16;
 }
 
 void launch_pair(int mode, int prec, const PairArgs& args, hipStream_t s) {
