@@ -73,7 +73,7 @@ typedef struct nf_params {
   uint32_t max_rounds;  /* round cap (reference: 1000, cudaMain.cu:113)    */
   uint32_t stall_rounds;/* stop after this many rounds with no acceptance  */
   uint32_t min_batch;   /* candidates per round, lower clamp               */
-  uint32_t max_batch;   /* candidates per round, upper clamp (0: derived)  */
+  uint32_t max_batch;   /* candidates per round, upper clamp (0: derived, three excluded volumes' worth) */
   double cell_size;     /* generation cell edge; 0: auto (about one typical
                            pair reach, smaller for long rods in small boxes);
                            < 0: |cell_size| times the auto edge              */
