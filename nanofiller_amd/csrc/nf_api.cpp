@@ -431,8 +431,7 @@ static void release_gen(nf_handle* h) {
   void* ptrs[] = {b.desc, b.st, b.ri, b.slot_base, b.stencil, b.agg, b.mesh, b.cell_inst, b.oP, b.oD,
                   b.oid, b.ostart, b.ocount, b.ohl, b.bP, b.bD, b.sP, b.sD, b.scell, b.srk,
                   b.surv, b.srank, b.pP, b.pD, b.pslot, b.palive, b.pcount, b.pstart, b.phl,
-                  b.order, b.aP, b.aD, b.aslot, b.arank, b.acount, b.astart, b.ahl, b.queues,
-                  b.scan_tmp};
+                  b.order, b.aP, b.aD, b.aslot, b.aidx, b.acount, b.astart, b.queues, b.scan_tmp};
   for (void* p : ptrs) dfree(h, p);
   b = GenBuffers();
 }
@@ -544,10 +543,9 @@ extern "C" int nf_create_batch(const nf_params* ps, int n, nf_handle** out) {
   ALLOC(b.aP, b.T_cap);
   ALLOC(b.aD, b.T_cap);
   ALLOC(b.aslot, b.T_cap);
-  ALLOC(b.arank, b.T_cap);
+  ALLOC(b.aidx, b.T_cap + 1);
   ALLOC(b.acount, b.total_cells);
   ALLOC(b.astart, b.total_cells + 1);
-  ALLOC(b.ahl, b.total_cells);
   ALLOC(b.phl, b.total_cells);
   ALLOC(b.queues, Q_N * QWORDS);
   ALLOC(b.scan_tmp, std::max(scan_tmp_words(b.total_cells), scan_tmp_words(b.T_cap)));

@@ -54,17 +54,12 @@ __global__ void __launch_bounds__(GEN_THREADS) k_ctl_begin(const InstDesc* __res
                                                            uint32_t* __restrict__ queues, int n_inst,
                                                            uint32_t* __restrict__ pcount,
                                                            uint32_t* __restrict__ phl,
-                                                           uint32_t* __restrict__ acount,
-                                                           uint32_t* __restrict__ ahl,
                                                            uint64_t ncells) {
-  // every thread: clear the per-cell proposal and survivor counts and
-  // longest rods
+  // every thread: clear the per-cell proposal counts and longest proposals
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncells;
        c += (uint64_t)gridDim.x * blockDim.x) {
     pcount[c] = 0u;
     phl[c] = 0u;
-    acount[c] = 0u;
-    ahl[c] = 0u;
   }
   // the first wave: the controller
   if (blockIdx.x != 0 || threadIdx.x >= WAVE) return;
@@ -127,12 +122,14 @@ __global__ void __launch_bounds__(GEN_THREADS) k_gen(const InstDesc* __restrict_
                                                      int32_t* __restrict__ scell,
                                                      uint32_t* __restrict__ srk,
                                                      uint32_t* __restrict__ surv,
+                                                     uint32_t* __restrict__ palive,
                                                      uint32_t* __restrict__ pcount,
                                                      float* __restrict__ phl) {
   const uint32_t T = ri->T;
   for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g <= T_cap;
        g += (uint64_t)gridDim.x * blockDim.x) {
     surv[g] = 0;
+    if (g < T_cap) palive[g] = 0;  // the survivors' scan reads all T_cap flags
     if (g >= T) continue;
     const int inst = find_inst(slot_base, n_inst, (uint32_t)g);
     const InstDesc& d = desc[inst];
@@ -177,54 +174,46 @@ __global__ void __launch_bounds__(GEN_THREADS) k_scatter(const RoundInfo* __rest
   }
 }
 
-// Survivors of the reject-vs-accepted pass: rank within their cell (any
-// order: no result of the same-round pass depends on it), count and longest
-// rod per cell.  One thread per proposal, like k_gen's binning.
-__global__ void __launch_bounds__(GEN_THREADS) k_alive_count(const RoundInfo* __restrict__ ri,
-                                                             const int32_t* __restrict__ scell,
-                                                             const uint32_t* __restrict__ pstart,
-                                                             const uint32_t* __restrict__ pslot,
-                                                             const uint32_t* __restrict__ palive,
-                                                             const float4* __restrict__ pP,
-                                                             uint32_t* __restrict__ arank,
-                                                             uint32_t* __restrict__ acount,
-                                                             float* __restrict__ ahl,
-                                                             uint64_t ncells) {
-  const uint32_t P = pstart[ncells];  // proposals of this round, all instances
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < P; k += gridDim.x * blockDim.x) {
-    if (!palive[k]) continue;
-    const int32_t c = scell[pslot[k]];
-    arank[k] = atomicAdd(&acount[c], 1u);
-    atomicMax((unsigned int*)&ahl[c], __float_as_uint(pP[k].w));
-  }
-}
-
-// Moves the survivors into their compacted cell ranges; the proposals the
-// reject-vs-accepted pass cut clear their survivor flag.
-__global__ void __launch_bounds__(GEN_THREADS) k_alive_scatter(const int32_t* __restrict__ scell,
-                                                               const uint32_t* __restrict__ pstart,
+// Moves the survivors of the reject-vs-accepted pass to their compacted
+// positions - aidx is the exclusive prefix of the alive flags over the
+// cell-sorted proposals, so each cell's survivors stay together and in
+// order - sets the compacted cell ranges, and clears the survivor flag of
+// the proposals that pass cut.
+__global__ void __launch_bounds__(GEN_THREADS) k_alive_scatter(const uint32_t* __restrict__ pstart,
                                                                const float4* __restrict__ pP,
                                                                const float4* __restrict__ pD,
                                                                const uint32_t* __restrict__ pslot,
                                                                const uint32_t* __restrict__ palive,
-                                                               const uint32_t* __restrict__ arank,
-                                                               const uint32_t* __restrict__ astart,
+                                                               const uint32_t* __restrict__ aidx,
                                                                float4* __restrict__ aP,
                                                                float4* __restrict__ aD,
                                                                uint32_t* __restrict__ aslot,
+                                                               uint32_t* __restrict__ astart,
+                                                               uint32_t* __restrict__ acount,
                                                                uint32_t* __restrict__ surv,
                                                                uint64_t ncells) {
-  const uint32_t P = pstart[ncells];
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < P; k += gridDim.x * blockDim.x) {
-    const uint32_t g = pslot[k];
-    if (!palive[k]) {
-      surv[g] = 0u;
-      continue;
+  const uint64_t P = pstart[ncells];  // proposals of this round, all instances
+  const uint64_t n = P > ncells + 1 ? P : ncells + 1;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    if (k < P) {
+      const uint32_t g = pslot[k];
+      if (palive[k]) {
+        const uint32_t dst = aidx[k];
+        aP[dst] = pP[k];
+        aD[dst] = pD[k];
+        aslot[dst] = g;
+      } else {
+        surv[g] = 0u;
+      }
     }
-    const uint32_t dst = astart[scell[g]] + arank[k];
-    aP[dst] = pP[k];
-    aD[dst] = pD[k];
-    aslot[dst] = g;
+    if (k < ncells) {
+      const uint32_t a0 = aidx[pstart[k]];
+      astart[k] = a0;
+      acount[k] = aidx[pstart[k + 1]] - a0;
+    } else if (k == ncells) {
+      astart[k] = aidx[pstart[k]];
+    }
   }
 }
 
@@ -352,12 +341,12 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   const int n = b.n_inst;
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_ctl_begin, dim3(gen_grid(b.total_cells)), dim3(GEN_THREADS), 0, s, b.desc,
-                     b.st, b.ri, b.slot_base, b.queues, n, b.pcount, (uint32_t*)b.phl, b.acount,
-                     (uint32_t*)b.ahl, (uint64_t)b.total_cells);
+                     b.st, b.ri, b.slot_base, b.queues, n, b.pcount, (uint32_t*)b.phl,
+                     (uint64_t)b.total_cells);
   if (ev) (void)hipEventRecord(ev[1], s);
   hipLaunchKernelGGL(k_gen, dim3(gen_grid(b.T_cap + 1)), dim3(GEN_THREADS), 0, s, b.desc, b.st,
-                     b.ri, b.slot_base, n, b.T_cap, b.sP, b.sD, b.scell, b.srk, b.surv, b.pcount,
-                     b.phl);
+                     b.ri, b.slot_base, n, b.T_cap, b.sP, b.sD, b.scell, b.srk, b.surv, b.palive,
+                     b.pcount, b.phl);
   if (ev) (void)hipEventRecord(ev[2], s);
   scan_u32(b.pcount, b.pstart, b.total_cells, b.scan_tmp, s);
   hipLaunchKernelGGL(k_scatter, dim3(gen_grid(b.T_cap)), dim3(GEN_THREADS), 0, s, b.ri, b.sP,
@@ -391,25 +380,24 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   launch_pair(PM_GEN_OLD, prec, a, s);
   if (ev) (void)hipEventRecord(ev[4], s);
   // stage 2: survivors vs higher-priority survivors, over the survivors
-  // compacted per cell
-  hipLaunchKernelGGL(k_alive_count, dim3(gen_grid(b.T_cap)), dim3(GEN_THREADS), 0, s, b.ri, b.scell,
-                     b.pstart, b.pslot, b.palive, b.pP, b.arank, b.acount, b.ahl, (uint64_t)b.total_cells);
-  scan_u32(b.acount, b.astart, b.total_cells, b.scan_tmp, s);
-  hipLaunchKernelGGL(k_alive_scatter, dim3(gen_grid(b.T_cap)), dim3(GEN_THREADS), 0, s, b.scell,
-                     b.pstart, b.pP, b.pD, b.pslot, b.palive, b.arank, b.astart, b.aP, b.aD, b.aslot,
-                     b.surv, (uint64_t)b.total_cells);
+  // compacted per cell (a cell's longest rod stays that of all its
+  // proposals: a bound, as the stencil walk needs)
+  scan_u32(b.palive, b.aidx, b.T_cap, b.scan_tmp, s);
+  hipLaunchKernelGGL(k_alive_scatter, dim3(gen_grid(std::max(b.T_cap, b.total_cells + 1))),
+                     dim3(GEN_THREADS), 0, s, b.pstart, b.pP, b.pD, b.pslot, b.palive, b.aidx, b.aP,
+                     b.aD, b.aslot, b.astart, b.acount, b.surv, (uint64_t)b.total_cells);
   a.prP = b.aP;
   a.prD = b.aD;
   a.prX = b.aslot;
   a.prStart = b.astart;
   a.prCount = b.acount;
-  a.prHl = b.ahl;
+  a.prHl = b.phl;
   a.nbP = b.aP;
   a.nbD = b.aD;
   a.nbX = b.aslot;
   a.nbStart = b.astart;
   a.nbCount = b.acount;
-  a.nbHl = b.ahl;
+  a.nbHl = b.phl;
   a.qctr = b.queues + Q_NEW * QWORDS;
   a.alive_out = nullptr;
   a.surv = b.surv;

@@ -121,10 +121,9 @@ struct GenBuffers {
   float4* aP = nullptr;
   float4* aD = nullptr;
   uint32_t* aslot = nullptr;   // T_cap
-  uint32_t* arank = nullptr;   // T_cap: rank of a proposal among its cell's survivors
+  uint32_t* aidx = nullptr;    // T_cap + 1: exclusive prefix of the alive flags
   uint32_t* acount = nullptr;  // total_cells
   uint32_t* astart = nullptr;  // total_cells + 1
-  float* ahl = nullptr;        // total_cells: longest survivor
   float* phl = nullptr;
   uint32_t* queues = nullptr;  // Q_N * QWORDS
   uint32_t* scan_tmp = nullptr;
