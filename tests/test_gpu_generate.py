@@ -69,9 +69,8 @@ def test_eager_and_profiled_agree():
         np.testing.assert_array_equal(o.view(np.uint32), outs[0].view(np.uint32))
 
 
-@pytest.mark.skip(reason="HIP-graph replay of the generation round timed out on the GPU with no "
-                         "cause found on the CPU; the path is off by default until one is shown")
 def test_graph_replay_agrees():
+    """HIP-graph replay of the round (NF_OPT_GRAPH) gives the eager packing."""
     p = cases.small_iso(nf.NF_PREC_FP32_RECHECK)
     with nf.Packing(p, {nf.NF_OPT_GRAPH: 1}) as pk:
         pk.generate()
