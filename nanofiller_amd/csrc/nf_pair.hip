@@ -405,8 +405,9 @@ __device__ __forceinline__ void next_cell(const PairArgs& a, const CellGeom& g, 
 // part from its lane, the slot's staged with the slot), and the
 // pass-specific pair filters.  Unfilled slots sit at 1e30, so the
 // distance test drops them.  Keeps every pair the full test could count or
-// cut (see process_cell).
-template <int MODE>
+// cut (see process_cell).  ORD: the batch holds slots the ordering below
+// can drop (else it keeps every pair, and is left out).
+template <int MODE, bool ORD>
 __device__ __forceinline__ bool prefilter(const float4 P1, float rp, uint32_t X1, uint32_t qprobe,
                                           const float4 Q2, uint32_t xj, uint32_t qa, uint32_t qb) {
   constexpr bool CNT = (MODE == PM_CNT);
@@ -417,7 +418,7 @@ __device__ __forceinline__ bool prefilter(const float4 P1, float rp, uint32_t X1
   // half-stencil passes (contacts, same-round): a slot of the home cell only
   // after the probe's index; another cell (an image may map back to the home
   // cell) never the probe itself
-  if (CNT || MODE == PM_GEN_NEW) f = f && (qprobe < qa || qprobe > qb);
+  if ((CNT || MODE == PM_GEN_NEW) && ORD) f = f && (qprobe < qa || qprobe > qb);
   return f;
 }
 
@@ -448,7 +449,7 @@ __device__ __forceinline__ void push_pairs(const PairArgs& a, const CellGeom& g,
 // the probes of the mask (read from their lanes' registers); TWO: the batch
 // fills the second slot of the lanes too.  rp_l is the probe part of the
 // lane's own probe's reach bound; the slot part sits in Q.w.
-template <int MODE, int PREC, bool TWO>
+template <int MODE, int PREC, bool TWO, bool ORD>
 __device__ __forceinline__ void probe_loop(const PairArgs& a, const CellGeom& g, PairLds& L,
                                            unsigned long long pmask, const float4 P1, uint32_t X1,
                                            float rp_l, uint32_t qbase, const float4 Q0, uint32_t x0,
@@ -464,8 +465,8 @@ __device__ __forceinline__ void probe_loop(const PairArgs& a, const CellGeom& g,
     const uint32_t Xp = (uint32_t)__builtin_amdgcn_readlane((int)X1, pl);
     const float rp = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rp_l), pl));
     const uint32_t qprobe = qbase + (uint32_t)pl;
-    const bool f0 = prefilter<MODE>(Pp, rp, Xp, qprobe, Q0, x0, qa0, qb0);
-    const bool f1 = TWO && prefilter<MODE>(Pp, rp, Xp, qprobe, Q1, x1, qa1, qb1);
+    const bool f0 = prefilter<MODE, ORD>(Pp, rp, Xp, qprobe, Q0, x0, qa0, qb0);
+    const bool f1 = TWO && prefilter<MODE, ORD>(Pp, rp, Xp, qprobe, Q1, x1, qa1, qb1);
     push_pairs<MODE, PREC, TWO>(a, g, L, f0, f1, (uint32_t)pl, qn, q2n, acc);
   }
 }
@@ -604,16 +605,34 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
       const uint32_t qa0 = src0, qa1 = src1;
       const uint32_t qb0 = (m0 & 0x80000000u) ? 0xFFFFFFFFu : src0;
       const uint32_t qb1 = (m1 & 0x80000000u) ? 0xFFFFFFFFu : src1;
+      // the ordering can only drop a pair whose slot is of the home cell, or
+      // is one of the chunk's probes (through an image)
+      bool ord = false;
+      if (MODE == PM_CNT || MODE == PM_GEN_NEW) {
+        const uint32_t qlo = ps + pb;
+        const bool o0 = v0 && ((m0 & 0x80000000u) != 0u || src0 - qlo < G);
+        const bool o1 = v1 && ((m1 & 0x80000000u) != 0u || src1 - qlo < G);
+        ord = __ballot(o0 || o1) != 0ull;
+      }
       __syncthreads();
       // ---- prefilter every (probe, slot) pair, queue the survivors
       // each lane's own probe part of the pair reach bound
       const float rp_l = (P1.w * 1.000001f + pad_r) * 1.000006f;
-      if (fill > (uint32_t)WAVE)
-        probe_loop<MODE, PREC, true>(a, g, L, pmask, P1, X1, rp_l, ps + pb, Q0, x0, qa0, qb0, Q1,
-                                     x1, qa1, qb1, qn, q2n, acc);
-      else
-        probe_loop<MODE, PREC, false>(a, g, L, pmask, P1, X1, rp_l, ps + pb, Q0, x0, qa0, qb0, Q1,
-                                      x1, qa1, qb1, qn, q2n, acc);
+      if (ord) {
+        if (fill > (uint32_t)WAVE)
+          probe_loop<MODE, PREC, true, true>(a, g, L, pmask, P1, X1, rp_l, ps + pb, Q0, x0, qa0, qb0,
+                                             Q1, x1, qa1, qb1, qn, q2n, acc);
+        else
+          probe_loop<MODE, PREC, false, true>(a, g, L, pmask, P1, X1, rp_l, ps + pb, Q0, x0, qa0, qb0,
+                                              Q1, x1, qa1, qb1, qn, q2n, acc);
+      } else {
+        if (fill > (uint32_t)WAVE)
+          probe_loop<MODE, PREC, true, false>(a, g, L, pmask, P1, X1, rp_l, ps + pb, Q0, x0, qa0, qb0,
+                                              Q1, x1, qa1, qb1, qn, q2n, acc);
+        else
+          probe_loop<MODE, PREC, false, false>(a, g, L, pmask, P1, X1, rp_l, ps + pb, Q0, x0, qa0,
+                                               qb0, Q1, x1, qa1, qb1, qn, q2n, acc);
+      }
       // ---- drain both queues before the slots are restaged
       if (qn > 0) {
         __syncthreads();
