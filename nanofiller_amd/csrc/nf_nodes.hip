@@ -24,9 +24,10 @@
 //  * consecutive nodes of a rod are joined by a segment of length
 //    L = (t_b - t_a) / a, cross-section radius A = a.
 //
-// One thread per contact for the contact points and screening (the rods of
-// the cells whose box lies within reach of the contact's midpoint), one
-// thread per rod for the point lists, sorting and merging.
+// One wave per contact for the contact points and screening (the lanes
+// spread over the rods of the cells whose box lies within reach of the
+// contact's midpoint, 64 at a time), one thread per rod for the point
+// lists, sorting and merging.
 #include "nf_internal.h"
 
 namespace nf {
@@ -52,7 +53,10 @@ __device__ __forceinline__ int img_c(int32_t code, int axis) {
 // binary search in the row pointers).
 __global__ void __launch_bounds__(NT) k_nd_points(NodeArgs a) {
   const uint32_t nc = a.rowptr[a.n];
-  for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < nc; k += gridDim.x * NT) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
+  const uint32_t w0 = (blockIdx.x * NT + threadIdx.x) / WAVE, nw = gridDim.x * NT / WAVE;
+  // one wave per contact: every lane holds the same contact values
+  for (uint32_t k = w0; k < nc; k += nw) {
     uint32_t lo = 0, hi = a.n;  // rowptr[lo] <= k < rowptr[hi]
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -66,7 +70,7 @@ __global__ void __launch_bounds__(NT) k_nd_points(NodeArgs a) {
     {
       const int32_t code = a.img[k];
       if (img_c(code, a.axis) != 0) {
-        a.pflag[k] = NODE_CROSSING;
+        if (lane == 0) a.pflag[k] = NODE_CROSSING;
         continue;
       }
       const uint32_t j = a.J[k];
@@ -77,8 +81,10 @@ __global__ void __launch_bounds__(NT) k_nd_points(NodeArgs a) {
       bool w;
       (void)pair_closest_f64(PI, DI, PJ, DJ, ux, uy, uz, 0.0, &w, &s, &t);
       const double tI = s * (double)PI.w, tJ = t * (double)PJ.w;
-      a.ptI[k] = tI;
-      a.ptJ[k] = tJ;
+      if (lane == 0) {
+        a.ptI[k] = tI;
+        a.ptJ[k] = tJ;
+      }
       // the segment between the contact points
       const double ax = (double)PI.x + tI * (double)DI.x, ay = (double)PI.y + tI * (double)DI.y,
                    az = (double)PI.z + tI * (double)DI.z;
@@ -126,30 +132,35 @@ __global__ void __launch_bounds__(NT) k_nd_points(NodeArgs a) {
             if (gap2 > reach * reach * 1.000001) continue;
             const uint32_t cell = (uint32_t)(m[0] + a.ncell[0] * (m[1] + a.ncell[1] * m[2]));
             const uint32_t e = a.cstart[cell] + a.ccount[cell];
-            for (uint32_t q = a.cstart[cell]; q < e; ++q) {
-              const uint32_t kk = a.cid[q];
-              if (kk == r || kk == j) continue;
-              const float4 PK = a.cP[q], DK = a.cD[q];
-              const double R = av + (double)DK.w;
-              // conservative centre test, then the exact segment distance
-              const double cx = ((double)PK.x + sh[0]) - mx, cy = ((double)PK.y + sh[1]) - my,
-                           cz = ((double)PK.z + sh[2]) - mz;
-              const double lim = ((double)PK.w + hs + R) * 1.000000001;
-              if ((cx * cx + cy * cy) + cz * cz > lim * lim) continue;
-              const double d2 = seg_seg_d2(mx, my, mz, hx, hy, hz, (double)PK.x + sh[0],
-                                           (double)PK.y + sh[1], (double)PK.z + sh[2],
-                                           (double)PK.w * (double)DK.x, (double)PK.w * (double)DK.y,
-                                           (double)PK.w * (double)DK.z);
-              if (d2 < R * R) {
-                screened = true;
-                break;
+            // the lanes take the cell's rods 64 at a time
+            for (uint32_t q0 = a.cstart[cell]; q0 < e && !screened; q0 += WAVE) {
+              const uint32_t q = q0 + lane;
+              bool hit = false;
+              if (q < e) {
+                const uint32_t kk = a.cid[q];
+                const float4 PK = a.cP[q], DK = a.cD[q];
+                const double R = av + (double)DK.w;
+                // conservative centre test, then the exact segment distance
+                const double cx = ((double)PK.x + sh[0]) - mx, cy = ((double)PK.y + sh[1]) - my,
+                             cz = ((double)PK.z + sh[2]) - mz;
+                const double lim = ((double)PK.w + hs + R) * 1.000000001;
+                if (kk != r && kk != j && (cx * cx + cy * cy) + cz * cz <= lim * lim) {
+                  const double d2 = seg_seg_d2(mx, my, mz, hx, hy, hz, (double)PK.x + sh[0],
+                                               (double)PK.y + sh[1], (double)PK.z + sh[2],
+                                               (double)PK.w * (double)DK.x, (double)PK.w * (double)DK.y,
+                                               (double)PK.w * (double)DK.z);
+                  hit = d2 < R * R;
+                }
               }
+              screened = __ballot(hit) != 0ull;
             }
           }
-      a.pflag[k] = screened ? NODE_SCREENED : NODE_KEPT;
-      if (!screened) {
-        atomicAdd(&a.npt[r], 1u);
-        atomicAdd(&a.npt[j], 1u);
+      if (lane == 0) {
+        a.pflag[k] = screened ? NODE_SCREENED : NODE_KEPT;
+        if (!screened) {
+          atomicAdd(&a.npt[r], 1u);
+          atomicAdd(&a.npt[j], 1u);
+        }
       }
     }
   }
@@ -309,7 +320,7 @@ void launch_node_count(const NodeArgs& a, uint32_t nc, unsigned long long* out, 
 }
 
 void launch_node_points(const NodeArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_nd_points, dim3(grid_n(a.n)), dim3(NT), 0, s, a);
+  hipLaunchKernelGGL(k_nd_points, dim3(8192), dim3(NT), 0, s, a);  // waves stride over the contacts
   hipLaunchKernelGGL(k_nd_elec, dim3(grid_n(a.n)), dim3(NT), 0, s, a);
 }
 
