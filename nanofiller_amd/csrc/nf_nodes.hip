@@ -132,6 +132,10 @@ __global__ void __launch_bounds__(NT) k_nd_points(NodeArgs a) {
             if (gap2 > reach * reach * 1.000001) continue;
             const uint32_t cell = (uint32_t)(m[0] + a.ncell[0] * (m[1] + a.ncell[1] * m[2]));
             const uint32_t e = a.cstart[cell] + a.ccount[cell];
+            // FP32 sieve ahead of the FP64 centre test: the midpoint in the
+            // cell's frame, rounded once (error far below the 1e-4 slack)
+            const float fx = (float)(mx - sh[0]), fy = (float)(my - sh[1]), fz = (float)(mz - sh[2]);
+            const float fr = (float)(hs + av);
             // the lanes take the cell's rods 64 at a time
             for (uint32_t q0 = a.cstart[cell]; q0 < e && !screened; q0 += WAVE) {
               const uint32_t q = q0 + lane;
@@ -139,17 +143,22 @@ __global__ void __launch_bounds__(NT) k_nd_points(NodeArgs a) {
               if (q < e) {
                 const uint32_t kk = a.cid[q];
                 const float4 PK = a.cP[q], DK = a.cD[q];
-                const double R = av + (double)DK.w;
-                // conservative centre test, then the exact segment distance
-                const double cx = ((double)PK.x + sh[0]) - mx, cy = ((double)PK.y + sh[1]) - my,
-                             cz = ((double)PK.z + sh[2]) - mz;
-                const double lim = ((double)PK.w + hs + R) * 1.000000001;
-                if (kk != r && kk != j && (cx * cx + cy * cy) + cz * cz <= lim * lim) {
-                  const double d2 = seg_seg_d2(mx, my, mz, hx, hy, hz, (double)PK.x + sh[0],
-                                               (double)PK.y + sh[1], (double)PK.z + sh[2],
-                                               (double)PK.w * (double)DK.x, (double)PK.w * (double)DK.y,
-                                               (double)PK.w * (double)DK.z);
-                  hit = d2 < R * R;
+                const float gx = PK.x - fx, gy = PK.y - fy, gz = PK.z - fz;
+                const float gl = (PK.w + fr + DK.w) * 1.0001f + 1e-4f;
+                const bool near = fmaf(gz, gz, fmaf(gy, gy, gx * gx)) <= gl * gl;
+                if (near && kk != r && kk != j) {
+                  const double R = av + (double)DK.w;
+                  // conservative centre test, then the exact segment distance
+                  const double cx = ((double)PK.x + sh[0]) - mx, cy = ((double)PK.y + sh[1]) - my,
+                               cz = ((double)PK.z + sh[2]) - mz;
+                  const double lim = ((double)PK.w + hs + R) * 1.000000001;
+                  if ((cx * cx + cy * cy) + cz * cz <= lim * lim) {
+                    const double d2 = seg_seg_d2(mx, my, mz, hx, hy, hz, (double)PK.x + sh[0],
+                                                 (double)PK.y + sh[1], (double)PK.z + sh[2],
+                                                 (double)PK.w * (double)DK.x, (double)PK.w * (double)DK.y,
+                                                 (double)PK.w * (double)DK.z);
+                    hit = d2 < R * R;
+                  }
                 }
               }
               screened = __ballot(hit) != 0ull;
