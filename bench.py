@@ -143,19 +143,34 @@ def leg_main(a, out: dict) -> None:
 
 
 def leg_batch(a, out: dict) -> None:
+    """256 packings: batched FP32 + recheck, batched plain FP64 (the batching
+    gain alone) and the plain FP64 single-instance path one packing at a time.
+    Instances past the jamming limit of random sequential addition end at the
+    round cap (configs.CFG3_MAX_ROUNDS) or stall, identically on every path."""
+    import collections
+
     ps32 = configs.cfg3(nf.NF_PREC_FP32_RECHECK)
     ps64 = configs.cfg3(nf.NF_PREC_FP64)
     total = sum(p.n_rods for p in ps32)
-    log(f"batch: {len(ps32)} packings, {total} rods, batched FP32 + re-check")
-    with nf.Packing(ps32) as pk:
-        pk.generate()  # warm-up
-        t0 = time.perf_counter()
-        sts = pk.generate()
-        dt = time.perf_counter() - t0
-        rods32 = [pk.rods(i) for i in range(len(ps32))]
+
+    def batched(ps, label):
+        log(f"batch: {len(ps)} packings, {total} rods, batched {label}")
+        with nf.Packing(ps) as pk:
+            pk.generate()  # warm-up
+            t0 = time.perf_counter()
+            sts = pk.generate()
+            dt = time.perf_counter() - t0
+            rods = [pk.rods(i) for i in range(len(ps))]
+        acc = sum(s.n_accepted for s in sts)
+        log(f"batched {label}: {acc / dt:.4e} rods/s ({dt * 1e3:.1f} ms, "
+            f"max rounds {max(s.rounds for s in sts)})")
+        return dt, sts, rods
+
+    dt, sts, rods32 = batched(ps32, "FP32 + re-check")
+    dt64b, sts64b, rods64b = batched(ps64, "plain FP64")
     acc = sum(s.n_accepted for s in sts)
-    b_rate = acc / dt
-    log(f"batched: {b_rate:.4e} rods/s ({dt * 1e3:.1f} ms, max rounds {max(s.rounds for s in sts)})")
+    same_b = all(r.shape == q.shape and np.array_equal(r.view(np.uint32), q.view(np.uint32))
+                 for r, q in zip(rods32, rods64b))
     # the plain FP64 single-instance path, one packing at a time
     with nf.Packing(ps64[0]) as pk:
         pk.generate()  # warm-up
@@ -171,18 +186,24 @@ def leg_batch(a, out: dict) -> None:
         acc64 += s.n_accepted
         same = same and r.shape == rods32[i].shape and np.array_equal(r.view(np.uint32),
                                                                       rods32[i].view(np.uint32))
-    s_rate = acc64 / t_seq
+    b_rate, b64_rate, s_rate = acc / dt, acc / dt64b, acc64 / t_seq
     log(f"sequential FP64: {s_rate:.4e} rods/s ({t_seq * 1e3:.1f} ms); identical packings {same}")
+    status = collections.Counter(nf.STATUS[s.status] for s in sts)
     out["extra"]["batch"] = {
         "workload": f"{len(ps32)} packings, N log-uniform 2k-200k (sum {total}), aspect 20-500, "
-                    "vf 0.1-2% (capped at 1.5/aspect), half isotropic / half vMF kappa 1-20",
+                    "vf 0.1-2%, half isotropic / half vMF kappa 1-20 about random axes, "
+                    f"round cap {configs.CFG3_MAX_ROUNDS}",
         "batched_fp32_recheck_rods_per_s": b_rate, "batched_ms": dt * 1e3,
+        "batched_fp64_rods_per_s": b64_rate, "batched_fp64_ms": dt64b * 1e3,
         "sequential_fp64_rods_per_s": s_rate, "sequential_fp64_ms": t_seq * 1e3,
         "speedup_batched_over_fp64_single": b_rate / s_rate,
-        "identical_packings": bool(same), "accepted": acc, "target": total,
+        "speedup_batching_fp64": b64_rate / s_rate,
+        "speedup_precision_batched": b_rate / b64_rate,
+        "identical_packings": bool(same and same_b), "accepted": acc, "target": total,
         "pair_tests_per_s": sum(s.pair_tests for s in sts) / dt,
         "max_rounds": max(s.rounds for s in sts),
-        "statuses": sorted({nf.STATUS[s.status] for s in sts}),
+        "statuses": dict(status),
+        "jammed_instances": status.get("max_rounds", 0) + status.get("stalled", 0),
     }
 
 
