@@ -196,6 +196,16 @@ typedef struct nf_hist_spec {
 int nf_histograms(nf_handle* h, const nf_hist_spec* spec, int64_t* out_centre,
                   int64_t* out_polar, int64_t* out_mutual, int64_t* out_phitheta);
 
+/* nf_histograms without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, limits, errors and results as nf_histograms; its scratch
+ * (bin edges and counters, at most 64 KiB) lives in a static device array,
+ * one per GPU, held until the process ends.  Calls that overlap on one GPU
+ * (two threads, or two handles' streams) race on that array. */
+int nf_histograms_noalloc(nf_handle* h, const nf_hist_spec* spec, int64_t* out_centre,
+                          int64_t* out_polar, int64_t* out_mutual, int64_t* out_phitheta);
+
 /* Centre-centre distance statistics, statKernel_CENT (inc/statKernel_CENT.h:
  * 8-122) driven by collectStats (src/statistics_collection.cu:866-1033), on
  * instance 0: every unordered pair of distinct rods over all periodic images

@@ -245,6 +245,8 @@ _SIGS = [
                          C.POINTER(C.c_uint64)]),
     ("nf_histograms", C.c_int, [C.c_void_p, C.POINTER(HistSpec), C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p]),
+    ("nf_histograms_noalloc", C.c_int, [C.c_void_p, C.POINTER(HistSpec), C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p]),
     ("nf_network", C.c_int, [C.c_void_p, C.POINTER(NetworkInfo)]),
     ("nf_get_network", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_write_vtk", C.c_int, [C.c_void_p, C.c_char_p]),
@@ -438,7 +440,9 @@ class Packing:
                 "kernel_ms": {"bin": ms[0], "pairs": ms[1], "csr": ms[2], "sort": ms[3]}}
 
     def histograms(self, bins=64, centre_axis=0, pol_axis=(1.0, 0.0, 0.0), mutual=True,
-                   phi_bins=0, theta_bins=0) -> dict:
+                   phi_bins=0, theta_bins=0, noalloc: bool = False) -> dict:
+        """Exact int64 histograms; noalloc=True calls nf_histograms_noalloc
+        (scratch in the library's static device array, no allocation)."""
         spec = HistSpec()
         spec.bins = bins
         spec.centre_axis = centre_axis
@@ -450,9 +454,10 @@ class Packing:
         p = np.zeros(bins, np.int64)
         m = np.zeros(bins, np.int64) if mutual else None
         pt = np.zeros(max(phi_bins * theta_bins, 1), np.int64) if phi_bins else None
-        _check(lib().nf_histograms(self._h, C.byref(spec), c.ctypes.data, p.ctypes.data,
-                                   m.ctypes.data if m is not None else None,
-                                   pt.ctypes.data if pt is not None else None), self._h)
+        fn = lib().nf_histograms_noalloc if noalloc else lib().nf_histograms
+        _check(fn(self._h, C.byref(spec), c.ctypes.data, p.ctypes.data,
+                  m.ctypes.data if m is not None else None,
+                  pt.ctypes.data if pt is not None else None), self._h)
         out = {"centre": c, "polar": p}
         if m is not None:
             out["mutual"] = m

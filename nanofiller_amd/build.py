@@ -38,6 +38,7 @@ SOURCES = [
     "nf_state.cpp",
     "nf_conduct.cpp",
     "nf_nodes.cpp",
+    "nf_noalloc.hip",
 ]
 HEADERS = ["nf_common.h", "nf_internal.h", "nf_host.h"]
 ARCH = os.environ.get("NF_OFFLOAD_ARCH", "gfx950")

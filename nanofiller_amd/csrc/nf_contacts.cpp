@@ -305,68 +305,99 @@ extern "C" int nf_contacts_stats(nf_handle* h, uint64_t* pair_tests, uint64_t* r
 // ===========================================================================
 // histograms
 // ===========================================================================
-extern "C" int nf_histograms(nf_handle* h, const nf_hist_spec* spec, int64_t* out_centre,
-                             int64_t* out_polar, int64_t* out_mutual, int64_t* out_phitheta) {
+namespace nf {
+
+int hist_prepare(nf_handle* h, const nf_hist_spec* spec, bool want_mutual, bool want_2d,
+                 double* edges, HistPlan* plan) {
   if (!h || !spec) return fail(h, NF_E_INVALID, "null argument");
   const int nb = spec->bins, npb = spec->phi_bins, ntb = spec->theta_bins;
-  if (nb < 1 || nb > 256) return fail(h, NF_E_INVALID, "bins must be 1..256");
+  if (nb < 1 || nb > HIST_BINS_MAX) return fail(h, NF_E_INVALID, "bins must be 1..256");
   if (spec->centre_axis < 0 || spec->centre_axis > 2) return fail(h, NF_E_INVALID, "centre_axis must be 0..2");
-  const bool two_d = npb > 0 && out_phitheta;
-  if (two_d && (npb % 2 != 0 || ntb < 1 || npb * ntb > 4096))
+  const bool two_d = npb > 0 && want_2d;
+  if (two_d && (npb % 2 != 0 || ntb < 1 || npb * ntb > HIST_2D_MAX))
     return fail(h, NF_E_INVALID, "phi-theta needs even phi_bins and phi*theta <= 4096");
-  if (out_mutual && !h->cb.valid) return fail(h, NF_E_STATE, "mutual-angle histogram needs nf_contacts first");
+  if (want_mutual && !h->cb.valid) return fail(h, NF_E_STATE, "mutual-angle histogram needs nf_contacts first");
   const double PI = 3.14159265358979323846;
-  std::vector<double> ep(nb + 1), em(nb + 1), et;
-  for (int k = 0; k <= nb; ++k) {
-    ep[k] = std::cos((double)k * PI / (double)nb);
-    em[k] = std::cos((double)k * (0.5 * PI) / (double)nb);
-  }
+  // edges: polar [nb+1], mutual [nb+1], then theta [ntb+1] and phi [half+1]
+  size_t k0 = 0;
+  for (int k = 0; k <= nb; ++k) edges[k0 + k] = std::cos((double)k * PI / (double)nb);
+  k0 += (size_t)nb + 1;
+  for (int k = 0; k <= nb; ++k) edges[k0 + k] = std::cos((double)k * (0.5 * PI) / (double)nb);
+  k0 += (size_t)nb + 1;
   if (two_d) {
     const int half = npb / 2;
-    et.resize((size_t)ntb + 1 + half + 1);
-    for (int k = 0; k <= ntb; ++k) et[k] = std::cos((double)k * PI / (double)ntb);
-    for (int k = 0; k <= half; ++k) et[ntb + 1 + k] = std::cos((double)k * PI / (double)half);
+    for (int k = 0; k <= ntb; ++k) edges[k0 + k] = std::cos((double)k * PI / (double)ntb);
+    k0 += (size_t)ntb + 1;
+    for (int k = 0; k <= half; ++k) edges[k0 + k] = std::cos((double)k * PI / (double)half);
+    k0 += (size_t)half + 1;
   }
   double mu[3] = {spec->pol_axis[0], spec->pol_axis[1], spec->pol_axis[2]};
   const double mn = std::sqrt((mu[0] * mu[0] + mu[1] * mu[1]) + mu[2] * mu[2]);
   if (!(mn > 0.0)) return fail(h, NF_E_INVALID, "pol_axis must be non-zero");
-  for (int k = 0; k < 3; ++k) mu[k] = mu[k] / mn;
-  const size_t nout = 3 * (size_t)nb + (two_d ? (size_t)npb * ntb : 0);
-  double* dE = nullptr;
-  unsigned long long* dOut = nullptr;
-  int rc = dalloc(h, &dE, 2 * (size_t)(nb + 1) + et.size());
-  if (rc == NF_OK) rc = dalloc(h, &dOut, nout);
-  if (rc != NF_OK) {
-    dfree(h, dE);
-    return rc;
-  }
+  for (int k = 0; k < 3; ++k) plan->mu[k] = mu[k] / mn;
+  plan->nb = nb;
+  plan->npb = two_d ? npb : 0;
+  plan->ntb = two_d ? ntb : 0;
+  plan->axis = spec->centre_axis;
+  plan->mutual = want_mutual;
+  plan->n_edges = k0;
+  plan->n_out = 3 * (size_t)nb + (two_d ? (size_t)npb * ntb : 0);
+  return NF_OK;
+}
+
+int hist_run(nf_handle* h, const HistPlan& pl, const double* edges, double* dE,
+             unsigned long long* dOut, int64_t* out_centre, int64_t* out_polar, int64_t* out_mutual,
+             int64_t* out_phitheta) {
   hipStream_t s = h->stream;
-  std::vector<double> all(ep);
-  all.insert(all.end(), em.begin(), em.end());
-  all.insert(all.end(), et.begin(), et.end());
-  hipError_t e = hipMemcpyAsync(dE, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemsetAsync(dOut, 0, sizeof(unsigned long long) * nout, s);
+  const int nb = pl.nb;
+  hipError_t e = hipMemcpyAsync(dE, edges, sizeof(double) * pl.n_edges, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    fill_u32((uint32_t*)dOut, 0u, 2 * pl.n_out, s);
+    e = hipGetLastError();
+  }
   const InstDesc& d = h->desc[0];
   const uint32_t n = h->n_rods[0];
   if (e == hipSuccess && n) {
-    launch_hist(h->gb.bP + d.rod_off, h->gb.bD + d.rod_off, n,
-                (out_mutual && h->cb.valid) ? h->cb.rowptr : nullptr, h->cb.J, dE, dE + (nb + 1),
-                two_d ? dE + 2 * (nb + 1) : nullptr, nb, two_d ? npb : 0, two_d ? ntb : 0,
-                spec->centre_axis, h->params[0].box[spec->centre_axis], mu, dOut, s);
+    launch_hist(h->gb.bP + d.rod_off, h->gb.bD + d.rod_off, n, pl.mutual ? h->cb.rowptr : nullptr,
+                h->cb.J, dE, dE + (nb + 1), pl.npb ? dE + 2 * (nb + 1) : nullptr, nb, pl.npb, pl.ntb,
+                pl.axis, h->params[0].box[pl.axis], pl.mu, dOut, s);
     e = hipGetLastError();
   }
-  std::vector<int64_t> hout(nout);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(hout.data(), dOut, sizeof(int64_t) * nout, hipMemcpyDeviceToHost, s);
+  // int64 and u64 share the layout; counts never reach 2^63
+  const int64_t* src = (const int64_t*)dOut;
+  struct {
+    int64_t* dst;
+    size_t off, len;
+  } cp[4] = {{out_centre, 0, (size_t)nb},
+             {out_polar, (size_t)nb, (size_t)nb},
+             {pl.mutual ? out_mutual : nullptr, 2 * (size_t)nb, (size_t)nb},
+             {pl.npb ? out_phitheta : nullptr, 3 * (size_t)nb, (size_t)pl.npb * pl.ntb}};
+  for (auto& c : cp)
+    if (e == hipSuccess && c.dst)
+      e = hipMemcpyAsync(c.dst, src + c.off, sizeof(int64_t) * c.len, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(h, NF_E_HIP, std::string("histograms: ") + hipGetErrorString(e));
+  return NF_OK;
+}
+
+}  // namespace nf
+
+extern "C" int nf_histograms(nf_handle* h, const nf_hist_spec* spec, int64_t* out_centre,
+                             int64_t* out_polar, int64_t* out_mutual, int64_t* out_phitheta) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  double edges[HIST_EDGES_MAX];
+  HistPlan pl;
+  int rc = hist_prepare(h, spec, out_mutual != nullptr, out_phitheta != nullptr, edges, &pl);
+  if (rc != NF_OK) return rc;
+  double* dE = nullptr;
+  unsigned long long* dOut = nullptr;
+  rc = dalloc(h, &dE, pl.n_edges);
+  if (rc == NF_OK) rc = dalloc(h, &dOut, pl.n_out);
+  if (rc == NF_OK)
+    rc = hist_run(h, pl, edges, dE, dOut, out_centre, out_polar, out_mutual, out_phitheta);
   dfree(h, dE);
   dfree(h, dOut);
-  if (e != hipSuccess) return fail(h, NF_E_HIP, std::string("histograms: ") + hipGetErrorString(e));
-  if (out_centre) std::memcpy(out_centre, hout.data(), sizeof(int64_t) * nb);
-  if (out_polar) std::memcpy(out_polar, hout.data() + nb, sizeof(int64_t) * nb);
-  if (out_mutual) std::memcpy(out_mutual, hout.data() + 2 * nb, sizeof(int64_t) * nb);
-  if (two_d) std::memcpy(out_phitheta, hout.data() + 3 * nb, sizeof(int64_t) * npb * ntb);
-  return NF_OK;
+  return rc;
 }
 
 // Centre-centre distance statistics of instance 0 (statKernel_CENT via

@@ -54,6 +54,29 @@ void release_contacts(nf_handle* h);
 double default_cell_edge(double reach_typ, double density, double factor);
 std::vector<int4> build_stencil(const int32_t nc[3], const double box[3], double reach, bool half);
 
+// Histogram limits (nf_hist_spec): 1-D bins, phi x theta cells, and the
+// bin-edge table: polar and mutual edges (bins + 1 each), then the theta
+// (theta_bins + 1) and phi (phi_bins / 2 + 1) edges of the 2-D histogram.
+constexpr int HIST_BINS_MAX = 256;
+constexpr int HIST_2D_MAX = 4096;
+constexpr size_t HIST_EDGES_MAX = 2 * (HIST_BINS_MAX + 1) + HIST_2D_MAX / 2 + 3;
+constexpr size_t HIST_OUT_MAX = 3 * HIST_BINS_MAX + HIST_2D_MAX;
+
+struct HistPlan {
+  int nb, npb, ntb, axis;  // 1-D bins, 2-D phi/theta bins (0: no 2-D), centre axis
+  bool mutual;             // mutual-angle histogram over the last contact list
+  double mu[3];            // unit polar axis
+  size_t n_edges, n_out;   // edge-table and output lengths (doubles, u64 words)
+};
+// Validates a histogram request and fills the edge table (edges[HIST_EDGES_MAX]).
+int hist_prepare(nf_handle* h, const nf_hist_spec* spec, bool want_mutual, bool want_2d,
+                 double* edges, HistPlan* plan);
+// Uploads the edges to dE[n_edges], clears dOut[n_out], counts, and copies the
+// counts to the caller's host arrays (any may be NULL); waits for the stream.
+int hist_run(nf_handle* h, const HistPlan& pl, const double* edges, double* dE,
+             unsigned long long* dOut, int64_t* out_centre, int64_t* out_polar, int64_t* out_mutual,
+             int64_t* out_phitheta);
+
 template <class T>
 int dalloc(nf_handle* h, T** p, size_t count) {
   void* q = nullptr;
