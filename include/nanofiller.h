@@ -178,6 +178,15 @@ int nf_get_contacts(nf_handle* h, uint32_t* row_ptr, uint32_t* col, float* gap,
  * reference keeps only an FP32 distance (src/get_contact_data.cu:28-68). */
 int nf_get_contact_gap64(nf_handle* h, double* gap64, uint64_t cap);
 
+/* Contact (tunnelling) areas S of the same contacts, in the same order:
+ * get_contact_surface / get_surface_cyl2cyl (src/get_contact_data.cu:
+ * 130-217) with the area model of surf_to_cylinder_capped (inc/intersect/
+ * intersect_cylinder_capped.h:204-293): the larger of the point-contact disc
+ * pi dr^2, the crossing ellipse pi/2 dr (A+ + A-) and the near-parallel strip
+ * 2 dr |overlap|, dr the radius of the surface patch within the contact gap
+ * of the other rod.  FP64, computed on the device with the contact list. */
+int nf_get_contact_area(nf_handle* h, double* area, uint64_t cap);
+
 /* ---- histograms: statKernel_ANGL / statKernel_MUT / centre profile
  *      (src/statistics_collection.cu:640-861, inc/statKernel_ANGL.h:13-76,
  *      inc/statKernel_MUT.h:26-131) ---------------------------------------- */
@@ -251,6 +260,30 @@ int nf_network(nf_handle* h, nf_network_info* info);
  * row_ptr[N+3], col[nnz], val[nnz] (FP64).  Any pointer may be NULL. */
 int nf_get_network(nf_handle* h, uint32_t* labels, uint32_t* row_ptr,
                    uint32_t* col, double* val, uint64_t cap);
+
+/* Dead-end pruning, the `reduce` verb (src/NanoFiller.cpp:71-90) with
+ * reduce() (src/conductivity.cu:79-138) and contact_count_kernel_full /
+ * reductKernel (inc/conductKernels.h:40-162): rounds of simultaneous removal
+ * of every rod with fewer than min_cont contacts among the rods still
+ * present (the reference uses 2), until a round removes nothing.  Contacts
+ * are those of the last nf_contacts call, one per periodic image, without
+ * those crossing the electrode axis (params.elec_axis); a coupling to an
+ * electrode plane (the rule of nf_network) counts as one contact - the
+ * reference instead keeps every rod of its electrode cell layers.  The kept
+ * rods replace instance 0's packing in id order (new ids 0..n_after-1); the
+ * contact list and everything built on it are released, so nf_contacts
+ * must run again for the pruned packing. */
+typedef struct nf_reduce_info {
+  uint32_t n_before;  /* rods before pruning                                 */
+  uint32_t n_after;   /* rods kept                                           */
+  uint32_t rounds;    /* removal rounds, the last one (removing nothing) included,
+                         as the reference's iteration count                  */
+  uint32_t pad;
+  double device_ms;   /* device time of the rounds and the compaction        */
+} nf_reduce_info;
+int nf_reduce(nf_handle* h, int32_t min_cont, nf_reduce_info* info);
+/* Old id of every rod kept by the last nf_reduce (n_after entries). */
+int nf_get_reduce_kept(nf_handle* h, uint32_t* old_ids, uint64_t cap);
 
 /* DC solve of the network between the electrodes (source at potential 1,
  * sink at 0): replaces tools/kirchhoff/kirchhoff.py:256-272 (direct sparse

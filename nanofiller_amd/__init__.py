@@ -200,6 +200,21 @@ class NodeInfo(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
 
 
+class ReduceInfo(C.Structure):
+    """nf_reduce_info (include/nanofiller.h)."""
+
+    _fields_ = [
+        ("n_before", C.c_uint32),
+        ("n_after", C.c_uint32),
+        ("rounds", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("device_ms", C.c_double),
+    ]
+
+    def to_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
+
+
 class ConductanceInfo(C.Structure):
     """nf_conductance_info (include/nanofiller.h)."""
 
@@ -236,6 +251,7 @@ _SIGS = [
     ("nf_contacts", C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_uint64)]),
     ("nf_get_contacts", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_get_contact_gap64", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("nf_get_contact_area", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_contacts_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_uint64)]),
     ("nf_centre_distance", C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p,
@@ -251,6 +267,8 @@ _SIGS = [
     ("nf_get_network", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_write_vtk", C.c_int, [C.c_void_p, C.c_char_p]),
     ("nf_write_coo", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("nf_reduce", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ReduceInfo)]),
+    ("nf_get_reduce_kept", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_conductance", C.c_int, [C.c_void_p, C.c_double, C.c_uint32, C.POINTER(ConductanceInfo),
                                  C.c_void_p]),
     ("nf_read_density_mesh", C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
@@ -430,6 +448,12 @@ class Packing:
         _check(lib().nf_get_contact_gap64(self._h, out.ctypes.data, n), self._h)
         return out
 
+    def contact_area(self, n: int) -> np.ndarray:
+        """Contact areas S of the last contacts() list, in its order."""
+        out = np.zeros(n, dtype=np.float64)
+        _check(lib().nf_get_contact_area(self._h, out.ctypes.data, n), self._h)
+        return out
+
     def contact_stats(self) -> dict:
         t, r, m, tb = C.c_uint64(), C.c_uint64(), C.c_double(), C.c_uint64()
         ms = (C.c_float * 4)()
@@ -529,6 +553,15 @@ class Packing:
 
     def write_node_coo(self, prefix: str = "") -> None:
         _check(lib().nf_write_node_coo(self._h, prefix.encode()), self._h)
+
+    def reduce(self, min_cont: int = 2) -> tuple[dict, np.ndarray]:
+        """Dead-end pruning of the last contacts(): (info, old id of every
+        kept rod).  The kept rods become the packing; call contacts() again."""
+        info = ReduceInfo()
+        _check(lib().nf_reduce(self._h, int(min_cont), C.byref(info)), self._h)
+        ids = np.zeros(info.n_after, np.uint32)
+        _check(lib().nf_get_reduce_kept(self._h, ids.ctypes.data, info.n_after), self._h)
+        return info.to_dict(), ids
 
     def conductance(self, tol: float = 1e-10, max_iter: int = 20000, potentials: bool = False):
         """DC conductance between the electrodes of the last network():

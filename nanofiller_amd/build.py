@@ -39,6 +39,7 @@ SOURCES = [
     "nf_conduct.cpp",
     "nf_nodes.cpp",
     "nf_noalloc.hip",
+    "nf_reduce.hip",
 ]
 HEADERS = ["nf_common.h", "nf_internal.h", "nf_host.h"]
 ARCH = os.environ.get("NF_OFFLOAD_ARCH", "gfx950")

@@ -2,7 +2,12 @@
 // main (src/NanoFiller.cpp:48-149):
 //
 //   nanofiller params.txt generate [write_vtk] [stat_phitheta]
-//              [stat_mutangle] [stat_contacts] [stat_conduct d [internal]]
+//              [stat_mutangle] [stat_contacts] [reduce d]
+//              [stat_conduct d [internal]]
+//
+// d is the field direction as the reference's bit mask: 1 = x, 2 = y, 4 = z
+// (init_contacts_map takes 7 - d as the periodic directions,
+// src/conductivity.cu:52-58; expand uses axis d >> 1, src/simPrms.cpp:760).
 //
 // generate          insert/reject generation (cudaMain); prints the count and
 //                   the volume fraction reached
@@ -14,7 +19,12 @@
 // stat_contacts     contacts per rod: mean and maximum (collect_CntctPerInc)
 // stat_vdw          van der Waals energy of the contacting pairs (collect_vdw,
 //                   src/statistics_collection.cu:53-154)
-// stat_conduct d    contacts, network along axis d, COO files internal.srt.coo
+// reduce d          dead-end pruning along field direction d (reduce,
+//                   src/conductivity.cu:79-138): rods with fewer than 2
+//                   contacts are removed round by round; the pruned packing
+//                   is the one the later verbs see, and reduced.bin is
+//                   written when WRT_BIN is set (src/NanoFiller.cpp:71-90)
+// stat_conduct d    contacts, network along direction d, COO files internal.srt.coo
 //                   / source.elec.coo / sink.elec.coo (collect_cunduct_data,
 //                   src/conductivity.cu:140-423), the DC conductance between
 //                   the electrodes (the solve of tools/kirchhoff/kirchhoff.py:
@@ -25,8 +35,8 @@
 // As in the reference, generate saves output.bin when WRT_BIN is set
 // (save_state_BIN, src/stateSR.cpp:54-88), and without generate the rods
 // are read from output.bin (read_state_BIN, :90-134).  Verbs outside the
-// hot path (plot, expand, reduce, stat_nematic, write_ort, write_lmp,
-// stat_exclsurf) are reported as unsupported.
+// hot path (plot, stat_nematic, write_ort, write_lmp, stat_exclsurf) are
+// reported as unsupported.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -37,6 +47,12 @@
 #include <vector>
 
 #include "nanofiller.h"
+
+// Axis of a field-direction bit mask (1, 2, 4), or -1.
+static int dir_axis(const char* s) {
+  const int d = std::atoi(s);
+  return d == 1 ? 0 : (d == 2 ? 1 : (d == 4 ? 2 : -1));
+}
 
 static int die(nf_handle* h, const char* what, int rc) {
   std::fprintf(stderr, "nanofiller: %s failed (%d): %s\n", what, rc, nf_last_error(h));
@@ -73,7 +89,8 @@ int main(int argc, char** argv) {
   if (argc < 3) {
     std::fprintf(stderr,
                  "usage: %s params.txt generate [write_vtk] [stat_phitheta] [stat_mutangle]\n"
-                 "       [stat_contacts] [stat_vdw] [stat_conduct d [internal]]\n",
+                 "       [stat_contacts] [stat_vdw] [reduce d] [stat_conduct d [internal]]\n"
+                 "       d: field direction 1 (x), 2 (y) or 4 (z)\n",
                  argv[0]);
     return 2;
   }
@@ -203,20 +220,44 @@ int main(int argc, char** argv) {
       }
       std::printf("van der Waals energy over %llu contacting pairs (gap < %g um): %.10e\n",
                   (unsigned long long)np_, p.contact_gap, ev);
-    } else if (!std::strcmp(v, "stat_conduct")) {
-      if (ai + 1 >= argc) {
-        std::fprintf(stderr, "nanofiller: stat_conduct needs an axis 0|1|2\n");
+    } else if (!std::strcmp(v, "reduce")) {
+      const int dir = ai + 1 < argc ? dir_axis(argv[++ai]) : -1;
+      if (dir < 0) {
+        std::fprintf(stderr, "nanofiller: reduce needs a field direction 1, 2 or 4\n");
         status = 2;
         break;
       }
-      const int dir = std::atoi(argv[++ai]);
+      std::printf("Eliminating inclusions with less then 2 contacts up to %f um: \n", p.contact_gap);
+      nf_params q = p;
+      q.elec_axis = dir;
+      nf_handle* h2 = nullptr;
+      rc = nf_create(&q, &h2);
+      if (rc == NF_OK) rc = nf_set_rods(h2, 0, rods.data(), n);
+      uint64_t nc2 = 0;
+      if (rc == NF_OK) rc = nf_contacts(h2, q.contact_gap, &nc2);
+      nf_reduce_info ri;
+      if (rc == NF_OK) rc = nf_reduce(h2, 2, &ri);
+      if (rc == NF_OK) rc = nf_get_rods(h2, 0, rods.data(), n, &n);
+      if (rc != NF_OK) {
+        status = die(h2, "reduce", rc);
+      } else {
+        std::printf("Iterations: %u\nInclusions remained: %u \n", ri.rounds, ri.n_after);
+        // the pruned packing is the state of the later verbs
+        rc = nf_set_rods(h, 0, rods.data(), n);
+        have_contacts = false;
+        if (rc == NF_OK && p.write_state) rc = nf_save_state(h, "reduced.bin");
+        if (rc != NF_OK) status = die(h, "reduce", rc);
+      }
+      if (h2) nf_destroy(h2);
+    } else if (!std::strcmp(v, "stat_conduct")) {
+      const int dir = ai + 1 < argc ? dir_axis(argv[++ai]) : -1;
       bool internal = false;
       if (ai + 1 < argc && !std::strcmp(argv[ai + 1], "internal")) {
         internal = true;
         ++ai;
       }
-      if (dir < 0 || dir > 2) {
-        std::fprintf(stderr, "nanofiller: stat_conduct axis must be 0, 1 or 2\n");
+      if (dir < 0) {
+        std::fprintf(stderr, "nanofiller: stat_conduct needs a field direction 1, 2 or 4\n");
         status = 2;
         break;
       }

@@ -341,6 +341,118 @@ NF_HD double seg_seg_d2(double c1x, double c1y, double c1z, double h1x, double h
   return (vx * vx + vy * vy) + vz * vz;
 }
 
+// --------------------------------------------------------------------------
+// Contact (tunnelling) area S of a rod pair: the model of
+// surf_to_cylinder_capped (inc/intersect/intersect_cylinder_capped.h:204-293)
+// fed with the line geometry of distance_cylinder_capped (:69-147), as
+// get_surface_cyl2cyl computes it per contact (src/get_contact_data.cu:
+// 130-172), evaluated in FP64.  Rod 2 sits at its periodic image
+// (P2 + shift).  Directions are unit vectors, t in length units.
+//
+//  - Lines: closest points of the two infinite axes (t1, t2) and their
+//    distance D; axes within 1e-6 of parallel take t1 = +-l1/2 towards rod 2
+//    and t2 its projection (the reference's parallel branch).
+//  - Point contact: the disc of radius dr = min(dr1, dr2) with
+//    dr_i = a_i sqrt(1 - (h_i / a_i)^2), h1 = max(D - a2 - rlim, 0),
+//    h2 = max(D - a1 - rlim, 0) (dr_i = 0 when h_i > a_i); S = pi dr^2.
+//  - Crossing (both closest points at least a radius inside their rods): an
+//    ellipse of semi-axes dr and (A+ + A-)/2 along the bisector ax1 of c1 and
+//    c2' = c2 sign(c1.c2); A+- is the smallest of dr2 / sin(gamma/2) (the
+//    strip-overlap extent) and the distances along ax1 from the contact
+//    points to the rod ends on that side, at least dr:
+//    S = max(S, pi/2 dr (A+ + A-)).
+//  - Near-parallel (sin gamma <= max(a) / min(l)): the overlap of the two
+//    rods' projections on c1 times the strip width 2 dr, as the reference's
+//    end-point projections (a separation rather than an overlap when the
+//    projections do not overlap): S = max(S, |p2 - p1| 2 dr).
+//
+// Two terms differ from the reference on purpose: the rod-end distances are
+// measured from the contact point (the reference adds the rod's centre
+// position r1 / r2 to them, :257-260, which makes S depend on where in the
+// box the pair lies), and rod 2's contact parameter along c2' is t2 sign(c1.c2)
+// (the reference's copysign(t2, c1.c2) drops the sign of t2, :253).
+// 1 - d^2 is clamped at 0 before its square root.
+// --------------------------------------------------------------------------
+NF_HD double dmin2(double a, double b) { return b < a ? b : a; }
+NF_HD double dmax2(double a, double b) { return b > a ? b : a; }
+
+NF_HD double contact_area_f64(const float4 P1, const float4 D1, const float4 P2, const float4 D2,
+                              const double sx, const double sy, const double sz, const double rlim) {
+  const double PI = 3.14159265358979323846;
+  const double l1 = (double)P1.w * 2.0, l2 = (double)P2.w * 2.0;
+  const double a1 = (double)D1.w, a2 = (double)D2.w;
+  const double c1x = (double)D1.x, c1y = (double)D1.y, c1z = (double)D1.z;
+  const double c2x = (double)D2.x, c2y = (double)D2.y, c2z = (double)D2.z;
+  const double bx = ((double)P2.x - (double)P1.x) + sx;
+  const double by = ((double)P2.y - (double)P1.y) + sy;
+  const double bz = ((double)P2.z - (double)P1.z) + sz;
+  const double d = (c1x * c2x + c1y * c2y) + c1z * c2z;
+  const double bc1 = (bx * c1x + by * c1y) + bz * c1z;
+  const double bc2 = (bx * c2x + by * c2y) + bz * c2z;
+  double t1, t2;
+  if (1.0 - fabs(d) > 1e-6) {
+    const double den = d * d - 1.0;
+    t1 = (d * bc2 - bc1) / den;
+    t2 = (bc2 - d * bc1) / den;
+  } else {
+    t1 = bc1 < 0.0 ? -(l1 * 0.5) : l1 * 0.5;
+    t2 = d * t1 - bc2;
+  }
+  const double vx = (t1 * c1x - t2 * c2x) - bx;
+  const double vy = (t1 * c1y - t2 * c2y) - by;
+  const double vz = (t1 * c1z - t2 * c2z) - bz;
+  const double D = sqrt_cr((vx * vx + vy * vy) + vz * vz);
+  const double h1 = dmax2((D - a2) - rlim, 0.0), h2 = dmax2((D - a1) - rlim, 0.0);
+  double dr1 = 0.0, dr2 = 0.0;
+  if (h1 <= a1) {
+    const double q = h1 / a1;
+    dr1 = a1 * sqrt_cr(dmax2(1.0 - q * q, 0.0));
+  }
+  if (h2 <= a2) {
+    const double q = h2 / a2;
+    dr2 = a2 * sqrt_cr(dmax2(1.0 - q * q, 0.0));
+  }
+  const double dr = dmin2(dr1, dr2);
+  double S = (PI * dr) * dr;
+  const double sg = d < 0.0 ? -1.0 : 1.0;
+  const double dp = d * sg;  // c1 . c2'
+  const double sng = sqrt_cr(dmax2(1.0 - dp * dp, 0.0));
+  const double at1 = fabs(t1), at2 = fabs(t2);
+  if (at1 < l1 * 0.5 - a1 && at2 < l2 * 0.5 - a2) {
+    // bisector of c1 and c2'
+    double ax = c1x + sg * c2x, ay = c1y + sg * c2y, az = c1z + sg * c2z;
+    const double an = sqrt_cr((ax * ax + ay * ay) + az * az);
+    ax = ax / an;
+    ay = ay / an;
+    az = az / an;
+    const double csg = sqrt_cr(dmax2(1.0 - sng * sng, 0.0));
+    const double sngh = sqrt_cr((1.0 - csg) * 0.5);
+    const double t2p = sg * t2;
+    const double k1 = (c1x * ax + c1y * ay) + c1z * az;                   // c1 . ax1
+    const double k2 = ((sg * c2x) * ax + (sg * c2y) * ay) + (sg * c2z) * az;  // c2' . ax1
+    const double dt0 = sngh > 1e-6 ? dr2 / sngh : l1 * 0.5;
+    const double dt1p = (l1 * 0.5 - t1) * k1;
+    const double dt1m = fabs((-(l1 * 0.5) - t1) * k1);
+    const double dt2p = (l2 * 0.5 - t2p) * k2;
+    const double dt2m = fabs((-(l2 * 0.5) - t2p) * k2);
+    const double Ap = dmax2(dr, dmin2(dt0, dmin2(dt1p, dt2p)));
+    const double Am = dmax2(dr, dmin2(dt0, dmin2(dt1m, dt2m)));
+    S = dmax2(S, ((0.5 * PI) * dr) * (Ap + Am));
+  }
+  const double amax = dmax2(a1, a2), lmin = dmin2(l1, l2);
+  if (sng <= amax / lmin && (at1 <= l1 * 0.5 + a1 || at2 <= l2 * 0.5 + a2)) {
+    // rod ends projected on c1, relative to rod 1's centre
+    const double t1a = bc1 < 0.0 ? -(l1 * 0.5) : l1 * 0.5;
+    const double t2a = bc2 < 0.0 ? -(l2 * 0.5) : l2 * 0.5;
+    const double p1a = t1a, p1b = -t1a;  // (c1 t) . c1 with |c1| = 1
+    const double p2a = bc1 + t2a * d, p2b = bc1 - t2a * d;
+    const double p1 = bc1 > 0.0 ? dmin2(p1a, p2a) : dmax2(p1a, p2a);
+    const double p2 = bc1 > 0.0 ? dmax2(p1b, p2b) : dmin2(p1b, p2b);
+    S = dmax2(S, (fabs(p2 - p1) * 2.0) * dr);
+  }
+  return S;
+}
+
 // Deterministic atan from +, -, *, / and the correctly rounded sqrt: two
 // half-angle reductions atan(a) = 2 atan(a / (1 + sqrt(1 + a^2))) bring
 // |a| <= tan(pi/16), then the alternating series to a^35; |x| > 1 uses

@@ -1,10 +1,11 @@
 // nf_contacts.cpp - host orchestration of contact detection and histograms
 // on the packing of instance 0.
 //
-// Contacts: bin -> count pass -> scan -> record pass -> canonical row sort,
-// the two-pass scheme of get_contact_counts / get_contact_arrays
-// (src/get_contact_arrays.cu:12-140) with the counts kept per rod id so the
-// record pass writes a CSR over I directly.
+// Contacts: bin -> one pair pass that appends every contact to an unsorted
+// list -> CSR over I (count, scan, scatter) -> canonical row sort -> contact
+// areas.  This replaces the COUNT and RECORD passes of get_contact_counts /
+// get_contact_arrays (src/get_contact_arrays.cu:12-140); a pass whose list
+// overflows the append buffer is repeated once at the exact size.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -216,6 +217,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   if (rc == NF_OK) rc = dalloc(h, &c.gap, total);
   if (rc == NF_OK) rc = dalloc(h, &c.gap64, total);
   if (rc == NF_OK) rc = dalloc(h, &c.img, total);
+  if (rc == NF_OK) rc = dalloc(h, &c.area, total);
   if (rc != NF_OK) {
     free_all();
     return rc;
@@ -230,6 +232,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
                      c.img, s);
   hipEventRecord(ev[4], s);
   launch_row_sort(c.rowptr, n, c.J, c.gap, c.gap64, c.img, s);
+  launch_contact_area(P, D, n, c.rowptr, c.J, c.img, p.box, gap, c.area, s);
   hipEventRecord(ev[5], s);
   HIPCHK(h, hipMemcpyAsync(h->cb_stats, ctr, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s));
   HIPCHK(h, sync_stream(s, "contact list"));
@@ -281,6 +284,16 @@ extern "C" int nf_get_contact_gap64(nf_handle* h, double* gap64, uint64_t cap) {
   if (cap < c.n_contacts) return fail(h, NF_E_CAPACITY, "buffer too small");
   if (c.n_contacts && gap64)
     HIPCHK(h, hipMemcpy(gap64, c.gap64, sizeof(double) * c.n_contacts, hipMemcpyDeviceToHost));
+  return NF_OK;
+}
+
+extern "C" int nf_get_contact_area(nf_handle* h, double* area, uint64_t cap) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  ContactBuffers& c = h->cb;
+  if (!c.valid) return fail(h, NF_E_STATE, "no contacts computed");
+  if (cap < c.n_contacts) return fail(h, NF_E_CAPACITY, "buffer too small");
+  if (c.n_contacts && area)
+    HIPCHK(h, hipMemcpy(area, c.area, sizeof(double) * c.n_contacts, hipMemcpyDeviceToHost));
   return NF_OK;
 }
 

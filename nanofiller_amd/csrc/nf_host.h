@@ -35,6 +35,7 @@ struct nf_handle {
   float cb_ms[4] = {0, 0, 0, 0};                  // contact kernel times: bin, pair pass, CSR build, row sort
   uint64_t contact_hint = 0;                      // contacts found by the last call (append capacity)
   nf_network_info net_info = {};
+  std::vector<uint32_t> reduce_ids;  // old id of every rod kept by the last nf_reduce
   float net_ms = 0.0f;  // device time of the last nf_network call
 };
 

@@ -153,6 +153,7 @@ struct ContactBuffers {
   float* gap = nullptr;
   double* gap64 = nullptr;
   int32_t* img = nullptr;
+  double* area = nullptr;      // contact (tunnelling) area S of each contact
   uint64_t n_contacts = 0, cap_contacts = 0;
   uint32_t n_rods = 0;
   double gap_cut = 0.0;
@@ -259,6 +260,11 @@ void launch_cell_scatter(const float4* P, const float4* D, uint32_t n, const int
                          float4* oD, uint32_t* oid, hipStream_t s);
 void launch_row_sort(const uint32_t* rowptr, uint32_t n, uint32_t* J, float* gap, double* gap64,
                      int32_t* img, hipStream_t s);
+// Contact area S (contact_area_f64) of every contact of the CSR, rod J at
+// its image; rlim is the contact gap.
+void launch_contact_area(const float4* P, const float4* D, uint32_t n, const uint32_t* rowptr,
+                         const uint32_t* J, const int32_t* img, const double box[3], double rlim,
+                         double* area, hipStream_t s);
 // CSR over I from the unsorted contact list: per-row counts, then scatter
 // (rowptr from scan_u32 of the counts in between).
 void launch_csr_count(const uint32_t* I, uint64_t c, uint32_t* cnt, hipStream_t s);

@@ -328,9 +328,12 @@ extern "C" int nf_write_vtk(nf_handle* h, const char* path) {
 // crossing the electrode axis); source.elec.coo / sink.elec.coo hold the
 // rod-electrode couplings with J = 4294967295 (source) / 4294967294 (sink),
 // the SRC_ELEC / SNK_ELEC ids of inc/definitions.h:17-19.  D is the surface
-// gap; S is the point-contact tunnelling area pi*dr^2 of
-// surf_to_cylinder_capped's default branch
-// (inc/intersect/intersect_cylinder_capped.h:226-233).
+// gap.  S of a rod-rod contact is its contact area from the device
+// (nf_get_contact_area: point disc, crossing ellipse or parallel strip of
+// surf_to_cylinder_capped, inc/intersect/intersect_cylinder_capped.h:
+// 204-293); a rod-plane coupling has no second axis, so its S is the
+// point-contact disc pi*dr^2 of that model's default branch (:224-233) with
+// the rod's own radius on both sides.
 // ===========================================================================
 static double point_area(double gap, double a1, double a2, double rlim) {
   const double PI = 3.14159265358979323846;
@@ -359,6 +362,9 @@ extern "C" int nf_write_coo(nf_handle* h, const char* prefix) {
   uint32_t got = 0;
   rc = nf_get_rods(h, 0, rods.data(), n, &got);
   if (rc != NF_OK) return rc;
+  std::vector<double> area(C);
+  rc = nf_get_contact_area(h, area.data(), C);
+  if (rc != NF_OK) return rc;
   const int ax = h->params[0].elec_axis;
   const double rlim = c.gap_cut;
   {
@@ -369,8 +375,7 @@ extern "C" int nf_write_coo(nf_handle* h, const char* prefix) {
       for (uint32_t k = row[i]; k < row[i + 1]; ++k) {
         const int im = (int)(int8_t)((img[k] >> (8 * ax)) & 0xFF);
         if (im != 0) continue;
-        const double S = point_area(gap[k], rods[8 * (size_t)i + 7], rods[8 * (size_t)col[k] + 7], rlim);
-        f << "\t" << i << "\t" << col[k] << "\t" << gap[k] << "\t" << (float)S << "\n";
+        f << "\t" << i << "\t" << col[k] << "\t" << gap[k] << "\t" << (float)area[k] << "\n";
       }
   }
   // electrode couplings: planes at 0 and box[ax]
@@ -430,9 +435,9 @@ extern "C" int nf_write_node_coo(nf_handle* h, const char* prefix) {
   std::vector<float> gap(C);
   rc = nf_get_contacts(h, row.data(), col.data(), gap.data(), nullptr, C);
   if (rc != NF_OK) return rc;
-  std::vector<uint32_t> rowof(C);
-  for (uint32_t i = 0; i < n; ++i)
-    for (uint32_t k = row[i]; k < row[i + 1]; ++k) rowof[k] = i;
+  std::vector<double> area(C);
+  rc = nf_get_contact_area(h, area.data(), C);
+  if (rc != NF_OK) return rc;
   std::vector<float> rods((size_t)n * 8);
   uint32_t got = 0;
   rc = nf_get_rods(h, 0, rods.data(), n, &got);
@@ -451,8 +456,7 @@ extern "C" int nf_write_node_coo(nf_handle* h, const char* prefix) {
   for (uint64_t e = 0; e < ne; ++e) {
     const uint32_t k = ek[e];
     const double g = gap[k] > 0.0f ? (double)gap[k] : 0.0;
-    fi << "\t" << ea[e] << "\t" << eb[e] << "\t" << (float)g << "\t"
-       << (float)point_area(g, rods[8 * (size_t)rowof[k] + 7], rods[8 * (size_t)col[k] + 7], rlim) << "\n";
+    fi << "\t" << ea[e] << "\t" << eb[e] << "\t" << (float)g << "\t" << (float)area[k] << "\n";
   }
   fk << "#\tI\tJ\tData1\tData2\n";
   fs << "#\tI\tJ\tData1\tData2\n";

@@ -123,6 +123,34 @@ void launch_row_sort(const uint32_t* rowptr, uint32_t n, uint32_t* J, float* gap
                      gap64, img);
 }
 
+// Contact area of every contact, one thread per row (rows are short).
+__global__ void k_contact_area(const float4* __restrict__ P, const float4* __restrict__ D, uint32_t n,
+                               const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ J,
+                               const int32_t* __restrict__ img, double bx, double by, double bz,
+                               double rlim, double* __restrict__ area) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t b = rowptr[i], e = rowptr[i + 1];
+    if (b == e) continue;
+    const float4 P1 = P[i], D1 = D[i];
+    for (uint32_t k = b; k < e; ++k) {
+      const uint32_t j = J[k];
+      const int32_t m = img[k];
+      const double sx = (double)(int)(int8_t)(m & 0xFF) * bx;
+      const double sy = (double)(int)(int8_t)((m >> 8) & 0xFF) * by;
+      const double sz = (double)(int)(int8_t)((m >> 16) & 0xFF) * bz;
+      area[k] = contact_area_f64(P1, D1, P[j], D[j], sx, sy, sz, rlim);
+    }
+  }
+}
+
+void launch_contact_area(const float4* P, const float4* D, uint32_t n, const uint32_t* rowptr,
+                         const uint32_t* J, const int32_t* img, const double box[3], double rlim,
+                         double* area, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_contact_area, dim3(grid_for(n, 256)), dim3(256), 0, s, P, D, n, rowptr, J, img,
+                     box[0], box[1], box[2], rlim, area);
+}
+
 // CSR over I from the unsorted contact list (the pair pass appends in any
 // order; the row sort makes every row canonical afterwards).
 __global__ void k_csr_count(const uint32_t* __restrict__ I, uint64_t c, uint32_t* __restrict__ cnt) {

@@ -61,6 +61,9 @@ def lib() -> C.CDLL:
                                        C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_int]),
             ("orc_pair_d2", C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, P(C.c_int)]),
+            ("orc_contact_area", C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
+            ("orc_contact_areas", None, [P(Params), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_int64, C.c_double, C.c_void_p]),
             ("orc_pair_batch", None, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                                       C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
             ("orc_generate", C.c_int, [P(Params), C.c_void_p, P(GenStats), C.c_void_p, C.c_int]),
@@ -71,6 +74,9 @@ def lib() -> C.CDLL:
                                          C.c_int64, P(HistSpec), C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p]),
             ("orc_atan", C.c_double, [C.c_double]),
+            ("orc_reduce", C.c_uint32, [P(Params), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_int64, C.c_double, C.c_int32, C.c_void_p,
+                                        P(C.c_uint32)]),
             ("orc_vdw_pair", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                        C.c_double, P(C.c_double)]),
             ("orc_vdw", C.c_uint64, [P(Params), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
@@ -106,6 +112,25 @@ def pair_d2(ri, rj, shift, R) -> tuple[float, bool]:
     w = C.c_int()
     d2 = lib().orc_pair_d2(a.ctypes.data, b.ctypes.data, s.ctypes.data, float(R), C.byref(w))
     return d2, bool(w.value)
+
+
+def contact_area(ri, rj, shift, rlim) -> float:
+    """Contact area S of rods ri, rj (8 floats each), rj displaced by shift."""
+    a = np.ascontiguousarray(ri, np.float32)
+    b = np.ascontiguousarray(rj, np.float32)
+    s = np.ascontiguousarray(shift, np.float64)
+    return lib().orc_contact_area(a.ctypes.data, b.ctypes.data, s.ctypes.data, float(rlim))
+
+
+def contact_areas(p, rods, I, J, image, rlim) -> np.ndarray:
+    r = np.ascontiguousarray(rods, np.float32)
+    I = np.ascontiguousarray(I, np.uint32)
+    J = np.ascontiguousarray(J, np.uint32)
+    im = np.ascontiguousarray(image, np.int32)
+    out = np.zeros(len(I), np.float64)
+    lib().orc_contact_areas(C.byref(p), r.ctypes.data, I.ctypes.data, J.ctypes.data, im.ctypes.data,
+                            len(I), float(rlim), out.ctypes.data)
+    return out
 
 
 def pair_batch(ra, rb, shifts, contact: bool, sep_or_gap: float, soft_frac: float = 1.0):
@@ -217,6 +242,20 @@ def network(p, rods, I, J, gap64, image):
                       im.ctypes.data, len(I), labels.ctypes.data, row.ctypes.data, col.ctypes.data,
                       val.ctypes.data, nnz, C.byref(info))
     return info.to_dict(), {"labels": labels, "row_ptr": row, "col": col, "val": val}
+
+
+def reduce(p, rods, I, J, image, gcut, min_cont=2):
+    """Dead-end pruning of a contact list: (keep mask, rounds)."""
+    r = np.ascontiguousarray(rods, np.float32)
+    I = np.ascontiguousarray(I, np.uint32)
+    J = np.ascontiguousarray(J, np.uint32)
+    im = np.ascontiguousarray(image, np.int32)
+    keep = np.zeros(max(r.shape[0], 1), np.uint8)
+    rounds = C.c_uint32()
+    lib().orc_reduce(C.byref(p), r.ctypes.data, r.shape[0], I.ctypes.data, J.ctypes.data,
+                     im.ctypes.data, len(I), float(gcut), int(min_cont), keep.ctypes.data,
+                     C.byref(rounds))
+    return keep[: r.shape[0]].astype(bool), rounds.value
 
 
 def atan(x: float) -> float:

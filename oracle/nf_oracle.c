@@ -256,6 +256,112 @@ static double hard_core(const nf_params* p, const float* ri, const float* rj) {
   return p->soft_frac * ((double)ri[7] + (double)rj[7]) + p->sep;
 }
 
+/* Contact (tunnelling) area S of the pair (ri, rj + shift): the area model
+ * of surf_to_cylinder_capped (inc/intersect/intersect_cylinder_capped.h:
+ * 204-293) on the line geometry of distance_cylinder_capped (:69-147), as
+ * get_surface_cyl2cyl applies it per contact (src/get_contact_data.cu:
+ * 130-172), in FP64 with unit directions and t in length units:
+ *  - t1, t2: closest points of the infinite axes (:82-105; within 1e-6 of
+ *    parallel, t1 = +-l1/2 towards rod j and t2 its projection), D their
+ *    distance (Dtt.x);
+ *  - point disc (:224-233): h1 = max(D - a2 - rlim, 0), h2 = max(D - a1 -
+ *    rlim, 0), dr_i = a_i sqrt(1 - (h_i/a_i)^2) when h_i <= a_i, dr =
+ *    min(dr1, dr2), S = pi dr^2;
+ *  - crossing ellipse (:237-266) when |t1| < l1/2 - a1 and |t2| < l2/2 - a2:
+ *    A+- = max(dr, min(dr2 / sin(gamma/2), rod-end distances along the
+ *    bisector ax1 of c1 and c2' = c2 sign(c1.c2))), S = max(S, pi/2 dr (A+ +
+ *    A-)); the end distances are measured from the contact points, and rod
+ *    j's parameter along c2' is t2 sign(c1.c2) (the reference adds the rod
+ *    centres to the end distances, :257-260, and uses copysign(t2, c1.c2),
+ *    :253);
+ *  - near-parallel strip (:268-290) when sin gamma <= max(a)/min(l) and
+ *    |t1| <= l1/2 + a1 or |t2| <= l2/2 + a2: the rod ends projected on c1
+ *    (relative to rod i's centre), S = max(S, |p2 - p1| 2 dr). */
+static double dmn(double a, double b) { return b < a ? b : a; }
+static double dmx(double a, double b) { return b > a ? b : a; }
+
+double orc_contact_area(const float* ri, const float* rj, const double shift[3], double rlim) {
+  const double PI = 3.14159265358979323846;
+  const double l1 = (double)ri[6], l2 = (double)rj[6];
+  const double a1 = (double)ri[7], a2 = (double)rj[7];
+  const double c1[3] = {(double)ri[3], (double)ri[4], (double)ri[5]};
+  const double c2[3] = {(double)rj[3], (double)rj[4], (double)rj[5]};
+  double b[3];
+  for (int k = 0; k < 3; ++k) b[k] = ((double)rj[k] - (double)ri[k]) + shift[k];
+  const double d = (c1[0] * c2[0] + c1[1] * c2[1]) + c1[2] * c2[2];
+  const double bc1 = (b[0] * c1[0] + b[1] * c1[1]) + b[2] * c1[2];
+  const double bc2 = (b[0] * c2[0] + b[1] * c2[1]) + b[2] * c2[2];
+  double t1, t2;
+  if (1.0 - fabs(d) > 1e-6) {
+    const double den = d * d - 1.0;
+    t1 = (d * bc2 - bc1) / den;
+    t2 = (bc2 - d * bc1) / den;
+  } else {
+    t1 = bc1 < 0.0 ? -(l1 * 0.5) : l1 * 0.5;
+    t2 = d * t1 - bc2;
+  }
+  double v2 = 0.0;
+  double v[3];
+  for (int k = 0; k < 3; ++k) v[k] = (t1 * c1[k] - t2 * c2[k]) - b[k];
+  v2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+  const double D = sqrt(v2);
+  const double h1 = dmx((D - a2) - rlim, 0.0), h2 = dmx((D - a1) - rlim, 0.0);
+  double dr1 = 0.0, dr2 = 0.0;
+  if (h1 <= a1) {
+    const double q = h1 / a1;
+    dr1 = a1 * sqrt(dmx(1.0 - q * q, 0.0));
+  }
+  if (h2 <= a2) {
+    const double q = h2 / a2;
+    dr2 = a2 * sqrt(dmx(1.0 - q * q, 0.0));
+  }
+  const double dr = dmn(dr1, dr2);
+  double S = (PI * dr) * dr;
+  const double sg = d < 0.0 ? -1.0 : 1.0;
+  const double dp = d * sg;
+  const double sng = sqrt(dmx(1.0 - dp * dp, 0.0));
+  const double at1 = fabs(t1), at2 = fabs(t2);
+  if (at1 < l1 * 0.5 - a1 && at2 < l2 * 0.5 - a2) {
+    double ax[3] = {c1[0] + sg * c2[0], c1[1] + sg * c2[1], c1[2] + sg * c2[2]};
+    const double an = sqrt((ax[0] * ax[0] + ax[1] * ax[1]) + ax[2] * ax[2]);
+    for (int k = 0; k < 3; ++k) ax[k] = ax[k] / an;
+    const double csg = sqrt(dmx(1.0 - sng * sng, 0.0));
+    const double sngh = sqrt((1.0 - csg) * 0.5);
+    const double t2p = sg * t2;
+    const double k1 = (c1[0] * ax[0] + c1[1] * ax[1]) + c1[2] * ax[2];
+    const double k2 = ((sg * c2[0]) * ax[0] + (sg * c2[1]) * ax[1]) + (sg * c2[2]) * ax[2];
+    const double dt0 = sngh > 1e-6 ? dr2 / sngh : l1 * 0.5;
+    const double dt1p = (l1 * 0.5 - t1) * k1;
+    const double dt1m = fabs((-(l1 * 0.5) - t1) * k1);
+    const double dt2p = (l2 * 0.5 - t2p) * k2;
+    const double dt2m = fabs((-(l2 * 0.5) - t2p) * k2);
+    const double Ap = dmx(dr, dmn(dt0, dmn(dt1p, dt2p)));
+    const double Am = dmx(dr, dmn(dt0, dmn(dt1m, dt2m)));
+    S = dmx(S, ((0.5 * PI) * dr) * (Ap + Am));
+  }
+  if (sng <= dmx(a1, a2) / dmn(l1, l2) && (at1 <= l1 * 0.5 + a1 || at2 <= l2 * 0.5 + a2)) {
+    const double t1a = bc1 < 0.0 ? -(l1 * 0.5) : l1 * 0.5;
+    const double t2a = bc2 < 0.0 ? -(l2 * 0.5) : l2 * 0.5;
+    const double p1a = t1a, p1b = -t1a;
+    const double p2a = bc1 + t2a * d, p2b = bc1 - t2a * d;
+    const double p1 = bc1 > 0.0 ? dmn(p1a, p2a) : dmx(p1a, p2a);
+    const double p2 = bc1 > 0.0 ? dmx(p1b, p2b) : dmn(p1b, p2b);
+    S = dmx(S, (fabs(p2 - p1) * 2.0) * dr);
+  }
+  return S;
+}
+
+/* Contact areas of a contact list (image packs J's shift as signed bytes). */
+void orc_contact_areas(const nf_params* p, const float* rods, const uint32_t* I, const uint32_t* J,
+                       const int32_t* image, int64_t nc, double rlim, double* out) {
+  for (int64_t k = 0; k < nc; ++k) {
+    double sh[3];
+    for (int a = 0; a < 3; ++a)
+      sh[a] = (double)(int)(int8_t)((image[k] >> (8 * a)) & 0xFF) * p->box[a];
+    out[k] = orc_contact_area(&rods[8 * (size_t)I[k]], &rods[8 * (size_t)J[k]], sh, rlim);
+  }
+}
+
 /* ===================================================================== */
 /* Candidate sampling                                                      */
 /* ===================================================================== */
@@ -948,6 +1054,58 @@ static double blocked_sum(const double* v, size_t n) {
     tot += s;
   }
   return tot;
+}
+
+/* Dead-end pruning: the `reduce` verb (src/NanoFiller.cpp:71-90) driving
+ * reduce() (src/conductivity.cu:79-138): each iteration recounts every
+ * present rod's contacts among the present rods (contact_count_kernel_full,
+ * inc/conductKernels.h:40-113) and removes, all at once, the rods whose
+ * count is below min_cont (reductKernel, :116-162); iterations repeat until
+ * one removes nothing (counted, as the reference's iter).  Contacts are
+ * the list entries whose image does not cross the electrode axis; a rod's
+ * coupling to an electrode plane (gap to the plane <= gcut, the rule of
+ * orc_network) counts as one contact, where the reference keeps the rods
+ * of its electrode cell layers.  keep[i] = 1 for the rods kept; returns
+ * their number. */
+uint32_t orc_reduce(const nf_params* p, const float* rods, uint32_t n, const uint32_t* I,
+                    const uint32_t* J, const int32_t* image, int64_t nc, double gcut,
+                    int32_t min_cont, uint8_t* keep, uint32_t* rounds) {
+  const int ax = p->elec_axis;
+  const double L = p->box[ax];
+  uint32_t* coup = (uint32_t*)calloc((size_t)n + 1, sizeof(uint32_t));
+  uint32_t* cnt = (uint32_t*)calloc((size_t)n + 1, sizeof(uint32_t));
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* r = &rods[8 * (size_t)i];
+    const double ext = fabs(((double)r[6] * 0.5) * (double)r[3 + ax]);
+    const double pos = (double)r[ax];
+    coup[i] = (((pos - ext) - (double)r[7]) <= gcut ? 1u : 0u) +
+              (((L - (pos + ext)) - (double)r[7]) <= gcut ? 1u : 0u);
+    keep[i] = 1;
+  }
+  uint32_t it = 0, present = n;
+  for (;;) {
+    ++it;
+    for (uint32_t i = 0; i < n; ++i) cnt[i] = coup[i];
+    for (int64_t k = 0; k < nc; ++k) {
+      int img[3];
+      unpack_img(image[k], img);
+      if (img[ax] != 0 || !keep[I[k]] || !keep[J[k]]) continue;
+      cnt[I[k]]++;
+      cnt[J[k]]++;
+    }
+    uint32_t removed = 0;
+    for (uint32_t i = 0; i < n; ++i)
+      if (keep[i] && cnt[i] < (uint32_t)min_cont) {
+        keep[i] = 0;
+        ++removed;
+      }
+    present -= removed;
+    if (removed == 0) break;
+  }
+  free(coup);
+  free(cnt);
+  *rounds = it;
+  return present;
 }
 
 /* Network extraction.  The reference's collect_cunduct_data

@@ -55,6 +55,12 @@ int orc_candidate(const nf_params* p, const double* agg, uint32_t round, uint32_
 double orc_pair_d2(const float* ri, const float* rj, const double shift[3], double R,
                    int* within);
 
+/* Contact area S of capsules ri and rj (rj displaced by shift) with contact
+ * gap rlim, and of a contact list (I, J, image as in orc_contacts). */
+double orc_contact_area(const float* ri, const float* rj, const double shift[3], double rlim);
+void orc_contact_areas(const nf_params* p, const float* rods, const uint32_t* I, const uint32_t* J,
+                       const int32_t* image, int64_t nc, double rlim, double* out);
+
 /* Batch of independent pair evaluations (tests). */
 void orc_pair_batch(int n, const float* ra, const float* rb, const double* sh, int contact,
                     double sep_or_gap, double soft_frac, double* d2out, int* within_out,
@@ -94,6 +100,12 @@ int64_t orc_network(const nf_params* p, const float* rods, uint32_t n,
                     const int32_t* image, int64_t nc, uint32_t* labels,
                     uint32_t* row_ptr, uint32_t* col, double* val, int64_t cap,
                     nf_network_info* info);
+
+/* Dead-end pruning (reduce): keep[i] = 1 for the kept rods; returns their
+ * number, *rounds the iterations including the last, empty one. */
+uint32_t orc_reduce(const nf_params* p, const float* rods, uint32_t n, const uint32_t* I,
+                    const uint32_t* J, const int32_t* image, int64_t nc, double gcut,
+                    int32_t min_cont, uint8_t* keep, uint32_t* rounds);
 
 /* Deterministic atan (+, -, *, / and the correctly rounded sqrt). */
 double orc_atan(double x);

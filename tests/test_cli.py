@@ -36,7 +36,7 @@ def test_cli_reports_missing_device(tmp_path):
 @pytest.mark.gpu
 def test_cli_end_to_end(tmp_path):
     r = _run(tmp_path, "generate", "write_vtk", "stat_phitheta", "stat_mutangle", "stat_contacts", "stat_vdw",
-             "stat_conduct", "0", "internal")
+             "stat_conduct", "1", "internal")
     assert r.returncode == 0, r.stderr
     assert "2000 inclusions generated!" in r.stdout
     for f in ("output.vtk", "rho.dat", "gamma.dat", "internal.srt.coo", "source.elec.coo",
@@ -62,3 +62,31 @@ def test_cli_end_to_end(tmp_path):
     assert f"contacts (gap <= 0.01 um): {c.n};" in r.stdout
     assert "effective conductance" in r.stdout
     assert "node network:" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cli_reduce(tmp_path):
+    """reduce 1: dead-end pruning along x, reduced.bin with WRT_BIN; the
+    pruned packing equals the library's nf_reduce on the same rods."""
+    (tmp_path / "params.txt").write_text(PARAMS.replace("STAT_DIST_LIM 0.01", "STAT_DIST_LIM 0.05")
+                                         + "WRT_BIN 1\n")
+    r = subprocess.run([CLI, "params.txt", "generate", "reduce", "1", "stat_contacts"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    p = nf.normalize(nf.load_params(str(tmp_path / "params.txt")))
+    p.elec_axis = 0
+    with nf.Packing(p) as pk:
+        pk.generate()
+        pk.contacts(p.contact_gap)
+        info, _ = pk.reduce(2)
+        pruned = pk.rods()
+    assert f"Inclusions remained: {info['n_after']} " in r.stdout
+    assert 0 < info["n_after"] < 2000
+    hd = nf.read_state_header(str(tmp_path / "reduced.bin"))
+    assert hd["n_rods"] == info["n_after"]
+    with nf.Packing(p) as pk:
+        assert pk.load_state(str(tmp_path / "reduced.bin")) == info["n_after"]
+        back = pk.rods()
+    # the state file stores rods by cell: the same set
+    key = lambda a: np.lexsort(a.view(np.uint32).T[::-1])
+    np.testing.assert_array_equal(back[key(back)].view(np.uint32), pruned[key(pruned)].view(np.uint32))
