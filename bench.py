@@ -81,6 +81,17 @@ def timed_generation(p, steps: int, warmup: int, options=None):
     return dt, sts, rods, cont
 
 
+def profiled_generation(p) -> dict:
+    """One generation with HIP events per kernel class: the two pair passes'
+    times and streamed bytes (probes and neighbour tiles)."""
+    with nf.Packing(p, {nf.NF_OPT_PROFILE: 1}) as pk:
+        sp = pk.generate()[0]
+    return {"wall_ms": sp.wall_ms, "rounds": sp.rounds, "launches": sp.rounds_launched,
+            "kernel_ms": {k: sp.kern_ms[i] for i, k in enumerate(nf.KERNEL_CLASSES)},
+            "reject_old_tile_bytes": sp.tile_bytes, "reject_new_tile_bytes": sp.tile_bytes_new,
+            "pair_tests": sp.pair_tests}
+
+
 def same_contacts(a, b) -> bool:
     return (np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col, b.col)
             and np.array_equal(a.image, b.image) and np.array_equal(a.gap.view(np.uint32),
@@ -121,6 +132,17 @@ def leg_main(a, out: dict) -> None:
         "profiled_kernel_ms": {k: sp.kern_ms[i] for i, k in enumerate(nf.KERNEL_CLASSES)},
         "profiled_wall_ms": sp.wall_ms,
     }
+    # FP32-vs-FP64 distance error: one generation that also evaluates every
+    # FP32-decided pair in FP64 (NF_OPT_DIAG); its packing must not change
+    with nf.Packing(p32, {nf.NF_OPT_DIAG: 1}) as pk:
+        sd = pk.generate()[0]
+        rods_diag = pk.rods()
+    out["extra"]["fp32_recheck"]["max_abs_d32_d64_um"] = sd.max_abs_d32_d64
+    out["extra"]["fp32_recheck"]["diag_identical_packing"] = bool(
+        rods_diag.shape == rods32.shape and np.array_equal(rods_diag.view(np.uint32), rods32.view(np.uint32)))
+    log(f"diag: max |d32 - d64| = {sd.max_abs_d32_d64:.3e} um")
+    out["extra"]["chi2"] = {"500k_lognormal_agglomerates_vmf_z5": chi2_entry(p32, rods32, 0, (0, 0, 1))}
+    leg_cfg1(a, out)
     if a.no_fp64:
         return
     p64 = configs.cfg2(nf.NF_PREC_FP64)
@@ -140,6 +162,41 @@ def leg_main(a, out: dict) -> None:
         "contacts": int(c32.n),
     }
     log(f"FP64: {rods64_s:.4e} rods/s; identical packing {same_rods}, contacts {same_c}")
+
+
+def chi2_entry(p, rods, axis, pol_axis) -> dict:
+    """chi-square of the packing's centre profile along `axis` and polar
+    angles about `pol_axis` against the target laws (64 bins)."""
+    from nanofiller_amd import targets
+
+    t = targets.packing_tests(p, rods, axis, pol_axis, bins=64)
+    return {"rods": int(len(rods)), "density_axis": "xyz"[axis],
+            "density": t["density"], "polar_axis": list(pol_axis), "polar": t["polar"]}
+
+
+def leg_cfg1(a, out: dict) -> None:
+    """20,000 rods, 5 um box, 1 um cells, seed 1: the plain FP64 GPU path and
+    the FP32 + recheck path on the same packing."""
+    res = {}
+    rods = {}
+    for name, prec in (("fp64_plain", nf.NF_PREC_FP64), ("fp32_recheck", nf.NF_PREC_FP32_RECHECK)):
+        p = configs.cfg1(prec)
+        dt, sts, r, c = timed_generation(p, a.steps, a.warmup)
+        rods[name] = (r, c)
+        res[name] = {"accepted_rods_per_s": sts[-1].n_accepted * a.steps / dt,
+                     "pair_tests_per_s": sum(s.pair_tests for s in sts) / dt,
+                     "ms_per_step": dt / a.steps * 1e3, "rounds": sts[-1].rounds,
+                     "pair_tests": sts[-1].pair_tests, "contacts": int(c.n)}
+    (r64, c64), (r32, c32) = rods["fp64_plain"], rods["fp32_recheck"]
+    res["identical_packing"] = bool(np.array_equal(r64.view(np.uint32), r32.view(np.uint32)))
+    res["identical_contact_lists"] = bool(same_contacts(c64, c32))
+    res["workload"] = ("20,000 rods, l 1 um, d 10 nm, 5 um periodic cube, uniform, isotropic, seed 1, "
+                       "1 um cells, contacts <= 2 nm")
+    out["extra"]["cfg1"] = res
+    out["extra"].setdefault("chi2", {})["20k_uniform_isotropic"] = chi2_entry(configs.cfg1(), r64, 0,
+                                                                             (0, 0, 1))
+    log(f"20k rods: FP64 {res['fp64_plain']['accepted_rods_per_s']:.4e} rods/s, "
+        f"FP32 {res['fp32_recheck']['accepted_rods_per_s']:.4e} rods/s")
 
 
 def leg_batch(a, out: dict) -> None:
@@ -222,14 +279,24 @@ def leg_contacts(a, out: dict) -> None:
         t0 = time.perf_counter()
         h = pk.histograms(bins=64, centre_axis=0, pol_axis=(1, 0, 0))
         t_h = time.perf_counter() - t0
+        rods = pk.rods()
+        pk.set_option(nf.NF_OPT_DIAG, 1)
+        pk.contacts(p.contact_gap)
+        dd_contacts = pk.contact_stats()["max_abs_d32_d64"]
+        pk.set_option(nf.NF_OPT_DIAG, 0)
         pk.set_option(nf.NF_OPT_CONTACT_PREC, nf.NF_PREC_FP64)
         t0 = time.perf_counter()
         c64 = pk.contacts(p.contact_gap)
         t_c64 = time.perf_counter() - t0
         cs64 = pk.contact_stats()
+    gen_prof = profiled_generation(p)
     km = cs["kernel_ms"]
     bytes_pass = cs["tile_bytes"]  # probe and neighbour bytes streamed by the pair pass
     achieved = bytes_pass / (km["pairs"] * 1e-3) / 1e9 if km["pairs"] > 0 else 0.0
+    # the algorithmic bytes of the contact pass: N * 32 B * 14 tiles (27-cell
+    # stencil with Newton's third law) + C * 12 B (I, J, gap)
+    alg_bytes = st.n_accepted * 32 * 14 + c32.n * 12
+    alg_achieved = alg_bytes / (km["pairs"] * 1e-3) / 1e9 if km["pairs"] > 0 else 0.0
     log(f"contacts: {c32.n} pairs, pair pass {km['pairs']:.3f} ms, CSR {km['csr']:.3f} ms, "
         f"{achieved:.0f} GB/s; FP64 pair pass {cs64['kernel_ms']['pairs']:.3f} ms")
     out["extra"]["contacts"] = {
@@ -237,14 +304,19 @@ def leg_contacts(a, out: dict) -> None:
         "generate_ms": t_gen * 1e3, "generate_rods_per_s": st.n_accepted / t_gen,
         "contacts": int(c32.n), "contacts_call_ms": t_c * 1e3, "kernel_ms": km,
         "pair_tests": cs["pair_tests"], "rechecks": cs["rechecks"],
-        "pair_kernel_roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                                  "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                                  "tile_bytes_per_pass": bytes_pass},
+        "pair_kernel_roofline": {"bound": "hbm", "achieved": alg_achieved, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": alg_achieved / HBM_PEAK_GBS,
+                                  "algorithmic_bytes_per_pass": alg_bytes,
+                                  "tile_bytes_per_pass": bytes_pass,
+                                  "tile_bytes_gbs": achieved},
+        "max_abs_d32_d64_um": dd_contacts,
+        "generation_profile": gen_prof,
         "fp64_contacts_call_ms": t_c64 * 1e3, "fp64_kernel_ms": cs64["kernel_ms"],
         "identical_fp32_fp64_lists": bool(same_contacts(c32, c64)),
         "histograms_ms": t_h * 1e3,
         "histogram_totals": {k: int(v.sum()) for k, v in h.items()},
     }
+    out["extra"].setdefault("chi2", {})["4M_uniform_vmf_x2"] = chi2_entry(p, rods, 0, (1, 0, 0))
 
 
 def leg_network(a, out: dict) -> None:
@@ -265,6 +337,7 @@ def leg_network(a, out: dict) -> None:
         t0 = time.perf_counter()
         ninfo, _ = pk.node_network()
         t_nd = time.perf_counter() - t0
+    gen_prof = profiled_generation(p)
     log(f"network: {c.n} contacts, {info['n_components']} components, largest {info['largest']}, "
         f"nnz {info['nnz']}, {t_n * 1e3:.1f} ms")
     log(f"network: node level {ninfo['n_nodes']} nodes, {ninfo['n_segments']} segments, "
@@ -275,7 +348,7 @@ def leg_network(a, out: dict) -> None:
                     "gap <= 2 nm, electrodes x = 0 / 40 um, g0 exp(-gap / 0.1 nm)",
         "generate_ms": t_gen * 1e3, "generate_rods_per_s": st.n_accepted / t_gen,
         "rounds": st.rounds, "contacts": int(c.n), "contacts_call_ms": t_c * 1e3,
-        "network_call_ms": t_n * 1e3, "info": info,
+        "network_call_ms": t_n * 1e3, "info": info, "generation_profile": gen_prof,
         "node_network": {"info": ninfo, "build_device_ms": ninfo["device_ms"],
                          "call_with_host_copies_ms": t_nd * 1e3},
     }
@@ -317,19 +390,51 @@ def leg_conduct(a, out: dict) -> None:
     }
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(threads: int):
-    """Oracle (FP64 C restatement) on a bounded sample: the same laws at 1/8
-    of the volume and the same rod density (62,500 rods in a 10 um cube)."""
+    """The oracle (FP64 C restatement of the generation: same cell list,
+    round rule and exact capsule distance; OpenMP over cells) on bounded
+    samples of the 20k-rod and 500k-rod workloads at the same rod density
+    and laws, each the median of 5 runs, on 1 thread and on `threads`."""
     import oracle
 
-    p = configs.cfg2(nf.NF_PREC_FP64, box=(10, 10, 10), n_rods=62500)
-    t0 = time.perf_counter()
-    rods, st, _ = oracle.generate(p, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": st["n_accepted"] / dt, "unit": "rods/s", "cores": threads, "kind": "port",
-            "sample": "oracle generation of 62,500 rods in a 10 um cube (same laws and density, "
-                      f"1/8 of the workload volume) in {dt:.2f} s; "
-                      f"{st['pair_tests'] / dt:.3e} pair-tests/s"}
+    samples = {
+        "20k_fp64": {threads: configs.cfg1(nf.NF_PREC_FP64),
+                     1: configs.cfg1(nf.NF_PREC_FP64, box=(2.5, 2.5, 2.5), n_rods=2500)},
+        "500k_lognormal_agglomerates": {
+            threads: configs.cfg2(nf.NF_PREC_FP64, box=(5, 5, 5), n_rods=7812),
+            1: configs.cfg2(nf.NF_PREC_FP64, box=(20 / 6, 20 / 6, 20 / 6), n_rods=2315)},
+    }
+    runs = {}
+    for name, by_t in samples.items():
+        for t, p in by_t.items():
+            rates, prates = [], []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                _, st, _ = oracle.generate(p, threads=t)
+                dt = time.perf_counter() - t0
+                rates.append(st["n_accepted"] / dt)
+                prates.append(st["pair_tests"] / dt)
+            runs[f"{name}_{t}t"] = {"threads": t, "rods": int(p.n_rods), "box_um": p.box[0],
+                                    "accepted_rods_per_s": float(np.median(rates)),
+                                    "pair_tests_per_s": float(np.median(prates))}
+            log(f"CPU {name} on {t} threads: {runs[f'{name}_{t}t']['accepted_rods_per_s']:.4e} rods/s")
+    main = runs[f"500k_lognormal_agglomerates_{threads}t"]
+    return {"value": main["accepted_rods_per_s"], "unit": "rods/s", "cores": threads, "kind": "port",
+            "sample": ("oracle generation of 7,812 rods in a 5 um cube (the 500k-rod workload's laws "
+                       "and rod density at 1/64 of its volume), median of 5; 1-thread and 20k-rod "
+                       "samples under 'runs'"),
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "runs": runs}
 
 
 def main() -> int:

@@ -130,6 +130,15 @@ int nf_params_load(const char* path, nf_params* p);
  * (src/simPrms.cpp:987-1002), float-exact box edges, batch bounds. */
 int nf_params_normalize(nf_params* p);
 
+/* Centres of the Gaussian agglomerates of the density law (NF_DEN_AGGLOM):
+ * 3 doubles each, as the generator places them (a pure function of seed,
+ * stream and box).  Returns the count (0 for other laws), or a negative
+ * status; out may be NULL to ask for the count.  The density is then
+ * rho(x) = 1 + (agg_peak - 1) exp(-d^2 / (2 agg_sigma^2)), d the periodic
+ * distance to the nearest centre, replacing the reference's density texture
+ * (src/den2tex.h:5-53) for analytic profiles. */
+int nf_agglomerate_centres(const nf_params* p, double* out, int32_t cap);
+
 /* ---- lifecycle ---------------------------------------------------------- */
 
 /* One packing.  Replaces the simPrms set-up in cudaMain (src/cudaMain.cu:61-97). */
@@ -413,11 +422,24 @@ enum {
   NF_OPT_PROFILE = 2,      /* 1: time each kernel class with HIP events (no graph)   */
   NF_OPT_DIAG = 3,         /* 1: also compute FP64 for FP32-decided pairs and report
                               max |d32 - d64|                                         */
-  NF_OPT_CONTACT_PREC = 4  /* NF_PREC_* for nf_contacts (-1: the params' precision)   */
+  NF_OPT_CONTACT_PREC = 4, /* NF_PREC_* for nf_contacts (-1: the params' precision)   */
+  NF_OPT_CONTACT_ENGINE = 5, /* NF_ENGINE_* candidate pairs of nf_contacts (default cells) */
+  NF_OPT_FOOTPRINT_CELL = 6  /* footprint engine: cell edge in 1/1000 of the typical rod
+                                length (0: default 250)                                  */
 };
+/* Candidate pairs of nf_contacts.  Both give the same contact list.
+ * FOOTPRINT: each rod binned into the fine cells its capsule (radius
+ * a + gap/2) can touch; rods sharing a cell are tested, each contact
+ * reported by the cell holding the midpoint of its closest points.
+ * CELLS: the cell-pair engine of the generator over a centre cell list; its
+ * capsule-test count is the number of pairs within centre reach (the
+ * reference's stencil count). */
+enum { NF_ENGINE_FOOTPRINT = 0, NF_ENGINE_CELLS = 1 };
 int nf_set_option(nf_handle* h, int key, int64_t value);
 
-/* Statistics of the last nf_contacts call: capsule tests on pairs in reach,
+/* Statistics of the last nf_contacts call: capsule tests on pairs in reach
+ * (with the footprint engine: on the candidate pairs within centre reach, a
+ * pair counted once per cell the two rods share),
  * FP64 rechecks, max |d32-d64| (NF_OPT_DIAG), HIP-event times in ms of the
  * four contact stages (binning, the pair pass that finds and appends every
  * contact, CSR build, row sort; kernel_ms[4]) and the probe + tile bytes the

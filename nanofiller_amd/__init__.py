@@ -31,6 +31,8 @@ NF_PREC_FP64, NF_PREC_FP32_RECHECK = 0, 1
 NF_STATUS_RUNNING, NF_STATUS_TARGET, NF_STATUS_MAX_ROUNDS, NF_STATUS_STALLED, NF_STATUS_OVERFLOW = 0, 1, 2, 3, 4
 STATUS = {0: "running", 1: "target", 2: "max_rounds", 3: "stalled", 4: "overflow"}
 NF_OPT_GRAPH, NF_OPT_PROFILE, NF_OPT_DIAG, NF_OPT_CONTACT_PREC = 1, 2, 3, 4
+NF_OPT_CONTACT_ENGINE, NF_OPT_FOOTPRINT_CELL = 5, 6
+NF_ENGINE_FOOTPRINT, NF_ENGINE_CELLS = 0, 1
 KERNEL_CLASSES = ["gen", "bin", "reject_old", "reject_new", "commit", "control", "contact", "other"]
 
 
@@ -240,6 +242,7 @@ _SIGS = [
     ("nf_params_default", None, [C.POINTER(Params)]),
     ("nf_params_load", C.c_int, [C.c_char_p, C.POINTER(Params)]),
     ("nf_params_normalize", C.c_int, [C.POINTER(Params)]),
+    ("nf_agglomerate_centres", C.c_int, [C.POINTER(Params), C.c_void_p, C.c_int32]),
     ("nf_create", C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
     ("nf_create_batch", C.c_int, [C.POINTER(Params), C.c_int, C.POINTER(C.c_void_p)]),
     ("nf_destroy", None, [C.c_void_p]),
@@ -349,6 +352,19 @@ def write_density_mesh(path: str, dims, values: np.ndarray) -> None:
     dim = (C.c_double * 3)(*[float(x) for x in dims])
     ext = (C.c_int32 * 3)(*v.shape)
     _check(lib().nf_write_density_mesh(path.encode(), dim, ext, v.ctypes.data))
+
+
+def agglomerate_centres(p: Params) -> np.ndarray:
+    """(n_agg, 3) centres of the agglomerate density law of p."""
+    n = lib().nf_agglomerate_centres(C.byref(p), None, 0)
+    if n < 0:
+        _check(n)
+    out = np.zeros((max(n, 0), 3), np.float64)
+    if n > 0:
+        rc = lib().nf_agglomerate_centres(C.byref(p), out.ctypes.data, n)
+        if rc < 0:
+            _check(rc)
+    return out
 
 
 def load_params(path: str, base: Params | None = None) -> Params:

@@ -27,6 +27,7 @@ SOURCES = [
     "nf_scan.hip",
     "nf_gen.hip",
     "nf_pair.hip",
+    "nf_seg.hip",
     "nf_post.hip",
     "nf_net.hip",
     "nf_solve.hip",
@@ -41,7 +42,7 @@ SOURCES = [
     "nf_noalloc.hip",
     "nf_reduce.hip",
 ]
-HEADERS = ["nf_common.h", "nf_internal.h", "nf_host.h"]
+HEADERS = ["nf_common.h", "nf_internal.h", "nf_host.h", "nf_seg.h"]
 ARCH = os.environ.get("NF_OFFLOAD_ARCH", "gfx950")
 
 
@@ -106,7 +107,8 @@ def build(force: bool = False, jobs: int = 8) -> str:
     # host build of the pair predicates for the CPU test suite (test only)
     harness = os.path.join(ROOT, "tests", "native", "band_harness.cpp")
     hlib = os.path.join(ROOT, "tests", "native", "libnf_band.so")
-    if os.path.exists(harness) and (force or _newer(hlib, [harness, os.path.join(CSRC, "nf_common.h")])):
+    if os.path.exists(harness) and (force or _newer(hlib, [harness, os.path.join(CSRC, "nf_common.h"),
+                                                           os.path.join(CSRC, "nf_seg.h")])):
         cmd = [hipcc(), "--offload-host-only", "-x", "hip", "-O2", "-fPIC", "-shared", "-std=c++17",
                "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), harness, "-o", hlib]
         r = subprocess.run(cmd, capture_output=True, text=True)

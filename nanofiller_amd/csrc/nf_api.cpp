@@ -281,9 +281,10 @@ extern "C" int nf_params_normalize(nf_params* p) {
 // stencils
 // ===========================================================================
 namespace nf {
-std::vector<int4> build_stencil(const int32_t nc[3], const double box[3], double reach, bool half) {
+std::vector<int4> build_stencil(const int32_t nc[3], const double box[3], double reach, bool half, int bx) {
   std::vector<std::pair<float, int4>> v;
   const double rm = reach + (double)CELL_MARGIN;
+  if (bx < 1 || (half && bx != 1)) bx = 1;
   int k[3];
   double E[3];
   for (int d = 0; d < 3; ++d) {
@@ -292,15 +293,16 @@ std::vector<int4> build_stencil(const int32_t nc[3], const double box[3], double
   }
   for (int dz = -k[2]; dz <= k[2]; ++dz)
     for (int dy = -k[1]; dy <= k[1]; ++dy)
-      for (int dx = -k[0]; dx <= k[0]; ++dx) {
+      for (int dx = -k[0]; dx <= k[0] + bx - 1; ++dx) {
         if (half) {
           const bool pos = dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0)));
           if (!pos) continue;
         }
-        const int dd[3] = {dx, dy, dz};
+        // whole cells between the offset cell and the block, per axis
+        const int gap[3] = {dx >= bx ? dx - bx : (dx < 0 ? -dx - 1 : 0), std::abs(dy) - 1, std::abs(dz) - 1};
         double m2 = 0.0;
         for (int d = 0; d < 3; ++d) {
-          const int a = std::abs(dd[d]) - 1;
+          const int a = gap[d];
           if (a > 0) m2 += (a * E[d]) * (a * E[d]);
         }
         if (m2 <= rm * rm) {
@@ -632,6 +634,15 @@ extern "C" int nf_set_option(nf_handle* h, int key, int64_t value) {
         return fail(h, NF_E_INVALID, "bad contact precision");
       h->opt_contact_prec = (int)value;
       break;
+    case NF_OPT_CONTACT_ENGINE:
+      if (value != NF_ENGINE_FOOTPRINT && value != NF_ENGINE_CELLS)
+        return fail(h, NF_E_INVALID, "bad contact engine");
+      h->opt_contact_engine = (int)value;
+      break;
+    case NF_OPT_FOOTPRINT_CELL:
+      if (value < 0 || value > 1000000) return fail(h, NF_E_INVALID, "footprint cell factor out of range");
+      h->opt_footprint_cell = (int)value;
+      break;
     default: return fail(h, NF_E_INVALID, "unknown option");
   }
   return NF_OK;
@@ -831,6 +842,16 @@ extern "C" int nf_set_rods(nf_handle* h, int inst, const float* rods, uint32_t n
   return NF_OK;
 }
 
+
+extern "C" int nf_agglomerate_centres(const nf_params* p, double* out, int32_t cap) {
+  if (!p) return fail(nullptr, NF_E_INVALID, "null parameters");
+  if (p->den_law != NF_DEN_AGGLOM) return 0;
+  if (p->n_agg < 0) return fail(nullptr, NF_E_INVALID, "n_agg must be >= 0");
+  if (out && cap < p->n_agg) return fail(nullptr, NF_E_CAPACITY, "buffer too small");
+  if (out)
+    for (int k = 0; k < p->n_agg; ++k) agg_centre(p->seed, p->stream, p->box, (uint32_t)k, &out[3 * k]);
+  return p->n_agg;
+}
 
 // ===========================================================================
 // diagnostics

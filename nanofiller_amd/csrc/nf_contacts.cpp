@@ -20,41 +20,42 @@ using namespace nf;
 
 namespace nf {
 
-int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned long long*) {
+// Reach bounds of the contact rods: the instance's bounds after generation,
+// else the largest half length and radius of the rods (copied once).
+static int contact_bounds(nf_handle* h, uint32_t n, const float4* P, const float4* D, float* hlmax,
+                          float* amax) {
+  *hlmax = h->desc[0].hl_max;
+  *amax = (float)h->params[0].radius;
+  if (h->generated) return NF_OK;
+  std::vector<float4> hP(n), hD(n);
+  if (n) {
+    HIPCHK(h, hipMemcpy(hP.data(), P, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(hD.data(), D, sizeof(float4) * n, hipMemcpyDeviceToHost));
+  }
+  *hlmax = 0.0f;
+  *amax = 0.0f;
+  for (uint32_t i = 0; i < n; ++i) {
+    *hlmax = std::max(*hlmax, hP[i].w);
+    *amax = std::max(*amax, hD[i].w);
+  }
+  return NF_OK;
+}
+
+static double contact_reach(const ContactBuffers& c) {
+  const double Rmax = ((double)c.a_max + (double)c.a_max) + c.gap_cut;
+  return ((double)c.hl_max + (double)c.hl_max) * REACH_PAD + Rmax;
+}
+
+// Centre cell list of the contact rods on cells of about `edge`, with the
+// stencil of every cell within `reach` of a home block of bx cells along x
+// (half, bx = 1: each cell pair once).
+static int build_cell_list(nf_handle* h, double edge, double reach, bool half, int bx = 1) {
   ContactBuffers& c = h->cb;
   const nf_params& p = h->params[0];
-  const InstDesc& d = h->desc[0];
-  const uint32_t n = h->n_rods[0];
-  const float4* P = h->gb.bP + d.rod_off;
-  const float4* D = h->gb.bD + d.rod_off;
+  const uint32_t n = c.n_rods;
+  const float4* P = h->gb.bP + h->desc[0].rod_off;
+  const float4* D = h->gb.bD + h->desc[0].rod_off;
   hipStream_t s = h->stream;
-  // release previous results
-  release_contacts(h);
-  c.n_rods = n;
-  c.gap_cut = gap;
-  // geometry: the largest half length / radius any rod of the instance can have
-  float hlmax = d.hl_max, amax = (float)p.radius;
-  if (!h->generated) {
-    // rods came from nf_set_rods: bound from the data (copied once)
-    std::vector<float4> hP(n), hD(n);
-    if (n) {
-      HIPCHK(h, hipMemcpy(hP.data(), P, sizeof(float4) * n, hipMemcpyDeviceToHost));
-      HIPCHK(h, hipMemcpy(hD.data(), D, sizeof(float4) * n, hipMemcpyDeviceToHost));
-    }
-    hlmax = 0.0f;
-    amax = 0.0f;
-    for (uint32_t i = 0; i < n; ++i) {
-      hlmax = std::max(hlmax, hP[i].w);
-      amax = std::max(amax, hD[i].w);
-    }
-  }
-  const double Rmax = ((double)amax + (double)amax) + gap;
-  const double reach = ((double)hlmax + (double)hlmax) * REACH_PAD + Rmax;
-  // cell edge about one typical pair reach; the sorted stencil reaches the
-  // longest rods and per-cell length bounds prune the rest
-  double edge = (p.len_law == NF_LEN_LOGNORMAL ? p.len : (double)hlmax * 2.0) * REACH_PAD + Rmax;
-  if (edge > reach) edge = reach;
-  edge = default_cell_edge(edge, (double)n / (p.box[0] * p.box[1] * p.box[2]), 1.0);
   uint64_t ncells = 1;
   double scale[3];
   for (int a = 0; a < 3; ++a) {
@@ -67,84 +68,216 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   }
   if (ncells > (1ull << 26)) return fail(h, NF_E_INVALID, "contact grid too large");
   c.ncells = (uint32_t)ncells;
-  std::vector<int4> st = build_stencil(c.ncell, p.box, reach, true);
+  std::vector<int4> st = build_stencil(c.ncell, p.box, reach, half, bx);
   c.st_n = (uint32_t)st.size();
-  int rc = NF_OK;
-  unsigned long long* ctr = nullptr;
   uint32_t* tmp = nullptr;
-  uint32_t* queues = nullptr;
+  int rc = NF_OK;
 #define ALLOC(ptr, cnt) \
   if (rc == NF_OK) rc = dalloc(h, &(ptr), (cnt));
-  ALLOC(c.cP, n);
-  ALLOC(c.cD, n);
-  ALLOC(c.cid, n);
+  ALLOC(c.cP, std::max<uint64_t>(n, 1));
+  ALLOC(c.cD, std::max<uint64_t>(n, 1));
+  ALLOC(c.cid, std::max<uint64_t>(n, 1));
   ALLOC(c.cstart, ncells + 1);
   ALLOC(c.ccount, ncells);
   ALLOC(c.chl, ncells);
   ALLOC(c.ccur, ncells);
   ALLOC(c.stencil, st.size());
-  ALLOC(c.cnt, (uint64_t)n + 1);
-  ALLOC(c.rowptr, (uint64_t)n + 1);
-  ALLOC(c.fill, n);
-  ALLOC(ctr, 5);  // pair tests, rechecks, max dd, tile bytes, append count
-  ALLOC(tmp, std::max(scan_tmp_words(ncells), scan_tmp_words((uint64_t)n + 1)));
-  ALLOC(queues, 2 * QWORDS);
+  ALLOC(tmp, scan_tmp_words(ncells));
 #undef ALLOC
   if (rc != NF_OK) {
-    dfree(h, ctr);
     dfree(h, tmp);
-    dfree(h, queues);
     return rc;
   }
   HIPCHK(h, hipMemcpyAsync(c.stencil, st.data(), sizeof(int4) * st.size(), hipMemcpyHostToDevice, s));
-  hipEvent_t* ev = h->ev;
-  hipEventRecord(ev[0], s);
-  // bin
   fill_u32(c.ccount, 0u, ncells, s);
   fill_u32((uint32_t*)c.chl, 0u, ncells, s);
   fill_u32(c.ccur, 0u, ncells, s);
-  fill_u32((uint32_t*)ctr, 0u, 10, s);
-  fill_u32(queues, 0u, 2 * QWORDS, s);
   launch_cell_count(P, n, c.ncell, scale, c.ccount, c.chl, s);
   scan_u32(c.ccount, c.cstart, ncells, tmp, s);
   launch_cell_scatter(P, D, n, c.ncell, scale, c.cstart, c.ccur, c.cP, c.cD, c.cid, s);
-  hipEventRecord(ev[1], s);
-  // one pair pass appends every contact to an unsorted list
-  PairArgs a{};
-  a.desc = nullptr;
-  a.st = nullptr;
-  a.cell_inst = nullptr;
-  a.ncells_total = c.ncells;
-  a.stencil = c.stencil;
-  a.st_off = 0;
-  a.st_n = c.st_n;
-  a.prP = c.cP;
-  a.prD = c.cD;
-  a.prX = c.cid;
-  a.prStart = c.cstart;
-  a.prCount = c.ccount;
-  a.nbP = c.cP;
-  a.nbD = c.cD;
-  a.nbX = c.cid;
-  a.nbStart = c.cstart;
-  a.nbCount = c.ccount;
-  a.nbHl = c.chl;
-  a.qctr = queues;
-  for (int k = 0; k < 3; ++k) {
-    a.box[k] = p.box[k];
-    a.boxf[k] = (float)p.box[k];
-    a.ncell[k] = c.ncell[k];
+  const hipError_t e = sync_stream(s, "contact cell list");
+  dfree(h, tmp);
+  if (e != hipSuccess) return fail(h, NF_E_HIP, std::string("contact cell list: ") + hipGetErrorString(e));
+  return NF_OK;
+}
+
+// Cell list of the cell-pair engine: cells of about one typical pair reach,
+// half stencil of the longest pair reach.
+int build_contact_cells(nf_handle* h) {
+  ContactBuffers& c = h->cb;
+  if (c.cP) return NF_OK;
+  const nf_params& p = h->params[0];
+  const double Rmax = ((double)c.a_max + (double)c.a_max) + c.gap_cut;
+  const double reach = contact_reach(c);
+  // cell edge about one typical pair reach; the sorted stencil reaches the
+  // longest rods and per-cell length bounds prune the rest
+  double edge = (p.len_law == NF_LEN_LOGNORMAL ? p.len : (double)c.hl_max * 2.0) * REACH_PAD + Rmax;
+  if (edge > reach) edge = reach;
+  edge = default_cell_edge(edge, (double)c.n_rods / (p.box[0] * p.box[1] * p.box[2]), 1.0);
+  return build_cell_list(h, edge, reach, true);
+}
+
+// Footprint engine: home cells sized so that about 48 rods cross one at the
+// density peak (rho (E^3 + 1.5 L E^2) rods of length L cross a cube of edge
+// E, by Cauchy's mean projected area), fine cells of about a quarter rod
+// length (the option sets the fraction) inside them.
+static void footprint_cells(nf_handle* h, uint32_t n, double* edge, int* fine) {
+  const nf_params& p = h->params[0];
+  const ContactBuffers& c = h->cb;
+  const double typ = p.len_law == NF_LEN_LOGNORMAL ? p.len : (double)c.hl_max * 2.0;
+  const double V = p.box[0] * p.box[1] * p.box[2];
+  double peak = 1.0;
+  if (p.den_law == NF_DEN_AGGLOM) peak = p.agg_peak;
+  if (p.den_law == NF_DEN_MESH) peak = 3.0;
+  const double rho = (double)n / V * peak;
+  double lo = 1e-3, hi = std::max(p.box[0], std::max(p.box[1], p.box[2]));
+  for (int it = 0; it < 60; ++it) {
+    const double e = 0.5 * (lo + hi);
+    if (rho * (e * e * e + 1.5 * typ * e * e) > 48.0)
+      hi = e;
+    else
+      lo = e;
   }
-  a.gap64 = gap;
-  a.gap32 = (float)gap;
+  *edge = std::max(lo, 4.0 * ((double)c.a_max + c.gap_cut));
+  const double ff = h->opt_footprint_cell > 0 ? h->opt_footprint_cell * 1e-3 : 0.25;
+  int f = (int)std::lround(*edge / std::max(typ * ff, 1e-6));
+  *fine = f < 1 ? 1 : (f > 4 ? 4 : f);
+}
+
+// One contact pass with the given engine.  Returns 1 (not an NF_ status)
+// when the footprint engine found a home cell with too many survivors.
+static int contacts_pass(nf_handle* h, double gap, uint64_t* n_out, int prec, bool cells) {
+  ContactBuffers& c = h->cb;
+  const nf_params& p = h->params[0];
+  const InstDesc& d = h->desc[0];
+  const uint32_t n = h->n_rods[0];
+  const float4* P = h->gb.bP + d.rod_off;
+  const float4* D = h->gb.bD + d.rod_off;
+  hipStream_t s = h->stream;
+  // release previous results
+  release_contacts(h);
+  c.n_rods = n;
+  c.gap_cut = gap;
+  float hlmax = 0.0f, amax = 0.0f;
+  int rc = contact_bounds(h, n, P, D, &hlmax, &amax);
+  if (rc != NF_OK) return rc;
+  c.hl_max = hlmax;
+  c.a_max = amax;
+  const double Rmax = ((double)amax + (double)amax) + gap;
+  const double reach = contact_reach(c);
   const double Lmax = std::max(p.box[0], std::max(p.box[1], p.box[2]));
-  a.ew = (float)(5.9604644775390625e-8 * (Lmax + reach) * 1.01);
-  a.hl_max = hlmax;
-  a.R_max = (float)Rmax * 1.000001f;
-  a.ctr = ctr;
-  a.diag = h->opt_diag;
-  // append capacity: the previous call's count, else one contact per two
-  // rods; a pass that overflows is repeated with the exact size
+  const double ew = 5.9604644775390625e-8 * (Lmax + reach) * 1.01;
+  unsigned long long* ctr = nullptr;
+  uint32_t* tmp = nullptr;
+  uint32_t* queues = nullptr;
+#define ALLOC(ptr, cnt) \
+  if (rc == NF_OK) rc = dalloc(h, &(ptr), (cnt));
+  ALLOC(c.cnt, (uint64_t)n + 1);
+  ALLOC(c.rowptr, (uint64_t)n + 1);
+  ALLOC(c.fill, std::max<uint64_t>(n, 1));
+  ALLOC(ctr, 8);  // pair tests, rechecks, max dd, tile bytes, append count, work counter, overflow
+  ALLOC(tmp, scan_tmp_words((uint64_t)n + 1));
+  ALLOC(queues, 2 * QWORDS);
+#undef ALLOC
+  auto free_scratch = [&]() {
+    dfree(h, ctr);
+    dfree(h, tmp);
+    dfree(h, queues);
+    ctr = nullptr;
+    tmp = queues = nullptr;
+  };
+  if (rc != NF_OK) {
+    free_scratch();
+    return rc;
+  }
+  hipEvent_t* ev = h->ev;
+  hipEventRecord(ev[0], s);
+  fill_u32((uint32_t*)ctr, 0u, 16, s);
+  fill_u32(queues, 0u, 2 * QWORDS, s);
+  PairArgs a{};
+  SegCrossArgs xa{};
+  // ---- bin: centre cell list
+  if (cells) {
+    rc = build_contact_cells(h);
+  } else {
+    double edge = 0.0;
+    int fine = 1;
+    footprint_cells(h, n, &edge, &fine);
+    xa.g = seg_grid(p.box, edge, (double)amax, gap, ew);
+    const double rpad = (double)xa.g.r + 2.0 * (double)CELL_MARGIN;
+    rc = build_cell_list(h, edge, (double)hlmax * REACH_PAD + rpad, false, SEG_ROW);
+    xa.fine = fine;
+    xa.rpad = (float)rpad;
+    const float rr = (float)((double)hlmax * REACH_PAD + rpad) * 1.0001f;
+    xa.reach2 = rr * rr;
+  }
+  if (rc != NF_OK) {
+    free_scratch();
+    return rc;
+  }
+  if (cells) {
+    a.desc = nullptr;
+    a.st = nullptr;
+    a.cell_inst = nullptr;
+    a.ncells_total = c.ncells;
+    a.stencil = c.stencil;
+    a.st_off = 0;
+    a.st_n = c.st_n;
+    a.prP = c.cP;
+    a.prD = c.cD;
+    a.prX = c.cid;
+    a.prStart = c.cstart;
+    a.prCount = c.ccount;
+    a.nbP = c.cP;
+    a.nbD = c.cD;
+    a.nbX = c.cid;
+    a.nbStart = c.cstart;
+    a.nbCount = c.ccount;
+    a.nbHl = c.chl;
+    a.qctr = queues;
+    for (int k = 0; k < 3; ++k) {
+      a.box[k] = p.box[k];
+      a.boxf[k] = (float)p.box[k];
+      a.ncell[k] = c.ncell[k];
+    }
+    a.gap64 = gap;
+    a.gap32 = (float)gap;
+    a.ew = (float)ew;
+    a.hl_max = hlmax;
+    a.R_max = (float)Rmax * 1.000001f;
+    a.ctr = ctr;
+    a.diag = h->opt_diag;
+  } else {
+    // the claim grid is the cell list's grid
+    for (int k = 0; k < 3; ++k) {
+      xa.g.nc[k] = c.ncell[k];
+      xa.g.h[k] = (float)(p.box[k] / (double)c.ncell[k]);
+      xa.g.inv_h[k] = (float)((double)c.ncell[k] / p.box[k]);
+      xa.g.inv_h64[k] = (double)c.ncell[k] / p.box[k];
+      xa.x.box[k] = p.box[k];
+      xa.x.boxf[k] = (float)p.box[k];
+    }
+    xa.g.ncells = c.ncells;
+    xa.cP = c.cP;
+    xa.cD = c.cD;
+    xa.cid = c.cid;
+    xa.cstart = c.cstart;
+    xa.ccount = c.ccount;
+    xa.chl = c.chl;
+    xa.stencil = c.stencil;
+    xa.st_n = c.st_n;
+    xa.x.gap64 = gap;
+    xa.x.gap32 = (float)gap;
+    xa.x.ew = (float)ew;
+    xa.x.diag = h->opt_diag;
+    xa.ctr = ctr;
+    xa.next = queues;  // eight counters, QSTRIDE words apart
+    xa.overflow = (uint32_t*)(ctr + 6);
+  }
+  hipEventRecord(ev[1], s);
+  // one pair pass appends every contact to an unsorted list.  Append
+  // capacity: the previous call's count, else one contact per two rods; a
+  // pass that overflows is repeated with the exact size
   uint64_t cap = std::max<uint64_t>(h->contact_hint + h->contact_hint / 8 + 1024, (uint64_t)n / 2 + 4096);
   if (cap > 0xFFFFFFF0ull) cap = 0xFFFFFFF0ull;
   uint32_t* appI = nullptr;
@@ -165,9 +298,7 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   };
   auto free_all = [&]() {
     free_app();
-    dfree(h, ctr);
-    dfree(h, tmp);
-    dfree(h, queues);
+    free_scratch();
   };
   uint32_t total = 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
@@ -182,23 +313,41 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
       return rc;
     }
     if (attempt > 0) {
-      fill_u32((uint32_t*)ctr, 0u, 10, s);
+      fill_u32((uint32_t*)ctr, 0u, 16, s);
       fill_u32(queues, 0u, 2 * QWORDS, s);
     }
-    a.app_n = (uint32_t*)(ctr + 4);
-    a.app_cap = (uint32_t)cap;
-    a.outI = appI;
-    a.outJ = appJ;
-    a.outGap = appG;
-    a.outGap64 = appG64;
-    a.outImg = appImg;
     hipEventRecord(ev[1], s);
-    launch_pair(PM_CNT, prec, a, s);
+    if (cells) {
+      a.app_n = (uint32_t*)(ctr + 4);
+      a.app_cap = (uint32_t)cap;
+      a.outI = appI;
+      a.outJ = appJ;
+      a.outGap = appG;
+      a.outGap64 = appG64;
+      a.outImg = appImg;
+      launch_pair(PM_CNT, prec, a, s);
+    } else {
+      xa.app_n = (uint32_t*)(ctr + 4);
+      xa.app_cap = (uint32_t)cap;
+      xa.outI = appI;
+      xa.outJ = appJ;
+      xa.outGap = appG;
+      xa.outGap64 = appG64;
+      xa.outImg = appImg;
+      launch_seg_cross(prec, xa, s);
+    }
     hipEventRecord(ev[2], s);
-    trace("contacts: %u rods, %u cells, %u stencil offsets, pair pass launched", n, c.ncells, c.st_n);
-    HIPCHK(h, hipMemcpyAsync(&total, a.app_n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    trace("contacts: %u rods, %s pass launched", n, cells ? "cell-pair" : "footprint");
+    uint32_t over = 0;
+    HIPCHK(h, hipMemcpyAsync(&total, (uint32_t*)(ctr + 4), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipMemcpyAsync(&over, (uint32_t*)(ctr + 6), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(h, sync_stream(s, "contact pair pass"));
     HIPCHK(h, hipGetLastError());
+    if (over) {
+      trace("contacts: a home cell has more survivors than fit; cell-pair engine instead");
+      free_all();
+      return 1;
+    }
     if ((uint64_t)total <= cap) break;
     trace("contacts: %u contacts exceed the append capacity %llu, repeating the pass", total,
           (unsigned long long)cap);
@@ -246,8 +395,16 @@ int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned l
   }
   free_all();
   c.valid = true;
+  c.engine = cells ? NF_ENGINE_CELLS : NF_ENGINE_FOOTPRINT;
   if (n_out) *n_out = total;
   return NF_OK;
+}
+
+int contacts_run(nf_handle* h, double gap, uint64_t* n_out, int prec, unsigned long long*) {
+  const bool cells = h->opt_contact_engine == NF_ENGINE_CELLS;
+  int rc = contacts_pass(h, gap, n_out, prec, cells);
+  if (rc == 1) rc = contacts_pass(h, gap, n_out, prec, true);
+  return rc;
 }
 
 }  // namespace nf

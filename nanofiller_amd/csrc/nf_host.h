@@ -23,6 +23,8 @@ struct nf_handle {
   hipEvent_t ev[9] = {};
   std::string err;
   int opt_graph = 0, opt_profile = 0, opt_diag = 0, opt_contact_prec = -1;
+  int opt_contact_engine = NF_ENGINE_CELLS;
+  int opt_footprint_cell = 0;  // footprint cell edge in 1/1000 of the typical rod length (0: 250)
   int device = 0;
   std::vector<uint32_t> n_rods;  // current rod count per instance
   bool generated = false;
@@ -51,9 +53,17 @@ void trace(const char* fmt, ...);
 int dalloc_bytes(nf_handle* h, void** p, size_t bytes);
 void dfree(nf_handle* h, void* p);
 void release_contacts(nf_handle* h);
+// Centre cell list of the contact rods (edge about one pair reach, sorted
+// stencil): built by the cell engine of nf_contacts, or on demand by the
+// node network.
+int build_contact_cells(nf_handle* h);
 // Default cell edge for a typical pair reach and a mean rod density.
 double default_cell_edge(double reach_typ, double density, double factor);
-std::vector<int4> build_stencil(const int32_t nc[3], const double box[3], double reach, bool half);
+// Neighbour offsets of a home block of bx cells along x (offsets from its
+// first cell) whose cells come within `reach` of the block, sorted by that
+// minimum distance (its square, slightly low, in .w as float bits); `half`
+// (bx = 1 only) keeps one of each +-offset pair.
+std::vector<int4> build_stencil(const int32_t nc[3], const double box[3], double reach, bool half, int bx = 1);
 
 // Histogram limits (nf_hist_spec): 1-D bins, phi x theta cells, and the
 // bin-edge table: polar and mutual edges (bins + 1 each), then the theta

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "nf_common.h"
+#include "nf_seg.h"
 
 namespace nf {
 
@@ -157,6 +158,8 @@ struct ContactBuffers {
   uint64_t n_contacts = 0, cap_contacts = 0;
   uint32_t n_rods = 0;
   double gap_cut = 0.0;
+  float hl_max = 0.0f, a_max = 0.0f;  // bounds on half length and radius of the rods
+  int engine = 0;                     // NF_ENGINE_* that found the last contact list
   bool valid = false;
   // network
   uint32_t* labels = nullptr;
@@ -229,6 +232,37 @@ struct PairArgs {
 
 enum PairMode { PM_GEN_OLD = 0, PM_GEN_NEW = 1, PM_CNT = 2 };
 
+// Arguments of the footprint contact pass (nf_seg.hip).
+// Home cells of one work unit of the footprint pass: a run along x.
+constexpr int SEG_ROW = 4;
+
+struct SegCrossArgs {
+  // centre cell list of the contact rods
+  const float4* cP;
+  const float4* cD;
+  const uint32_t* cid;
+  const uint32_t* cstart;
+  const uint32_t* ccount;
+  const float* chl;
+  const int4* stencil;  // stencil of a SEG_ROW-cell home run, sorted by minimum distance
+  uint32_t st_n;
+  float reach2;  // (hl_max + rpad)^2: the stencil walk stops past it
+  float rpad;    // footprint radius plus the cell margin of a centre
+  int fine;      // fine cells per axis of a home cell (1..4)
+  SegGrid g;     // the cell grid (claim rule) and footprint radius
+  SegCtx x;
+  uint32_t* next;      // eight work counters over the work units, QSTRIDE words apart
+  uint32_t* overflow;  // set when a home cell has more survivors than fit
+  uint32_t* app_n;
+  uint32_t app_cap;
+  uint32_t* outI;
+  uint32_t* outJ;
+  float* outGap;
+  double* outGap64;
+  int32_t* outImg;
+  unsigned long long* ctr;  // [0] capsule tests, [1] rechecks, [2] max dd bits, [3] bytes
+};
+
 // ---- launchers (nf_scan.hip) ----
 void scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s);
 size_t scan_tmp_words(uint64_t n);
@@ -251,6 +285,9 @@ int pair_grid(int mode);
 // per wave of the grid.
 int pair_part_shift(int mode);
 int pair_split_min(int mode);
+
+// ---- launchers (nf_seg.hip) ----
+void launch_seg_cross(int prec, const SegCrossArgs& a, hipStream_t s);
 
 // ---- launchers (nf_post.hip) ----
 void launch_cell_count(const float4* P, uint32_t n, const int32_t ncell[3], const double scale[3],

@@ -37,6 +37,9 @@ extern "C" int nf_node_network(nf_handle* h, nf_node_info* info) {
   const nf_params& p = h->params[0];
   if (p.elec_axis < 0 || p.elec_axis > 2) return fail(h, NF_E_INVALID, "elec_axis must be 0..2");
   free_nodes(h);
+  // the screening walk runs over the centre cell list of the contact rods
+  int rc0 = build_contact_cells(h);
+  if (rc0 != NF_OK) return rc0;
   const uint32_t n = c.n_rods;
   const uint64_t nc = c.n_contacts;
   hipStream_t s = h->stream;
@@ -55,8 +58,10 @@ extern "C" int nf_node_network(nf_handle* h, nf_node_info* info) {
   }
   a.L = p.box[p.elec_axis];
   a.gcut = p.contact_gap;
-  a.hl_max = 0.5 * (double)(float)(p.len_law == NF_LEN_LOGNORMAL ? p.len_max : p.len);
-  a.a_max = (double)(float)p.radius;
+  // the bounds the contact pass used: from the data when the rods came from
+  // nf_set_rods or a state file, so longer or thicker rods are reached too
+  a.hl_max = (double)c.hl_max;
+  a.a_max = (double)c.a_max;
   a.cP = c.cP;
   a.cD = c.cD;
   a.cid = c.cid;

@@ -49,8 +49,9 @@ __device__ __forceinline__ int img_c(int32_t code, int axis) {
 
 }  // namespace
 
-// Contact points and screening, one thread per contact (its row found by
-// binary search in the row pointers).
+// Contact points and screening, one wave per contact (its row found by
+// binary search in the row pointers); the lanes span the rods of the cells
+// around the contact's virtual inclusion.
 __global__ void __launch_bounds__(NT) k_nd_points(NodeArgs a) {
   const uint32_t nc = a.rowptr[a.n];
   const uint32_t lane = threadIdx.x & (WAVE - 1);

@@ -178,7 +178,7 @@ extern "C" int nf_reduce(nf_handle* h, int32_t min_cont, nf_reduce_info* info) {
   a.D = D;
   a.axis = p.elec_axis;
   a.L = p.box[p.elec_axis];
-  a.gcut = c.gap_cut;
+  a.gcut = p.contact_gap;  // the electrode rule of nf_network
   a.min_cont = (uint32_t)min_cont;
   uint32_t *keep = nullptr, *pos = nullptr, *tmp = nullptr, *oid = nullptr;
   float4 *oP = nullptr, *oD = nullptr;

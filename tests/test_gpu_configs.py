@@ -73,9 +73,13 @@ def test_contact_workload_small(prec):
     with nf.Packing(p) as pk:
         pk.set_rods(rods)
         c = pk.contacts(p.contact_gap)
-        st = pk.contact_stats()
         g64 = pk.contact_gap64(c.n)
         h = pk.histograms(bins=64, centre_axis=0, pol_axis=(1, 0, 0))
+        pk.set_option(nf.NF_OPT_CONTACT_ENGINE, nf.NF_ENGINE_CELLS)
+        cc = pk.contacts(p.contact_gap)
+        st = pk.contact_stats()
+    np.testing.assert_array_equal(cc.col, c.col)
+    np.testing.assert_array_equal(cc.row_ptr, c.row_ptr)
     np.testing.assert_array_equal(c.I, o["I"])
     np.testing.assert_array_equal(c.col, o["J"])
     np.testing.assert_array_equal(c.image, o["image"])
@@ -129,6 +133,14 @@ def test_contact_workload_full_size_properties():
         c = pk.contacts(p.contact_gap)
         g64 = pk.contact_gap64(c.n)
         h = pk.histograms(bins=64, centre_axis=0, pol_axis=(1, 0, 0))
+        pk.set_option(nf.NF_OPT_CONTACT_ENGINE, nf.NF_ENGINE_CELLS)
+        cc = pk.contacts(p.contact_gap)
+        g64c = pk.contact_gap64(cc.n)
+    # the footprint and cell engines find the same contact list
+    np.testing.assert_array_equal(cc.row_ptr, c.row_ptr)
+    np.testing.assert_array_equal(cc.col, c.col)
+    np.testing.assert_array_equal(cc.image, c.image)
+    np.testing.assert_array_equal(g64c.view(np.uint64), g64.view(np.uint64))
     assert st.n_accepted == 4000000 and nf.STATUS[st.status] == "target"
     assert h["centre"].sum() == 4000000 and h["polar"].sum() == 4000000
     assert h["mutual"].sum() == c.n

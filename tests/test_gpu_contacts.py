@@ -14,14 +14,15 @@ def _packing(p):
     return rods
 
 
+@pytest.mark.parametrize("engine", [nf.NF_ENGINE_FOOTPRINT, nf.NF_ENGINE_CELLS])
 @pytest.mark.parametrize("prec", [nf.NF_PREC_FP64, nf.NF_PREC_FP32_RECHECK])
 @pytest.mark.parametrize("case", ["cfg1", "small_cfg2", "tiny_box"])
-def test_contacts_parity(case, prec):
+def test_contacts_parity(case, prec, engine):
     p = getattr(cases, case)(prec)
     rods = _packing(p)
     gap = p.contact_gap
     o = oracle.contacts(p, rods, gap, threads=16)
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, {nf.NF_OPT_CONTACT_ENGINE: engine}) as pk:
         pk.set_rods(rods)
         c = pk.contacts(gap)
         st = pk.contact_stats()
@@ -31,7 +32,25 @@ def test_contacts_parity(case, prec):
     np.testing.assert_array_equal(c.image, o["image"])
     np.testing.assert_array_equal(c.gap.view(np.uint32), o["gap"].view(np.uint32))
     np.testing.assert_array_equal(g64.view(np.uint64), o["gap64"].view(np.uint64))
-    assert st["pair_tests"] == o["pair_tests"]
+    if engine == nf.NF_ENGINE_CELLS:
+        assert st["pair_tests"] == o["pair_tests"]  # pairs within centre reach
+
+
+@pytest.mark.parametrize("cell", [100, 250, 600, 5000])
+def test_footprint_cell_edge_does_not_change_contacts(cell):
+    """Any footprint cell edge (a tenth of a rod length to more than the box)
+    gives the same contact list."""
+    p = cases.small_cfg2(nf.NF_PREC_FP32_RECHECK)
+    rods = _packing(p)
+    o = oracle.contacts(p, rods, p.contact_gap, threads=16)
+    with nf.Packing(p, {nf.NF_OPT_FOOTPRINT_CELL: cell}) as pk:
+        pk.set_rods(rods)
+        c = pk.contacts(p.contact_gap)
+        g64 = pk.contact_gap64(c.n)
+    np.testing.assert_array_equal(c.I, o["I"])
+    np.testing.assert_array_equal(c.col, o["J"])
+    np.testing.assert_array_equal(c.image, o["image"])
+    np.testing.assert_array_equal(g64.view(np.uint64), o["gap64"].view(np.uint64))
 
 
 def test_histograms_parity():

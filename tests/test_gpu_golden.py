@@ -18,7 +18,12 @@ def test_device_matches_golden(name):
         rods = pk.rods()
         c = pk.contacts(p.contact_gap)
         g64 = pk.contact_gap64(c.n)
+        # the cell engine counts the oracle's pairs within centre reach
+        pk.set_option(nf.NF_OPT_CONTACT_ENGINE, nf.NF_ENGINE_CELLS)
+        cc = pk.contacts(p.contact_gap)
         cst = pk.contact_stats()
+        pk.set_option(nf.NF_OPT_CONTACT_ENGINE, nf.NF_ENGINE_FOOTPRINT)
+        c = pk.contacts(p.contact_gap)
         h = pk.histograms(bins=hs["bins"], centre_axis=hs["centre_axis"],
                           pol_axis=tuple(hs["pol_axis"]), phi_bins=hs["phi_bins"],
                           theta_bins=hs["theta_bins"])
@@ -33,6 +38,8 @@ def test_device_matches_golden(name):
     np.testing.assert_array_equal(c.gap.view(np.uint32), g["gap_bits"])
     np.testing.assert_array_equal(g64.view(np.uint64), g["gap64_bits"])
     assert cst["pair_tests"] == meta["pair_tests"]
+    np.testing.assert_array_equal(cc.col, c.col)
+    np.testing.assert_array_equal(cc.row_ptr, c.row_ptr)
     for k in ("centre", "polar", "mutual", "phitheta"):
         np.testing.assert_array_equal(h[k], g["h_" + k])
     assert info == meta["network"]
