@@ -16,13 +16,17 @@ The same JSON line carries:
   * batch: 256 independent packings of uneven size (2k-200k rods) in one
     batched FP32 + recheck generation, against the same 256 packings made one
     at a time by the plain FP64 path, with identical packings;
-  * contacts: 4,000,000 rods in a 50 um cube: contact list (FP32 + recheck,
-    and FP64) and three 64-bin int64 histograms, contact-kernel roofline;
+  * contacts: 4,000,000 rods in a 50 um cube: contact list by the footprint
+    engine (FP32 + recheck, and FP64) and by the cell-pair engine, three
+    64-bin int64 histograms, contact-kernel HBM and VALU-issue rooflines;
   * network: 8,000,000 rods in a 40 um cube: contacts, components and the
     FP64 Kirchhoff matrix;
   * cpu_baseline: the C oracle on a bounded sample of the headline workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs main,batch,contacts,network,conduct]
+(legs prof_main and prof_contacts run only the kernels a counter profile is
+taken on: one headline generation; one 4M generation and one contact pass
+per engine)
 """
 from __future__ import annotations
 
@@ -47,6 +51,10 @@ K_NEW = nf.KERNEL_CLASSES.index("reject_new")
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_pair_generate_cfg2_500k_hbm_counters.md")
+VALU_SUMMARY = os.path.join(ROOT, "profiles", "pair_kernels_valu_counters.md")
+# VALU issue roof: a wave64 VALU instruction occupies a 16-lane SIMD for 4
+# cycles; 256 CUs x 4 SIMDs at the 2.4 GHz peak engine clock
+SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 4, 2.4e9
 
 
 def profiled_traffic():
@@ -61,6 +69,32 @@ def profiled_traffic():
         return float(m.group(1)) if m else None
     except OSError:
         return None
+
+
+def profiled_valu(key: str):
+    """SQ_INSTS_VALU per launch of one pair kernel on one workload from the
+    committed rocprofv3 counter summary (lines 'valu_per_launch <key>: N'),
+    or None when absent."""
+    import re
+
+    try:
+        with open(VALU_SUMMARY) as f:
+            m = re.search(r"valu_per_launch " + re.escape(key) + r": (\d+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def valu_roof(key: str, ms_per_launch: float):
+    """VALU-issue roofline of a kernel: its counted wave instructions at 4
+    cycles each spread over every SIMD, against its measured time."""
+    n = profiled_valu(key)
+    if n is None or ms_per_launch <= 0:
+        return None
+    issue_ms = n * VALU_CYCLES / SIMDS / CLOCK_HZ * 1e3
+    return {"bound": "valu", "instructions_per_launch": n, "issue_ms": issue_ms,
+            "launch_ms": ms_per_launch, "frac": issue_ms / ms_per_launch,
+            "source": os.path.relpath(VALU_SUMMARY, ROOT)}
 
 
 def log(msg: str) -> None:
@@ -122,7 +156,7 @@ def leg_main(a, out: dict) -> None:
         "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT),
         "kernel": "k_pair<reject vs accepted, FP32+recheck>",
         "bytes_per_launch": sp.tile_bytes / launches, "avg_launch_ms": k_old / launches,
-        "launches": launches,
+        "launches": launches, "valu": valu_roof("k_pair<0, 1> 500k", k_old / launches),
     }
     out["extra"]["fp32_recheck"] = {
         "accepted_rods_per_s": rods_s, "pair_tests_per_s": pair_s, "ms_per_step": ms_step,
@@ -268,6 +302,7 @@ def leg_contacts(a, out: dict) -> None:
     p = configs.cfg4(nf.NF_PREC_FP32_RECHECK)
     log("contacts: 4M rods, generate")
     with nf.Packing(p) as pk:
+        pk.generate()  # warm-up
         t0 = time.perf_counter()
         st = pk.generate()[0]
         t_gen = time.perf_counter() - t0
@@ -276,10 +311,20 @@ def leg_contacts(a, out: dict) -> None:
         c32 = pk.contacts(p.contact_gap)
         t_c = time.perf_counter() - t0
         cs = pk.contact_stats()
+        eng = pk.contact_engine()
         t0 = time.perf_counter()
         h = pk.histograms(bins=64, centre_axis=0, pol_axis=(1, 0, 0))
         t_h = time.perf_counter() - t0
         rods = pk.rods()
+        # the cell-pair engine on the same packing (its test count is the
+        # number of pairs within centre reach)
+        pk.set_option(nf.NF_OPT_CONTACT_ENGINE, nf.NF_ENGINE_CELLS)
+        pk.contacts(p.contact_gap)
+        t0 = time.perf_counter()
+        cc = pk.contacts(p.contact_gap)
+        t_cc = time.perf_counter() - t0
+        csc = pk.contact_stats()
+        pk.set_option(nf.NF_OPT_CONTACT_ENGINE, nf.NF_ENGINE_FOOTPRINT)
         pk.set_option(nf.NF_OPT_DIAG, 1)
         pk.contacts(p.contact_gap)
         dd_contacts = pk.contact_stats()["max_abs_d32_d64"]
@@ -291,24 +336,31 @@ def leg_contacts(a, out: dict) -> None:
         cs64 = pk.contact_stats()
     gen_prof = profiled_generation(p)
     km = cs["kernel_ms"]
-    bytes_pass = cs["tile_bytes"]  # probe and neighbour bytes streamed by the pair pass
-    achieved = bytes_pass / (km["pairs"] * 1e-3) / 1e9 if km["pairs"] > 0 else 0.0
     # the algorithmic bytes of the contact pass: N * 32 B * 14 tiles (27-cell
     # stencil with Newton's third law) + C * 12 B (I, J, gap)
     alg_bytes = st.n_accepted * 32 * 14 + c32.n * 12
     alg_achieved = alg_bytes / (km["pairs"] * 1e-3) / 1e9 if km["pairs"] > 0 else 0.0
-    log(f"contacts: {c32.n} pairs, pair pass {km['pairs']:.3f} ms, CSR {km['csr']:.3f} ms, "
-        f"{achieved:.0f} GB/s; FP64 pair pass {cs64['kernel_ms']['pairs']:.3f} ms")
+    alg_cells = alg_bytes / (csc["kernel_ms"]["pairs"] * 1e-3) / 1e9 if csc["kernel_ms"]["pairs"] > 0 else 0.0
+    log(f"contacts: {c32.n} pairs, footprint pass {km['pairs']:.3f} ms ({alg_achieved:.0f} GB/s), "
+        f"cell-pair pass {csc['kernel_ms']['pairs']:.3f} ms; FP64 pass {cs64['kernel_ms']['pairs']:.3f} ms")
     out["extra"]["contacts"] = {
         "workload": "4,000,000 rods, 50 um cube, l 1 um, d 10 nm, uniform, vMF(+x, 2), gap <= 2 nm",
         "generate_ms": t_gen * 1e3, "generate_rods_per_s": st.n_accepted / t_gen,
-        "contacts": int(c32.n), "contacts_call_ms": t_c * 1e3, "kernel_ms": km,
+        "generate_rounds": st.rounds,
+        "contacts": int(c32.n), "contacts_call_ms": t_c * 1e3, "kernel_ms": km, "engine": eng,
         "pair_tests": cs["pair_tests"], "rechecks": cs["rechecks"],
+        "streamed_bytes": cs["tile_bytes"],
         "pair_kernel_roofline": {"bound": "hbm", "achieved": alg_achieved, "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": alg_achieved / HBM_PEAK_GBS,
                                   "algorithmic_bytes_per_pass": alg_bytes,
-                                  "tile_bytes_per_pass": bytes_pass,
-                                  "tile_bytes_gbs": achieved},
+                                  "kernel": "k_seg_cross (footprint engine, FP32+recheck)",
+                                  "valu": valu_roof("k_seg_cross 4M", km["pairs"])},
+        "cell_engine": {"contacts_call_ms": t_cc * 1e3, "kernel_ms": csc["kernel_ms"],
+                        "pairs_within_reach": csc["pair_tests"], "streamed_bytes": csc["tile_bytes"],
+                        "identical_list": bool(same_contacts(c32, cc)),
+                        "roofline": {"bound": "hbm", "achieved": alg_cells, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": alg_cells / HBM_PEAK_GBS,
+                                     "valu": valu_roof("k_pair<2, 1> 4M", csc["kernel_ms"]["pairs"])}},
         "max_abs_d32_d64_um": dd_contacts,
         "generation_profile": gen_prof,
         "fp64_contacts_call_ms": t_c64 * 1e3, "fp64_kernel_ms": cs64["kernel_ms"],
@@ -390,6 +442,36 @@ def leg_conduct(a, out: dict) -> None:
     }
 
 
+def leg_prof_main(a, out: dict) -> None:
+    """One generation of the headline workload and nothing else: the
+    command that rocprofv3 counter summaries of its kernels are taken on."""
+    p = configs.cfg2(nf.NF_PREC_FP32_RECHECK)
+    with nf.Packing(p) as pk:
+        st = pk.generate()[0]
+    out["extra"]["prof_main"] = {"workload": "one 500k-rod headline generation", "rounds": st.rounds,
+                                 "rounds_launched": st.rounds_launched, "tile_bytes": st.tile_bytes,
+                                 "accepted": st.n_accepted}
+
+
+def leg_prof_contacts(a, out: dict) -> None:
+    """One generation of the 4M-rod contact workload, one footprint contact
+    pass and one cell-pair contact pass (FP32 + recheck): the command that
+    rocprofv3 counter summaries of the contact kernels are taken on."""
+    p = configs.cfg4(nf.NF_PREC_FP32_RECHECK)
+    with nf.Packing(p) as pk:
+        st = pk.generate()[0]
+        c = pk.contacts(p.contact_gap)
+        cs = pk.contact_stats()
+        pk.set_option(nf.NF_OPT_CONTACT_ENGINE, nf.NF_ENGINE_CELLS)
+        cc = pk.contacts(p.contact_gap)
+        csc = pk.contact_stats()
+    out["extra"]["prof_contacts"] = {"workload": "one 4M-rod generation + footprint pass + cell-pair pass",
+                                     "rounds": st.rounds, "contacts": int(c.n),
+                                     "footprint_streamed_bytes": cs["tile_bytes"],
+                                     "cell_engine_streamed_bytes": csc["tile_bytes"],
+                                     "identical_list": bool(same_contacts(c, cc))}
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -462,7 +544,8 @@ def main() -> int:
     if "main" in legs:
         leg_main(a, res)
     for name, fn in (("batch", leg_batch), ("contacts", leg_contacts), ("network", leg_network),
-                     ("conduct", leg_conduct)):
+                     ("conduct", leg_conduct), ("prof_main", leg_prof_main),
+                     ("prof_contacts", leg_prof_contacts)):
         if name in legs:
             fn(a, res)
     cpu = None

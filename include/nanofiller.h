@@ -423,9 +423,12 @@ enum {
   NF_OPT_DIAG = 3,         /* 1: also compute FP64 for FP32-decided pairs and report
                               max |d32 - d64|                                         */
   NF_OPT_CONTACT_PREC = 4, /* NF_PREC_* for nf_contacts (-1: the params' precision)   */
-  NF_OPT_CONTACT_ENGINE = 5, /* NF_ENGINE_* candidate pairs of nf_contacts (default cells) */
-  NF_OPT_FOOTPRINT_CELL = 6  /* footprint engine: cell edge in 1/1000 of the typical rod
-                                length (0: default 250)                                  */
+  NF_OPT_CONTACT_ENGINE = 5, /* NF_ENGINE_* candidate pairs of nf_contacts (default
+                                footprint)                                                */
+  NF_OPT_FOOTPRINT_FINE = 6, /* footprint engine: fine cell edge in 1/1000 of the typical
+                                rod length (0: default 200)                               */
+  NF_OPT_FOOTPRINT_OCC = 7   /* footprint engine: rods crossing a home cell at the mean
+                                density, which sizes the home cells (0: default 64)       */
 };
 /* Candidate pairs of nf_contacts.  Both give the same contact list.
  * FOOTPRINT: each rod binned into the fine cells its capsule (radius
@@ -446,6 +449,14 @@ int nf_set_option(nf_handle* h, int key, int64_t value);
  * pair pass streamed. */
 int nf_contacts_stats(nf_handle* h, uint64_t* pair_tests, uint64_t* rechecks, double* max_dd,
                       float* kernel_ms, uint64_t* tile_bytes);
+
+/* Engine that found the last contact list (NF_ENGINE_*; the footprint
+ * engine falls back to the cell engine when a single home cell is crossed
+ * by more rods than fit), and for the footprint engine its home cell edge
+ * (um), fine cells per axis (0 for the cell engine) and the number of work
+ * units (runs of home cells) it redid one cell at a time.  No counterpart in
+ * the reference: a diagnostic for tests and tuning. */
+int nf_contacts_engine(nf_handle* h, int* engine, double* home_edge, int* fine, uint64_t* split_units);
 
 /* Diagnostics (no counterpart in the reference): evaluates one FP64 device
  * building block element-wise on the GPU so callers can check it against

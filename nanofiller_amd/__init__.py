@@ -31,7 +31,7 @@ NF_PREC_FP64, NF_PREC_FP32_RECHECK = 0, 1
 NF_STATUS_RUNNING, NF_STATUS_TARGET, NF_STATUS_MAX_ROUNDS, NF_STATUS_STALLED, NF_STATUS_OVERFLOW = 0, 1, 2, 3, 4
 STATUS = {0: "running", 1: "target", 2: "max_rounds", 3: "stalled", 4: "overflow"}
 NF_OPT_GRAPH, NF_OPT_PROFILE, NF_OPT_DIAG, NF_OPT_CONTACT_PREC = 1, 2, 3, 4
-NF_OPT_CONTACT_ENGINE, NF_OPT_FOOTPRINT_CELL = 5, 6
+NF_OPT_CONTACT_ENGINE, NF_OPT_FOOTPRINT_FINE, NF_OPT_FOOTPRINT_OCC = 5, 6, 7
 NF_ENGINE_FOOTPRINT, NF_ENGINE_CELLS = 0, 1
 KERNEL_CLASSES = ["gen", "bin", "reject_old", "reject_new", "commit", "control", "contact", "other"]
 
@@ -255,6 +255,8 @@ _SIGS = [
     ("nf_get_contacts", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_get_contact_gap64", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_get_contact_area", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("nf_contacts_engine", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
     ("nf_contacts_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_uint64)]),
     ("nf_centre_distance", C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p,
@@ -478,6 +480,15 @@ class Packing:
         return {"pair_tests": t.value, "rechecks": r.value, "max_abs_d32_d64": m.value,
                 "tile_bytes": tb.value,
                 "kernel_ms": {"bin": ms[0], "pairs": ms[1], "csr": ms[2], "sort": ms[3]}}
+
+    def contact_engine(self) -> dict:
+        """Engine that found the last contact list (after any fallback), and
+        the footprint engine's home cell edge, fine cells per axis and work
+        units redone one cell at a time."""
+        e, edge, f, sp = C.c_int(), C.c_double(), C.c_int(), C.c_uint64()
+        _check(lib().nf_contacts_engine(self._h, C.byref(e), C.byref(edge), C.byref(f), C.byref(sp)),
+               self._h)
+        return {"engine": e.value, "home_edge": edge.value, "fine": f.value, "split_units": sp.value}
 
     def histograms(self, bins=64, centre_axis=0, pol_axis=(1.0, 0.0, 0.0), mutual=True,
                    phi_bins=0, theta_bins=0, noalloc: bool = False) -> dict:

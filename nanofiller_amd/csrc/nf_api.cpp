@@ -639,9 +639,13 @@ extern "C" int nf_set_option(nf_handle* h, int key, int64_t value) {
         return fail(h, NF_E_INVALID, "bad contact engine");
       h->opt_contact_engine = (int)value;
       break;
-    case NF_OPT_FOOTPRINT_CELL:
-      if (value < 0 || value > 1000000) return fail(h, NF_E_INVALID, "footprint cell factor out of range");
-      h->opt_footprint_cell = (int)value;
+    case NF_OPT_FOOTPRINT_FINE:
+      if (value < 0 || value > 1000000) return fail(h, NF_E_INVALID, "footprint fine cell factor out of range");
+      h->opt_footprint_fine = (int)value;
+      break;
+    case NF_OPT_FOOTPRINT_OCC:
+      if (value < 0 || value > 200) return fail(h, NF_E_INVALID, "footprint occupancy out of range (0..200)");
+      h->opt_footprint_occ = (int)value;
       break;
     default: return fail(h, NF_E_INVALID, "unknown option");
   }

@@ -117,29 +117,32 @@ int build_contact_cells(nf_handle* h) {
   return build_cell_list(h, edge, reach, true);
 }
 
-// Footprint engine: home cells sized so that about 48 rods cross one at the
-// density peak (rho (E^3 + 1.5 L E^2) rods of length L cross a cube of edge
-// E, by Cauchy's mean projected area), fine cells of about a quarter rod
-// length (the option sets the fraction) inside them.
+// Footprint engine: home cells sized so that about 64 rods (the occupancy
+// option) cross one at the mean density (rho (E^3 + 1.5 L E^2) rods of
+// length L cross a cube of edge E, by Cauchy's mean projected area); a work
+// unit that overflows its survivor table in a denser region is redone one
+// cell at a time.  Fine cells of about a fifth of a rod length (the fine
+// option sets the fraction) inside them.  Measured on 4M uniform 1 um rods,
+// 200k 4 um rods, 500k log-normal rods in agglomerates and 8M rods in 16
+// agglomerates: occupancy 32-64 and fine fraction 0.2 are within 20% of
+// the best for each; from 96 on, units are redone cell by cell too often.
 static void footprint_cells(nf_handle* h, uint32_t n, double* edge, int* fine) {
   const nf_params& p = h->params[0];
   const ContactBuffers& c = h->cb;
   const double typ = p.len_law == NF_LEN_LOGNORMAL ? p.len : (double)c.hl_max * 2.0;
   const double V = p.box[0] * p.box[1] * p.box[2];
-  double peak = 1.0;
-  if (p.den_law == NF_DEN_AGGLOM) peak = p.agg_peak;
-  if (p.den_law == NF_DEN_MESH) peak = 3.0;
-  const double rho = (double)n / V * peak;
+  const double rho = (double)n / V;
+  const double occ = h->opt_footprint_occ > 0 ? (double)h->opt_footprint_occ : 64.0;
   double lo = 1e-3, hi = std::max(p.box[0], std::max(p.box[1], p.box[2]));
   for (int it = 0; it < 60; ++it) {
     const double e = 0.5 * (lo + hi);
-    if (rho * (e * e * e + 1.5 * typ * e * e) > 48.0)
+    if (rho * (e * e * e + 1.5 * typ * e * e) > occ)
       hi = e;
     else
       lo = e;
   }
   *edge = std::max(lo, 4.0 * ((double)c.a_max + c.gap_cut));
-  const double ff = h->opt_footprint_cell > 0 ? h->opt_footprint_cell * 1e-3 : 0.25;
+  const double ff = h->opt_footprint_fine > 0 ? h->opt_footprint_fine * 1e-3 : 0.2;
   int f = (int)std::lround(*edge / std::max(typ * ff, 1e-6));
   *fine = f < 1 ? 1 : (f > 4 ? 4 : f);
 }
@@ -203,6 +206,8 @@ static int contacts_pass(nf_handle* h, double gap, uint64_t* n_out, int prec, bo
     double edge = 0.0;
     int fine = 1;
     footprint_cells(h, n, &edge, &fine);
+    c.fp_edge = edge;
+    c.fp_fine = fine;
     xa.g = seg_grid(p.box, edge, (double)amax, gap, ew);
     const double rpad = (double)xa.g.r + 2.0 * (double)CELL_MARGIN;
     rc = build_cell_list(h, edge, (double)hlmax * REACH_PAD + rpad, false, SEG_ROW);
@@ -383,7 +388,7 @@ static int contacts_pass(nf_handle* h, double gap, uint64_t* n_out, int prec, bo
   launch_row_sort(c.rowptr, n, c.J, c.gap, c.gap64, c.img, s);
   launch_contact_area(P, D, n, c.rowptr, c.J, c.img, p.box, gap, c.area, s);
   hipEventRecord(ev[5], s);
-  HIPCHK(h, hipMemcpyAsync(h->cb_stats, ctr, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipMemcpyAsync(h->cb_stats, ctr, sizeof(unsigned long long) * 6, hipMemcpyDeviceToHost, s));
   HIPCHK(h, sync_stream(s, "contact list"));
   trace("contacts: %u recorded and sorted", total);
   HIPCHK(h, hipGetLastError());
@@ -396,6 +401,10 @@ static int contacts_pass(nf_handle* h, double gap, uint64_t* n_out, int prec, bo
   free_all();
   c.valid = true;
   c.engine = cells ? NF_ENGINE_CELLS : NF_ENGINE_FOOTPRINT;
+  if (cells) {
+    c.fp_edge = 0.0;
+    c.fp_fine = 0;
+  }
   if (n_out) *n_out = total;
   return NF_OK;
 }
@@ -469,6 +478,17 @@ extern "C" int nf_contacts_stats(nf_handle* h, uint64_t* pair_tests, uint64_t* r
   if (kernel_ms)
     for (int k = 0; k < 4; ++k) kernel_ms[k] = h->cb_ms[k];
   if (tile_bytes) *tile_bytes = h->cb_stats[3];
+  return NF_OK;
+}
+
+extern "C" int nf_contacts_engine(nf_handle* h, int* engine, double* home_edge, int* fine,
+                                  uint64_t* split_units) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  if (!h->cb.valid) return fail(h, NF_E_STATE, "no contacts computed");
+  if (engine) *engine = h->cb.engine;
+  if (home_edge) *home_edge = h->cb.fp_edge;
+  if (fine) *fine = h->cb.fp_fine;
+  if (split_units) *split_units = h->cb.engine == NF_ENGINE_FOOTPRINT ? h->cb_stats[5] : 0;
   return NF_OK;
 }
 

@@ -23,8 +23,9 @@ struct nf_handle {
   hipEvent_t ev[9] = {};
   std::string err;
   int opt_graph = 0, opt_profile = 0, opt_diag = 0, opt_contact_prec = -1;
-  int opt_contact_engine = NF_ENGINE_CELLS;
-  int opt_footprint_cell = 0;  // footprint cell edge in 1/1000 of the typical rod length (0: 250)
+  int opt_contact_engine = NF_ENGINE_FOOTPRINT;
+  int opt_footprint_fine = 0;  // footprint fine cell edge in 1/1000 of the typical rod length (0: 200)
+  int opt_footprint_occ = 0;   // footprint home cell: rods crossing it at the mean density (0: 64)
   int device = 0;
   std::vector<uint32_t> n_rods;  // current rod count per instance
   bool generated = false;
@@ -33,7 +34,7 @@ struct nf_handle {
   std::unordered_map<void*, size_t> allocs;
   std::multimap<size_t, void*> pool;
   size_t pool_bytes = 0;
-  unsigned long long cb_stats[4] = {0, 0, 0, 0};  // contact pair tests, rechecks, max dd bits
+  unsigned long long cb_stats[6] = {0, 0, 0, 0, 0, 0};  // contact pair tests, rechecks, max dd bits
   float cb_ms[4] = {0, 0, 0, 0};                  // contact kernel times: bin, pair pass, CSR build, row sort
   uint64_t contact_hint = 0;                      // contacts found by the last call (append capacity)
   nf_network_info net_info = {};

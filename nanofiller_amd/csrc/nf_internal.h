@@ -160,6 +160,8 @@ struct ContactBuffers {
   double gap_cut = 0.0;
   float hl_max = 0.0f, a_max = 0.0f;  // bounds on half length and radius of the rods
   int engine = 0;                     // NF_ENGINE_* that found the last contact list
+  double fp_edge = 0.0;               // footprint engine: home cell edge (um) and fine cells
+  int fp_fine = 0;                    // per axis of the last pass (0: cell engine)
   bool valid = false;
   // network
   uint32_t* labels = nullptr;
@@ -260,7 +262,8 @@ struct SegCrossArgs {
   float* outGap;
   double* outGap64;
   int32_t* outImg;
-  unsigned long long* ctr;  // [0] capsule tests, [1] rechecks, [2] max dd bits, [3] bytes
+  unsigned long long* ctr;  // [0] capsule tests, [1] rechecks, [2] max dd bits, [3] bytes,
+                            // [5] units redone one cell at a time
 };
 
 // ---- launchers (nf_scan.hip) ----

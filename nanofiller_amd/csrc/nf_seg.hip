@@ -31,8 +31,9 @@
 //    survive in that cell and share the midpoint's fine cell) and reported
 //    once.
 //
-// A unit with more than SMAX survivors raises the overflow flag and is
-// skipped; the host then runs the cell-pair engine instead.  Control flow
+// A unit with more than SMAX survivors is redone one home cell at a time; a
+// single cell with more raises the overflow flag and is skipped, and the
+// host then runs the cell-pair engine instead.  Control flow
 // outside seg_screen and seg_full is wave-uniform; cross-lane operations run
 // with every lane active.
 #include "nf_internal.h"
@@ -128,16 +129,221 @@ __device__ __forceinline__ void x_screen(const SegCrossArgs& a, XLds& L, uint32_
   }
 }
 
+// Gather of the run of ws home cells from x = xs in row (h1, h2): the rods
+// whose capsule crosses one of them go to the survivor table with the cells
+// they cross.  Returns their count; past SMAX only the first SMAX are kept.
+__device__ __forceinline__ uint32_t x_gather(const SegCrossArgs& a, XLds& L, int xs, int ws, int h1, int h2,
+                                             float mx, unsigned long long& bytes) {
+  const uint32_t lane = threadIdx.x;
+  float lo[3], hi[3];
+  lo[0] = (float)xs * a.g.h[0];
+  hi[0] = (float)(xs + ws) * a.g.h[0];
+  lo[1] = (float)h1 * a.g.h[1];
+  hi[1] = (float)(h1 + 1) * a.g.h[1];
+  lo[2] = (float)h2 * a.g.h[2];
+  hi[2] = (float)(h2 + 1) * a.g.h[2];
+  uint32_t S = 0;
+  for (uint32_t sb = 0; sb < a.st_n; sb += WAVE) {
+    const uint32_t si = sb + lane;
+    const bool sv = si < a.st_n;
+    const int4 so = a.stencil[sv ? si : a.st_n - 1u];
+    const float md2 = __int_as_float(so.w);
+    const int raw[3] = {xs + so.x, h1 + so.y, h2 + so.z};
+    int m[3], im[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int q = floor_div(raw[k], a.g.nc[k]);
+      im[k] = q;
+      m[k] = raw[k] - q * a.g.nc[k];
+    }
+    const uint32_t nc = (uint32_t)(m[0] + a.g.nc[0] * (m[1] + a.g.nc[1] * m[2]));
+    const uint32_t n = a.ccount[nc];
+    const float hl = a.chl[nc];
+    const uint32_t st0 = a.cstart[nc];
+    const float rr = hl * 1.000001f + a.rpad;
+    const bool rel = sv && n > 0u && md2 <= rr * rr;
+    const bool beyond = sv && md2 > a.reach2;
+    const uint32_t meta = seg_pack(im[0], im[1], im[2]);
+    // the block's relevant cells back to back: inclusive prefix of counts
+    const uint32_t cnt = rel ? n : 0u;
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_up((int)incl, o, WAVE);
+      incl += lane >= (uint32_t)o ? t : 0u;
+    }
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, WAVE - 1);
+    L.gpre[lane] = incl - cnt;
+    L.gst[lane] = st0;
+    L.gmeta[lane] = meta;
+    __syncthreads();
+    // rod pos of the packed list: the last lane whose exclusive prefix is
+    // <= pos holds it (lanes without rods share the next lane's prefix).
+    // An empty cell's start can be one past the last rod, so idle lanes
+    // load rod 0, which exists whenever total > 0.
+    float4 P = make_float4(0.f, 0.f, 0.f, 0.f), D = P;
+    uint32_t id = 0u, cm = 0u;
+    bool v = false;
+    auto fetch = [&](uint32_t b0) {
+      const uint32_t pos = b0 + lane;
+      v = pos < total;
+      uint32_t k = 0;
+#pragma unroll
+      for (uint32_t step = WAVE / 2; step > 0; step >>= 1)
+        if (L.gpre[k + step] <= pos) k += step;
+      cm = L.gmeta[k];
+      const uint32_t src = v ? L.gst[k] + (pos - L.gpre[k]) : 0u;
+      P = a.cP[src];
+      D = a.cD[src];
+      id = a.cid[src];
+    };
+    if (total > 0u) fetch(0u);
+    for (uint32_t b0 = 0; b0 < total; b0 += WAVE) {
+      // this batch, then the next batch's loads in flight while it is tested
+      const float4 bP = P, bD = D;
+      const uint32_t bid = id, bcm = cm;
+      const bool bv = v;
+      if (b0 + WAVE < total) fetch(b0 + WAVE);
+      bytes += bv ? 36ull : 0ull;
+      const float p[3] = {bP.x + (float)ximg(bcm, 0) * a.x.boxf[0], bP.y + (float)ximg(bcm, 1) * a.x.boxf[1],
+                          bP.z + (float)ximg(bcm, 2) * a.x.boxf[2]};
+      const float c[3] = {bD.x, bD.y, bD.z};
+      const uint32_t cells = bv ? seg_row_cells(p, c, bP.w, lo, hi, xs, ws, a.g.h[0], a.g.r, mx) : 0u;
+      const bool keep = cells != 0u;
+      const unsigned long long b = __ballot(keep);
+      const uint32_t at = S + xprefix(b);
+      if (keep && at < (uint32_t)SMAX) {
+        L.sP[at] = bP;
+        L.sD[at] = bD;
+        L.sX[at] = bid;
+        L.sM[at] = bcm;
+        L.sC[at] = (uint8_t)cells;
+      }
+      S += (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    if (__ballot(beyond) != 0ull) break;
+  }
+  return S;
+}
+
+// Steps 2 and 3 for each home cell of the gathered run (S <= SMAX
+// survivors in the table).
+template <int PREC>
+__device__ __forceinline__ void x_cells(const SegCrossArgs& a, XLds& L, uint32_t S, int xs, int ws, int h1,
+                                        int h2, const float e[3], const float ie[3], uint32_t& tests,
+                                        uint32_t& rech, double& maxdd) {
+  const uint32_t lane = threadIdx.x;
+  const int f = a.fine;
+  const uint32_t F = (uint32_t)(f * f * f);  // fine cells per home cell, at most FMAX
+  for (int ci = 0; ci < ws; ++ci) {
+    const uint32_t cell =
+        (uint32_t)(xs + ci) + (uint32_t)a.g.nc[0] * ((uint32_t)h1 + (uint32_t)a.g.nc[1] * (uint32_t)h2);
+    const float clo[3] = {(float)(xs + ci) * a.g.h[0], (float)h1 * a.g.h[1], (float)h2 * a.g.h[2]};
+    // ---- 2. this cell's survivors, their fine masks and the fine cells' sets
+    uint32_t Sc = 0;
+    for (uint32_t w0 = 0; w0 < S; w0 += WAVE) {
+      const uint32_t idx = w0 + lane;
+      const bool in = idx < S && ((L.sC[idx] >> ci) & 1u) != 0u;
+      const unsigned long long b = __ballot(in);
+      if (in) L.lst[Sc + xprefix(b)] = (uint8_t)idx;
+      Sc += (uint32_t)__popcll(b);
+    }
+    // the fine cells' sets start empty (F <= 64 rows, one per lane)
+    if (lane < F) {
+#pragma unroll
+      for (int k = 0; k < SWORDS; ++k) L.mem[lane][k] = 0ull;
+    }
+    __syncthreads();
+    for (uint32_t wd = 0; wd * WAVE < Sc; ++wd) {
+      const uint32_t k = wd * WAVE + lane;
+      if (k < Sc) {
+        const uint32_t id = L.lst[k];
+        const float4 P = L.sP[id], D = L.sD[id];
+        const uint32_t cm = L.sM[id];
+        const float p[3] = {P.x + (float)ximg(cm, 0) * a.x.boxf[0], P.y + (float)ximg(cm, 1) * a.x.boxf[1],
+                            P.z + (float)ximg(cm, 2) * a.x.boxf[2]};
+        const float c[3] = {D.x, D.y, D.z};
+        const unsigned long long fm = seg_fine_mask(p, c, P.w, clo, e, ie, f, a.g.r);
+        L.fm[k] = fm;
+        // this survivor joins the set of each fine cell it touches (a few
+        // LDS atomics per lane instead of a ballot per fine cell)
+        unsigned long long t = fm;
+        while (t != 0ull) {
+          const int fc = (int)__builtin_ctzll(t);
+          t &= t - 1ull;
+          atomicOr(&L.mem[fc][wd], 1ull << lane);
+        }
+      }
+    }
+    __syncthreads();
+    // ---- 3. candidate pairs through the two queues
+    const uint32_t nw = (Sc + WAVE - 1) / WAVE;
+    uint32_t qn = 0, q2n = 0;
+    for (uint32_t wd = 0; wd < nw; ++wd) {
+      const uint32_t i = wd * WAVE + lane;
+      const bool v = i < Sc;
+      const uint32_t si = v ? (uint32_t)L.lst[i] : 0u;
+      // partners: later list positions sharing a fine cell
+      unsigned long long fm = v ? L.fm[i] : 0ull;
+      unsigned long long part[SWORDS];
+#pragma unroll
+      for (int k = 0; k < SWORDS; ++k) part[k] = 0ull;
+      while (fm != 0ull) {
+        const int fc = (int)__builtin_ctzll(fm);
+        fm &= fm - 1ull;
+#pragma unroll
+        for (int k = 0; k < SWORDS; ++k) part[k] |= L.mem[fc][k];
+      }
+      // keep j > i among the filled words
+#pragma unroll
+      for (int k = 0; k < SWORDS; ++k) {
+        if ((uint32_t)k < wd || (uint32_t)k >= nw) part[k] = 0ull;
+        if ((uint32_t)k == wd) part[k] &= (lane == 63u) ? 0ull : (~0ull << (lane + 1u));
+      }
+      for (;;) {
+        // this lane's next partner: the lowest set bit of the first
+        // non-empty word (unrolled: the words stay in registers)
+        bool have = false;
+        uint32_t j = 0;
+#pragma unroll
+        for (int k = 0; k < SWORDS; ++k) {
+          if (!have && part[k] != 0ull) {
+            j = (uint32_t)(k * 64) + (uint32_t)__builtin_ctzll(part[k]);
+            part[k] &= part[k] - 1ull;
+            have = true;
+          }
+        }
+        const unsigned long long bm = __ballot(have);
+        if (bm == 0ull) break;
+        if (have) L.q[qn + xprefix(bm)] = si | ((uint32_t)L.lst[j] << 8);
+        qn += (uint32_t)__popcll(bm);
+        if (qn >= (uint32_t)WAVE) {
+          __syncthreads();
+          x_screen<PREC>(a, L, qn - WAVE, WAVE, q2n, cell, tests, rech, maxdd);
+          qn -= WAVE;
+          __syncthreads();
+        }
+      }
+    }
+    if (qn > 0) {
+      __syncthreads();
+      x_screen<PREC>(a, L, 0, qn, q2n, cell, tests, rech, maxdd);
+      __syncthreads();
+    }
+    if (q2n > 0) x_full<PREC>(a, L, min(lane, q2n - 1u), lane < q2n, cell, tests, rech, maxdd);
+    __syncthreads();
+  }
+}
+
 template <int PREC>
 __global__ void __launch_bounds__(WAVE) k_seg_cross(SegCrossArgs a) {
   __shared__ XLds L;
   const uint32_t lane = threadIdx.x;
-  const int f = a.fine;
-  const uint32_t F = (uint32_t)(f * f * f);
   float e[3], ie[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    e[k] = a.g.h[k] / (float)f;
+    e[k] = a.g.h[k] / (float)a.fine;
     ie[k] = 1.0f / e[k];
   }
   const uint32_t nc0 = (uint32_t)a.g.nc[0], nc1 = (uint32_t)a.g.nc[1];
@@ -171,189 +377,27 @@ __global__ void __launch_bounds__(WAVE) k_seg_cross(SegCrossArgs a) {
     const int x0 = (int)((u - row * nseg) * (uint32_t)SEG_ROW);
     const int w = min(SEG_ROW, (int)nc0 - x0);
     const int h1 = (int)(row % nc1), h2 = (int)(row / nc1);
-    float lo[3], hi[3];
-    lo[0] = (float)x0 * a.g.h[0];
-    hi[0] = (float)(x0 + w) * a.g.h[0];
-    lo[1] = (float)h1 * a.g.h[1];
-    hi[1] = (float)(h1 + 1) * a.g.h[1];
-    lo[2] = (float)h2 * a.g.h[2];
-    hi[2] = (float)(h2 + 1) * a.g.h[2];
-    // ---- 1. gather the unit's survivors
-    uint32_t S = 0;
-    for (uint32_t sb = 0; sb < a.st_n; sb += WAVE) {
-      const uint32_t si = sb + lane;
-      const bool sv = si < a.st_n;
-      const int4 so = a.stencil[sv ? si : a.st_n - 1u];
-      const float md2 = __int_as_float(so.w);
-      const int raw[3] = {x0 + so.x, h1 + so.y, h2 + so.z};
-      int m[3], im[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int q = floor_div(raw[k], a.g.nc[k]);
-        im[k] = q;
-        m[k] = raw[k] - q * a.g.nc[k];
-      }
-      const uint32_t nc = (uint32_t)(m[0] + a.g.nc[0] * (m[1] + a.g.nc[1] * m[2]));
-      const uint32_t n = a.ccount[nc];
-      const float hl = a.chl[nc];
-      const uint32_t st0 = a.cstart[nc];
-      const float rr = hl * 1.000001f + a.rpad;
-      const bool rel = sv && n > 0u && md2 <= rr * rr;
-      const bool beyond = sv && md2 > a.reach2;
-      const uint32_t meta = seg_pack(im[0], im[1], im[2]);
-      // the block's relevant cells back to back: inclusive prefix of counts
-      const uint32_t cnt = rel ? n : 0u;
-      uint32_t incl = cnt;
-#pragma unroll
-      for (int o = 1; o < WAVE; o <<= 1) {
-        const uint32_t t = (uint32_t)__shfl_up((int)incl, o, WAVE);
-        incl += lane >= (uint32_t)o ? t : 0u;
-      }
-      const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, WAVE - 1);
-      L.gpre[lane] = incl - cnt;
-      L.gst[lane] = st0;
-      L.gmeta[lane] = meta;
-      __syncthreads();
-      // rod pos of the packed list: the last lane whose exclusive prefix is
-      // <= pos holds it (lanes without rods share the next lane's prefix).
-      // An empty cell's start can be one past the last rod, so idle lanes
-      // load rod 0, which exists whenever total > 0.
-      float4 P = make_float4(0.f, 0.f, 0.f, 0.f), D = P;
-      uint32_t id = 0u, cm = 0u;
-      bool v = false;
-      auto fetch = [&](uint32_t b0) {
-        const uint32_t pos = b0 + lane;
-        v = pos < total;
-        uint32_t k = 0;
-#pragma unroll
-        for (uint32_t step = WAVE / 2; step > 0; step >>= 1)
-          if (L.gpre[k + step] <= pos) k += step;
-        cm = L.gmeta[k];
-        const uint32_t src = v ? L.gst[k] + (pos - L.gpre[k]) : 0u;
-        P = a.cP[src];
-        D = a.cD[src];
-        id = a.cid[src];
-      };
-      if (total > 0u) fetch(0u);
-      for (uint32_t b0 = 0; b0 < total; b0 += WAVE) {
-        // this batch, then the next batch's loads in flight while it is tested
-        const float4 bP = P, bD = D;
-        const uint32_t bid = id, bcm = cm;
-        const bool bv = v;
-        if (b0 + WAVE < total) fetch(b0 + WAVE);
-        bytes += bv ? 36ull : 0ull;
-        const float p[3] = {bP.x + (float)ximg(bcm, 0) * a.x.boxf[0], bP.y + (float)ximg(bcm, 1) * a.x.boxf[1],
-                            bP.z + (float)ximg(bcm, 2) * a.x.boxf[2]};
-        const float c[3] = {bD.x, bD.y, bD.z};
-        const uint32_t cells = bv ? seg_row_cells(p, c, bP.w, lo, hi, x0, w, a.g.h[0], a.g.r, mx) : 0u;
-        const bool keep = cells != 0u;
-        const unsigned long long b = __ballot(keep);
-        const uint32_t at = S + xprefix(b);
-        if (keep && at < (uint32_t)SMAX) {
-          L.sP[at] = bP;
-          L.sD[at] = bD;
-          L.sX[at] = bid;
-          L.sM[at] = bcm;
-          L.sC[at] = (uint8_t)cells;
-        }
-        S += (uint32_t)__popcll(b);
-      }
-      __syncthreads();
-      if (__ballot(beyond) != 0ull) break;
-    }
-    if (S > (uint32_t)SMAX) {
-      if (lane == 0) atomicOr(a.overflow, 1u);
-      __syncthreads();
-      continue;
-    }
-    for (int ci = 0; ci < w; ++ci) {
-      const uint32_t cell = (uint32_t)(x0 + ci) + nc0 * ((uint32_t)h1 + nc1 * (uint32_t)h2);
-      const float clo[3] = {(float)(x0 + ci) * a.g.h[0], lo[1], lo[2]};
-      // ---- 2. this cell's survivors, their fine masks and the fine cells' sets
-      uint32_t Sc = 0;
-      for (uint32_t w0 = 0; w0 < S; w0 += WAVE) {
-        const uint32_t idx = w0 + lane;
-        const bool in = idx < S && ((L.sC[idx] >> ci) & 1u) != 0u;
-        const unsigned long long b = __ballot(in);
-        if (in) L.lst[Sc + xprefix(b)] = (uint8_t)idx;
-        Sc += (uint32_t)__popcll(b);
-      }
-      __syncthreads();
-      for (uint32_t wd = 0; wd * WAVE < Sc; ++wd) {
-        const uint32_t k = wd * WAVE + lane;
-        unsigned long long fm = 0ull;
-        if (k < Sc) {
-          const uint32_t id = L.lst[k];
-          const float4 P = L.sP[id], D = L.sD[id];
-          const uint32_t cm = L.sM[id];
-          const float p[3] = {P.x + (float)ximg(cm, 0) * a.x.boxf[0], P.y + (float)ximg(cm, 1) * a.x.boxf[1],
-                              P.z + (float)ximg(cm, 2) * a.x.boxf[2]};
-          const float c[3] = {D.x, D.y, D.z};
-          fm = seg_fine_mask(p, c, P.w, clo, e, ie, f, a.g.r);
-          L.fm[k] = fm;
-        }
-        for (uint32_t fc = 0; fc < F; ++fc) {
-          const unsigned long long b = __ballot(((fm >> fc) & 1ull) != 0ull);
-          if (lane == 0) L.mem[fc][wd] = b;
-        }
-      }
-      __syncthreads();
-      // ---- 3. candidate pairs through the two queues
-      const uint32_t nw = (Sc + WAVE - 1) / WAVE;
-      uint32_t qn = 0, q2n = 0;
-      for (uint32_t wd = 0; wd < nw; ++wd) {
-        const uint32_t i = wd * WAVE + lane;
-        const bool v = i < Sc;
-        const uint32_t si = v ? (uint32_t)L.lst[i] : 0u;
-        // partners: later list positions sharing a fine cell
-        unsigned long long fm = v ? L.fm[i] : 0ull;
-        unsigned long long part[SWORDS];
-#pragma unroll
-        for (int k = 0; k < SWORDS; ++k) part[k] = 0ull;
-        while (fm != 0ull) {
-          const int fc = (int)__builtin_ctzll(fm);
-          fm &= fm - 1ull;
-#pragma unroll
-          for (int k = 0; k < SWORDS; ++k) part[k] |= L.mem[fc][k];
-        }
-        // keep j > i among the filled words
-#pragma unroll
-        for (int k = 0; k < SWORDS; ++k) {
-          if ((uint32_t)k < wd || (uint32_t)k >= nw) part[k] = 0ull;
-          if ((uint32_t)k == wd) part[k] &= (lane == 63u) ? 0ull : (~0ull << (lane + 1u));
-        }
-        for (;;) {
-          // this lane's next partner: the lowest set bit of the first
-          // non-empty word (unrolled: the words stay in registers)
-          bool have = false;
-          uint32_t j = 0;
-#pragma unroll
-          for (int k = 0; k < SWORDS; ++k) {
-            if (!have && part[k] != 0ull) {
-              j = (uint32_t)(k * 64) + (uint32_t)__builtin_ctzll(part[k]);
-              part[k] &= part[k] - 1ull;
-              have = true;
-            }
-          }
-          const unsigned long long bm = __ballot(have);
-          if (bm == 0ull) break;
-          if (have) L.q[qn + xprefix(bm)] = si | ((uint32_t)L.lst[j] << 8);
-          qn += (uint32_t)__popcll(bm);
-          if (qn >= (uint32_t)WAVE) {
-            __syncthreads();
-            x_screen<PREC>(a, L, qn - WAVE, WAVE, q2n, cell, tests, rech, maxdd);
-            qn -= WAVE;
-            __syncthreads();
-          }
-        }
-      }
-      if (qn > 0) {
-        __syncthreads();
-        x_screen<PREC>(a, L, 0, qn, q2n, cell, tests, rech, maxdd);
+    // the whole run, or, when its survivors overflow the table, one cell at
+    // a time; a single cell that overflows raises the flag (the host then
+    // runs the cell-pair engine)
+    int xs = x0, ws = w;
+    for (;;) {
+      const uint32_t S = x_gather(a, L, xs, ws, h1, h2, mx, bytes);
+      if (S <= (uint32_t)SMAX) {
+        x_cells<PREC>(a, L, S, xs, ws, h1, h2, e, ie, tests, rech, maxdd);
+      } else if (ws > 1) {
+        if (lane == 0) atomicAdd(&a.ctr[5], 1ull);
+        ws = 1;
+        continue;
+      } else {
+        if (lane == 0) atomicOr(a.overflow, 1u);
         __syncthreads();
       }
-      if (q2n > 0) x_full<PREC>(a, L, min(lane, q2n - 1u), lane < q2n, cell, tests, rech, maxdd);
-      __syncthreads();
+      if (ws == 1 && xs + 1 < x0 + w) {
+        ++xs;
+        continue;
+      }
+      break;
     }
   }
   // one atomic per counter per wave, every lane converged

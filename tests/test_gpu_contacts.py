@@ -36,17 +36,30 @@ def test_contacts_parity(case, prec, engine):
         assert st["pair_tests"] == o["pair_tests"]  # pairs within centre reach
 
 
-@pytest.mark.parametrize("cell", [100, 250, 600, 5000])
-def test_footprint_cell_edge_does_not_change_contacts(cell):
-    """Any footprint cell edge (a tenth of a rod length to more than the box)
-    gives the same contact list."""
+@pytest.mark.parametrize("fine,occ,engine", [(100, 0, nf.NF_ENGINE_FOOTPRINT), (250, 0, nf.NF_ENGINE_FOOTPRINT),
+                                              (600, 0, nf.NF_ENGINE_FOOTPRINT), (5000, 0, nf.NF_ENGINE_FOOTPRINT),
+                                              (0, 8, nf.NF_ENGINE_FOOTPRINT), (0, 90, nf.NF_ENGINE_FOOTPRINT),
+                                              (0, 200, nf.NF_ENGINE_CELLS)])
+def test_footprint_cell_sizes_do_not_change_contacts(fine, occ, engine):
+    """Any footprint fine cell (a tenth of a rod length to a whole home cell)
+    and home cell size gives the same contact list: runs of home cells whose
+    crossing rods overflow the survivor table in the agglomerates are redone
+    one cell at a time (occupancy 90), and home cells so large that one cell
+    overflows fall back to the cell engine (200), with the same list."""
     p = cases.small_cfg2(nf.NF_PREC_FP32_RECHECK)
     rods = _packing(p)
     o = oracle.contacts(p, rods, p.contact_gap, threads=16)
-    with nf.Packing(p, {nf.NF_OPT_FOOTPRINT_CELL: cell}) as pk:
+    opts = {nf.NF_OPT_CONTACT_ENGINE: nf.NF_ENGINE_FOOTPRINT, nf.NF_OPT_FOOTPRINT_FINE: fine,
+            nf.NF_OPT_FOOTPRINT_OCC: occ}
+    with nf.Packing(p, opts) as pk:
         pk.set_rods(rods)
         c = pk.contacts(p.contact_gap)
         g64 = pk.contact_gap64(c.n)
+        used = pk.contact_engine()
+    print(fine, occ, used, flush=True)
+    assert used["engine"] == engine
+    if occ == 90:
+        assert used["split_units"] > 0
     np.testing.assert_array_equal(c.I, o["I"])
     np.testing.assert_array_equal(c.col, o["J"])
     np.testing.assert_array_equal(c.image, o["image"])
