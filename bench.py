@@ -71,6 +71,22 @@ def profiled_traffic():
         return None
 
 
+CONTACT_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "contacts_4M_hbm_counters.md")
+
+
+def profiled_contact_traffic():
+    """HBM bytes of one 4M-rod footprint contact pass from the committed
+    FETCH_SIZE / WRITE_SIZE summary (calibrated), or None when absent."""
+    import re
+
+    try:
+        with open(CONTACT_TRAFFIC_SUMMARY) as f:
+            m = re.search(r"traffic_bytes_per_pass k_seg_cross<1>: (\d+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def profiled_valu(key: str):
     """SQ_INSTS_VALU per launch of one pair kernel on one workload from the
     committed rocprofv3 counter summary (lines 'valu_per_launch <key>: N'),
@@ -353,6 +369,8 @@ def leg_contacts(a, out: dict) -> None:
         "pair_kernel_roofline": {"bound": "hbm", "achieved": alg_achieved, "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": alg_achieved / HBM_PEAK_GBS,
                                   "algorithmic_bytes_per_pass": alg_bytes,
+                                  "traffic": profiled_contact_traffic(),
+                                  "traffic_source": os.path.relpath(CONTACT_TRAFFIC_SUMMARY, ROOT),
                                   "kernel": "k_seg_cross (footprint engine, FP32+recheck)",
                                   "valu": valu_roof("k_seg_cross 4M", km["pairs"])},
         "cell_engine": {"contacts_call_ms": t_cc * 1e3, "kernel_ms": csc["kernel_ms"],

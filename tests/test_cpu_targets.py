@@ -55,3 +55,9 @@ def test_polar_bins_against_candidates(law, kappa, axis):
 def test_chi2_merges_small_bins():
     r = targets.chi2_test([0, 1, 100, 100, 2], [0.0005, 0.0005, 0.49, 0.49, 0.019])
     assert r["dof"] == 1 and r["n"] == 203  # bins {0,1,2}, {3,4}
+
+
+def test_total_variation_distance():
+    # exact frequencies: no distance; moving 10% of the mass between two bins: 0.1
+    assert targets.chi2_test([25, 25, 50], [0.25, 0.25, 0.5])["tv"] == 0.0
+    assert abs(targets.chi2_test([350, 150, 500], [0.25, 0.25, 0.5])["tv"] - 0.1) < 1e-12
