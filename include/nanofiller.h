@@ -146,6 +146,10 @@ int nf_create(const nf_params* p, nf_handle** out);
 /* n independent packings generated together (batched persistent rounds). */
 int nf_create_batch(const nf_params* ps, int n, nf_handle** out);
 void nf_destroy(nf_handle* h);
+/* The normalised parameters of instance inst (no counterpart in the
+ * reference, whose simPrms fields are read directly; e.g. the box of a
+ * handle made by nf_expand). */
+int nf_get_params(nf_handle* h, int inst, nf_params* out);
 /* Message of the last failure on h (h == NULL: last failure of nf_create*). */
 const char* nf_last_error(const nf_handle* h);
 /* Number of instances held by h. */
@@ -409,6 +413,34 @@ int nf_read_state_header(const char* path, nf_state_header* out);
  * simPrms::asmbl_cnt_data (src/simPrms.cpp:329-391).  The file's box must
  * equal the handle's; the rod count must not exceed the handle's target. */
 int nf_load_state(nf_handle* h, const char* path, uint32_t* n_rods);
+
+/* Milestone state dumps during nf_generate (VOLUME_PERCENT_LIST,
+ * src/cudaMain.cu:136-142, listed at src/simPrms.cpp:246-251): after each
+ * round whose packing has reached the next listed volume fraction (percent,
+ * N pi a^2 l / V * 100 with the nominal a and l), the packing is written as
+ * nf_save_state to <dir>/state_<vf>.bin, vf the fraction reached ("%4.2f"),
+ * and every listed value reached by then is passed, as the reference's one
+ * check per round does.  The packing is the same as without milestones; the
+ * rounds are synchronised with the host one at a time while a milestone is
+ * pending.  n = 0 clears the list.  Single-instance handles. */
+int nf_set_milestones(nf_handle* h, const double* vf_percent, int32_t n, const char* dir);
+/* The VOLUME_PERCENT_LIST of a parameter file ("VOLUME_PERCENT_LIST n v1 ..
+ * vn", percent): returns n (0 when the keyword is absent) and fills up to
+ * cap values. */
+int nf_params_milestones(const char* path, double* vf_percent, int32_t cap);
+
+/* expand (simPrms::expand, src/simPrms.cpp:744-852; verb at
+ * src/NanoFiller.cpp:67-69): a new single-instance handle *out holding
+ * instance 0's packing with its box grown along axis k = dir >> 1 (dir the
+ * reference's bit mask: 1 = x, 2 = y, 4 = z) by a buffer of periodic copies
+ * on each side.  The buffer is B = ceil((l_max + contact_gap) / E) E, E the
+ * generation cell edge along k (the reference's NeiOrder cells of the
+ * domain grid); it must not exceed the box.  Copies of the rods whose centre
+ * lies within B of the far face go below the box, copies of those within B
+ * of the near face above it, and every coordinate along k is shifted by B so
+ * that the new box is [0, L + 2B).  Rod i keeps id i; the copies follow in id
+ * order, first those below, then those above.  h is left unchanged. */
+int nf_expand(nf_handle* h, int dir, nf_handle** out);
 
 /* ---- library ----------------------------------------------------------- */
 

@@ -251,6 +251,10 @@ _SIGS = [
     ("nf_generate", C.c_int, [C.c_void_p, C.POINTER(GenStats)]),
     ("nf_get_rods", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("nf_set_rods", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32]),
+    ("nf_get_params", C.c_int, [C.c_void_p, C.c_int, C.POINTER(Params)]),
+    ("nf_expand", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    ("nf_set_milestones", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_char_p]),
+    ("nf_params_milestones", C.c_int, [C.c_char_p, C.c_void_p, C.c_int32]),
     ("nf_contacts", C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_uint64)]),
     ("nf_get_contacts", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("nf_get_contact_gap64", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
@@ -311,6 +315,16 @@ def lib() -> C.CDLL:
             f.argtypes = args
         _lib_handle = L
     return _lib_handle
+
+
+def params_milestones(path: str) -> list[float]:
+    """The VOLUME_PERCENT_LIST of a parameter file (percent)."""
+    n = lib().nf_params_milestones(path.encode(), None, 0)
+    if n < 0:
+        _check(n)
+    out = np.zeros(max(n, 1), np.float64)
+    lib().nf_params_milestones(path.encode(), out.ctypes.data, n)
+    return [float(x) for x in out[:n]]
 
 
 def exported_symbols() -> list[str]:
@@ -401,7 +415,11 @@ class Contacts:
 class Packing:
     """One handle: a single packing or a batch of independent packings."""
 
-    def __init__(self, params, options: dict | None = None):
+    def __init__(self, params, options: dict | None = None, _handle=None):
+        if _handle is not None:
+            self._h = _handle  # a handle the library made (nf_expand)
+            self.n_inst = 1
+            return
         plist = [params] if isinstance(params, Params) else list(params)
         arr = (Params * len(plist))(*plist)
         h = C.c_void_p()
@@ -410,6 +428,25 @@ class Packing:
         self.n_inst = len(plist)
         for k, v in (options or {}).items():
             self.set_option(k, v)
+
+    def params(self, inst: int = 0) -> Params:
+        """The instance's normalised parameters."""
+        p = Params()
+        _check(lib().nf_get_params(self._h, inst, C.byref(p)), self._h)
+        return p
+
+    def expand(self, direction: int) -> "Packing":
+        """A new packing: this one with periodic copies along the axis of
+        `direction` (bit mask 1 = x, 2 = y, 4 = z), nf_expand."""
+        g = C.c_void_p()
+        _check(lib().nf_expand(self._h, int(direction), C.byref(g)), self._h)
+        return Packing(None, _handle=g)
+
+    def set_milestones(self, vf_percent, directory: str = "") -> None:
+        """State dumps at volume fractions (percent) during generate()."""
+        v = np.ascontiguousarray(vf_percent, np.float64)
+        _check(lib().nf_set_milestones(self._h, v.ctypes.data if len(v) else None, len(v),
+                                       directory.encode()), self._h)
 
     def close(self) -> None:
         if getattr(self, "_h", None):

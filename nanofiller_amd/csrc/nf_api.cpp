@@ -685,6 +685,48 @@ static int build_graph(nf_handle* h, int prec, int diag) {
   return NF_OK;
 }
 
+// Milestone dumps (VOLUME_PERCENT_LIST, src/cudaMain.cu:136-142): after a
+// round, instance 0's volume fraction from its accepted count; when it has
+// reached the next listed value, its packing so far (the accepted rods sit
+// at the front of its rod buffer, in acceptance order) is saved as a state
+// file named by the fraction reached, and every value reached is passed.
+// The reference compares the fraction in percent with the list converted
+// to fractions (a hundredfold early trigger) and reads past the list's end
+// after its last value; here both are percent and the list ends the dumps.
+static int milestone_check(nf_handle* h, size_t* next) {
+  if (*next >= h->milestones.size()) return NF_OK;
+  InstState st0{};
+  HIPCHK(h, hipMemcpy(&st0, h->gb.st, sizeof(InstState), hipMemcpyDeviceToHost));
+  const nf_params& p = h->params[0];
+  const double PI = 3.14159265358979323846;
+  const double vf = (double)st0.n_acc * PI * p.radius * p.radius * p.len / (p.box[0] * p.box[1] * p.box[2]) * 100.0;
+  if (vf < h->milestones[*next]) return NF_OK;
+  char name[64];
+  std::snprintf(name, sizeof(name), "state_%4.2f.bin", vf);
+  const std::string path = h->milestone_dir.empty() ? std::string(name) : h->milestone_dir + "/" + name;
+  const uint32_t keep = h->n_rods[0];
+  h->n_rods[0] = st0.n_acc;
+  const int rc = nf_save_state(h, path.c_str());
+  h->n_rods[0] = keep;
+  if (rc != NF_OK) return rc;
+  trace("generate: milestone %s (%u rods)", path.c_str(), st0.n_acc);
+  while (*next < h->milestones.size() && h->milestones[*next] <= vf) ++*next;
+  return NF_OK;
+}
+
+extern "C" int nf_set_milestones(nf_handle* h, const double* vf_percent, int32_t n, const char* dir) {
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  if (n < 0 || (n > 0 && !vf_percent)) return fail(h, NF_E_INVALID, "bad milestone list");
+  if (n > 0 && h->params.size() != 1) return fail(h, NF_E_INVALID, "milestones need a single-instance handle");
+  std::vector<double> v(vf_percent, vf_percent + n);
+  for (size_t i = 0; i < v.size(); ++i)
+    if (!(v[i] > 0.0) || (i > 0 && !(v[i] > v[i - 1])))
+      return fail(h, NF_E_INVALID, "milestones must be positive and ascending");
+  h->milestones = v;
+  h->milestone_dir = dir ? dir : "";
+  return NF_OK;
+}
+
 extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   h->err.clear();
@@ -712,6 +754,7 @@ extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
   int batch = 1;
   std::vector<std::array<hipEvent_t, 8>> prof_ev;
   size_t prof_used = 0;
+  size_t next_ms = 0;  // next milestone (volume fraction) to dump
   hipError_t e = hipSuccess;
   for (;;) {
     for (int k = 0; k < batch && e == hipSuccess; ++k) {
@@ -738,10 +781,19 @@ extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
     ri = *ri_host;
     trace("generate: %u rounds launched, controller rounds %u, running %u, last T %u", launched,
           ri.rounds, ri.running, ri.T);
+    if (!h->milestones.empty()) {
+      const int mrc = milestone_check(h, &next_ms);
+      if (mrc != NF_OK) {
+        (void)hipHostFree(ri_host);
+        return mrc;
+      }
+    }
     if (ri.running == 0 || launched > max_rounds + 2) break;
     // next batch: the rounds the controller expects to need at the last
-    // round's acceptance (at most 8 between host checks)
+    // round's acceptance (at most 8 between host checks); one at a time
+    // while a milestone is pending
     batch = (int)std::min<uint32_t>(std::max<uint32_t>(ri.est, 1u), 8u);
+    if (next_ms < h->milestones.size()) batch = 1;
   }
   const auto t1 = std::chrono::steady_clock::now();
   (void)hipHostFree(ri_host);
@@ -846,6 +898,12 @@ extern "C" int nf_set_rods(nf_handle* h, int inst, const float* rods, uint32_t n
   return NF_OK;
 }
 
+
+extern "C" int nf_get_params(nf_handle* h, int inst, nf_params* out) {
+  if (!h || !out || inst < 0 || inst >= (int)h->params.size()) return fail(h, NF_E_INVALID, "bad instance");
+  *out = h->params[inst];
+  return NF_OK;
+}
 
 extern "C" int nf_agglomerate_centres(const nf_params* p, double* out, int32_t cap) {
   if (!p) return fail(nullptr, NF_E_INVALID, "null parameters");

@@ -2,7 +2,7 @@
 // main (src/NanoFiller.cpp:48-149):
 //
 //   nanofiller params.txt generate [write_vtk] [stat_phitheta]
-//              [stat_mutangle] [stat_contacts] [reduce d]
+//              [stat_mutangle] [stat_contacts] [expand d] [reduce d]
 //              [stat_conduct d [internal]]
 //
 // d is the field direction as the reference's bit mask: 1 = x, 2 = y, 4 = z
@@ -10,7 +10,12 @@
 // src/conductivity.cu:52-58; expand uses axis d >> 1, src/simPrms.cpp:760).
 //
 // generate          insert/reject generation (cudaMain); prints the count and
-//                   the volume fraction reached
+//                   the volume fraction reached; with VOLUME_PERCENT_LIST in
+//                   the parameter file, state_<vf>.bin milestone dumps
+//                   (src/cudaMain.cu:136-142)
+// expand d          periodic-copy buffer along direction d (simPrms::expand,
+//                   src/simPrms.cpp:744-852): the later verbs see the
+//                   expanded packing and box
 // write_vtk         output.vtk (write_CNTvtk, src/IO.cpp:38-76)
 // stat_phitheta     rho.dat, 32 x 32 phi-theta counts (collect_PhiTheta_distr,
 //                   src/statistics_collection.cu:769-861)
@@ -89,7 +94,7 @@ int main(int argc, char** argv) {
   if (argc < 3) {
     std::fprintf(stderr,
                  "usage: %s params.txt generate [write_vtk] [stat_phitheta] [stat_mutangle]\n"
-                 "       [stat_contacts] [stat_vdw] [reduce d] [stat_conduct d [internal]]\n"
+                 "       [stat_contacts] [stat_vdw] [expand d] [reduce d] [stat_conduct d [internal]]\n"
                  "       d: field direction 1 (x), 2 (y) or 4 (z)\n",
                  argv[0]);
     return 2;
@@ -118,6 +123,17 @@ int main(int argc, char** argv) {
   nf_gen_stats st;
   std::memset(&st, 0, sizeof(st));
   if (gen) {
+    const int nm = nf_params_milestones(argv[1], nullptr, 0);
+    if (nm > 0) {
+      std::vector<double> ms((size_t)nm);
+      nf_params_milestones(argv[1], ms.data(), nm);
+      rc = nf_set_milestones(h, ms.data(), nm, ".");
+      if (rc != NF_OK) {
+        const int r = die(h, "VOLUME_PERCENT_LIST", rc);
+        nf_destroy(h);
+        return r;
+      }
+    }
     rc = nf_generate(h, &st);
     if (rc != NF_OK) {
       const int r = die(h, "generate", rc);
@@ -220,6 +236,29 @@ int main(int argc, char** argv) {
       }
       std::printf("van der Waals energy over %llu contacting pairs (gap < %g um): %.10e\n",
                   (unsigned long long)np_, p.contact_gap, ev);
+    } else if (!std::strcmp(v, "expand")) {
+      const int d = ai + 1 < argc ? std::atoi(argv[++ai]) : 0;
+      if (dir_axis(std::to_string(d).c_str()) < 0) {
+        std::fprintf(stderr, "nanofiller: expand needs a direction 1, 2 or 4\n");
+        status = 2;
+        break;
+      }
+      std::printf("Expanding in %d direction:... \n", d);
+      nf_handle* g = nullptr;
+      rc = nf_expand(h, d, &g);
+      if (rc != NF_OK) {
+        status = die(h, "expand", rc);
+      } else {
+        // the expanded packing is the state of the later verbs
+        nf_destroy(h);
+        h = g;
+        nf_get_params(h, 0, &p);
+        nf_get_rods(h, 0, nullptr, 0, &n);
+        rods.assign((size_t)n * 8 + 8, 0.0f);
+        nf_get_rods(h, 0, rods.data(), n, &n);
+        have_contacts = false;
+        std::printf("New box: %f - %f - %f, new cnt number: %u\n", p.box[0], p.box[1], p.box[2], n);
+      }
     } else if (!std::strcmp(v, "reduce")) {
       const int dir = ai + 1 < argc ? dir_axis(argv[++ai]) : -1;
       if (dir < 0) {

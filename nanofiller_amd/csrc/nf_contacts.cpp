@@ -22,8 +22,7 @@ namespace nf {
 
 // Reach bounds of the contact rods: the instance's bounds after generation,
 // else the largest half length and radius of the rods (copied once).
-static int contact_bounds(nf_handle* h, uint32_t n, const float4* P, const float4* D, float* hlmax,
-                          float* amax) {
+int contact_bounds(nf_handle* h, uint32_t n, const float4* P, const float4* D, float* hlmax, float* amax) {
   *hlmax = h->desc[0].hl_max;
   *amax = (float)h->params[0].radius;
   if (h->generated) return NF_OK;

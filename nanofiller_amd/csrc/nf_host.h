@@ -28,6 +28,8 @@ struct nf_handle {
   int opt_footprint_occ = 0;   // footprint home cell: rods crossing it at the mean density (0: 64)
   int device = 0;
   std::vector<uint32_t> n_rods;  // current rod count per instance
+  std::vector<double> milestones;  // volume fractions (percent) of the state dumps, ascending
+  std::string milestone_dir;       // where nf_generate writes them
   bool generated = false;
   // device memory: live blocks (pointer -> size-class bytes) and a pool of
   // freed blocks by size class, reused by later calls of the same handle
@@ -58,6 +60,9 @@ void release_contacts(nf_handle* h);
 // stencil): built by the cell engine of nf_contacts, or on demand by the
 // node network.
 int build_contact_cells(nf_handle* h);
+// Largest half length and radius of instance 0's n rods at P, D: the
+// generation's bounds after nf_generate, else scanned (one host copy).
+int contact_bounds(nf_handle* h, uint32_t n, const float4* P, const float4* D, float* hlmax, float* amax);
 // Default cell edge for a typical pair reach and a mean rod density.
 double default_cell_edge(double reach_typ, double density, double factor);
 // Neighbour offsets of a home block of bx cells along x (offsets from its

@@ -295,6 +295,11 @@ void launch_seg_cross(int prec, const SegCrossArgs& a, hipStream_t s);
 // ---- launchers (nf_post.hip) ----
 void launch_cell_count(const float4* P, uint32_t n, const int32_t ncell[3], const double scale[3],
                        uint32_t* count, float* hl, hipStream_t s);
+void launch_expand_flags(const float4* P, uint32_t n, int k, double L, double B, uint32_t* below, uint32_t* above,
+                         hipStream_t s);
+void launch_expand_scatter(const float4* P, const float4* D, uint32_t n, int k, double L, double B,
+                           const uint32_t* below, const uint32_t* rank_below, const uint32_t* above,
+                           const uint32_t* rank_above, uint32_t n_below, float4* oP, float4* oD, hipStream_t s);
 void launch_cell_scatter(const float4* P, const float4* D, uint32_t n, const int32_t ncell[3],
                          const double scale[3], const uint32_t* start, uint32_t* cur, float4* oP,
                          float4* oD, uint32_t* oid, hipStream_t s);

@@ -65,7 +65,7 @@ extern "C" int nf_params_load(const char* path, nf_params* p) {
       p->vol_frac = v / 100.0;
       p->n_rods = 0;
     } else if (key == "VOLUME_PERCENT_LIST") {
-      warn += "VOLUME_PERCENT_LIST milestone dumps are not written; ";
+      // milestone dumps: read by nf_params_milestones
     } else if (key == "VOL_FRAC_PRECISION") {
       double v;
       is >> v;
@@ -241,6 +241,28 @@ extern "C" int nf_params_load(const char* path, nf_params* p) {
 // Density maps: read_density (src/IO.cpp:11-36) - text: physDim[3], ext[3],
 // then the values in C order ([ix][iy][iz], x slowest).
 // ===========================================================================
+extern "C" int nf_params_milestones(const char* path, double* vf_percent, int32_t cap) {
+  if (!path) return fail(nullptr, NF_E_INVALID, "null argument");
+  std::ifstream f(path);
+  if (!f.is_open()) return fail(nullptr, NF_E_IO, std::string("File ") + path + " not found");
+  std::string line;
+  int32_t count = 0;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string key;
+    if (!(is >> key) || upper(key) != "VOLUME_PERCENT_LIST") continue;
+    int32_t n = 0;
+    if (!(is >> n) || n < 0) return fail(nullptr, NF_E_INVALID, std::string(path) + ": VOLUME_PERCENT_LIST: expected a count");
+    for (int32_t i = 0; i < n; ++i) {
+      double v;
+      if (!(is >> v)) return fail(nullptr, NF_E_INVALID, std::string(path) + ": VOLUME_PERCENT_LIST: expected " + std::to_string(n) + " values");
+      if (vf_percent && i < cap) vf_percent[i] = v;
+    }
+    count = n;
+  }
+  return count;
+}
+
 extern "C" int nf_read_density_mesh(const char* path, double dim[3], int32_t ext[3], float* values,
                                     uint64_t cap) {
   if (!path || !dim || !ext) return fail(nullptr, NF_E_INVALID, "null argument");

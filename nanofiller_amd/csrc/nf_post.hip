@@ -561,4 +561,68 @@ void launch_selftest(int which, const double* in, double* out, uint64_t n, hipSt
   hipLaunchKernelGGL(k_selftest, dim3(grid_for(n, 256)), dim3(256), 0, s, which, in, out, n);
 }
 
+
+// ---------------------------------------------------------------------------
+// expand: periodic copies of the rods near the two faces normal to axis k.
+// A rod whose centre lies within B of the far face (x >= L - B) gets a copy
+// below the box, one within B of the near face (x < B) a copy above it;
+// every coordinate along k is then shifted by B (new box [0, L + 2B)).  The
+// shifts are FP64 sums rounded once to FP32.
+// ---------------------------------------------------------------------------
+__global__ void k_expand_flags(const float4* __restrict__ P, uint32_t n, int k, double L, double B,
+                               uint32_t* __restrict__ below, uint32_t* __restrict__ above) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float4 p = P[i];
+    const double x = (double)(k == 0 ? p.x : (k == 1 ? p.y : p.z));
+    below[i] = x >= L - B ? 1u : 0u;
+    above[i] = x < B ? 1u : 0u;
+  }
+}
+
+__device__ __forceinline__ float4 expand_shift(float4 p, int k, double s) {
+  if (k == 0)
+    p.x = (float)((double)p.x + s);
+  else if (k == 1)
+    p.y = (float)((double)p.y + s);
+  else
+    p.z = (float)((double)p.z + s);
+  return p;
+}
+
+__global__ void k_expand_scatter(const float4* __restrict__ P, const float4* __restrict__ D, uint32_t n, int k,
+                                 double L, double B, const uint32_t* __restrict__ below,
+                                 const uint32_t* __restrict__ rank_below, const uint32_t* __restrict__ above,
+                                 const uint32_t* __restrict__ rank_above, uint32_t n_below,
+                                 float4* __restrict__ oP, float4* __restrict__ oD) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float4 p = P[i], d = D[i];
+    oP[i] = expand_shift(p, k, B);
+    oD[i] = d;
+    if (below[i]) {
+      const uint32_t j = n + rank_below[i];
+      oP[j] = expand_shift(p, k, B - L);
+      oD[j] = d;
+    }
+    if (above[i]) {
+      const uint32_t j = n + n_below + rank_above[i];
+      oP[j] = expand_shift(p, k, B + L);
+      oD[j] = d;
+    }
+  }
+}
+
+void launch_expand_flags(const float4* P, uint32_t n, int k, double L, double B, uint32_t* below, uint32_t* above,
+                         hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_expand_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, P, n, k, L, B, below, above);
+}
+
+void launch_expand_scatter(const float4* P, const float4* D, uint32_t n, int k, double L, double B,
+                           const uint32_t* below, const uint32_t* rank_below, const uint32_t* above,
+                           const uint32_t* rank_above, uint32_t n_below, float4* oP, float4* oD, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_expand_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, P, D, n, k, L, B, below,
+                     rank_below, above, rank_above, n_below, oP, oD);
+}
+
 }  // namespace nf

@@ -19,6 +19,7 @@
 // src/simPrms.cpp:691-743, leaves it before save_state_BIN), so ttlCNT =
 // numCNT.  The reader keeps every slot with l > TOL, the rule of
 // simPrms::asmbl_cnt_data (src/simPrms.cpp:329-391), in slot order.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -196,5 +197,97 @@ extern "C" int nf_load_state(nf_handle* h, const char* path, uint32_t* n_rods) {
   rc = nf_set_rods(h, 0, n ? rods.data() : nullptr, n);
   if (rc != NF_OK) return rc;
   if (n_rods) *n_rods = n;
+  return NF_OK;
+}
+
+// ===========================================================================
+// expand (simPrms::expand, src/simPrms.cpp:744-852): the reference copies,
+// for each new boundary cell, the rods of the old cell NeiOrder cells away
+// across the periodic boundary, on the host.  Here the copies are chosen by
+// centre on the device (flags, scans for their id-ordered ranks, one scatter
+// into the new handle's rod buffers).
+// ===========================================================================
+extern "C" int nf_expand(nf_handle* h, int dir, nf_handle** out) {
+  using namespace nf;
+  if (!out) return fail(h, NF_E_INVALID, "null output handle");
+  *out = nullptr;
+  if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  if (dir != 1 && dir != 2 && dir != 4) return fail(h, NF_E_INVALID, "dir must be 1 (x), 2 (y) or 4 (z)");
+  const int k = dir >> 1;
+  const nf_params& p = h->params[0];
+  const nf::InstDesc& d = h->desc[0];
+  const uint32_t n = h->n_rods[0];
+  const float4* P = h->gb.bP + d.rod_off;
+  const float4* D = h->gb.bD + d.rod_off;
+  float hlmax = 0.0f, amax = 0.0f;
+  int rc = contact_bounds(h, n, P, D, &hlmax, &amax);
+  if (rc != NF_OK) return rc;
+  const double L = p.box[k];
+  const double E = L / (double)d.ncell[k];
+  const double reach = 2.0 * (double)hlmax + p.contact_gap;
+  const double B = std::ceil(reach / E) * E;
+  if (!(B <= L)) return fail(h, NF_E_INVALID, "expand: the buffer (rod length + contact gap) is wider than the box");
+  hipStream_t s = h->stream;
+  uint32_t *below = nullptr, *above = nullptr, *rb = nullptr, *ra = nullptr, *tmp = nullptr;
+  // flags per rod; their exclusive scans (ranks) plus the total at [n]
+  if (rc == NF_OK) rc = dalloc(h, &below, std::max<uint64_t>(n, 1));
+  if (rc == NF_OK) rc = dalloc(h, &above, std::max<uint64_t>(n, 1));
+  if (rc == NF_OK) rc = dalloc(h, &rb, (uint64_t)n + 1);
+  if (rc == NF_OK) rc = dalloc(h, &ra, (uint64_t)n + 1);
+  if (rc == NF_OK) rc = dalloc(h, &tmp, scan_tmp_words(n));
+  auto release = [&]() {
+    dfree(h, below);
+    dfree(h, above);
+    dfree(h, rb);
+    dfree(h, ra);
+    dfree(h, tmp);
+  };
+  if (rc != NF_OK) {
+    release();
+    return rc;
+  }
+  launch_expand_flags(P, n, k, L, B, below, above, s);
+  scan_u32(below, rb, n, tmp, s);
+  scan_u32(above, ra, n, tmp, s);
+  uint32_t nb = 0, na = 0;
+  hipError_t e = hipMemcpyAsync(&nb, rb + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&na, ra + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = sync_stream(s, "expand count");
+  if (e != hipSuccess) {
+    release();
+    return fail(h, NF_E_HIP, std::string("expand: ") + hipGetErrorString(e));
+  }
+  const uint64_t total = (uint64_t)n + nb + na;
+  if (total > 0xFFFFFFF0ull) {
+    release();
+    return fail(h, NF_E_INVALID, "expand: too many rods");
+  }
+  nf_params q = p;
+  q.box[k] = L + 2.0 * B;
+  q.n_rods = (uint32_t)total;
+  q.vol_frac = 0.0;
+  nf_handle* g = nullptr;
+  rc = nf_create(&q, &g);
+  if (rc != NF_OK) {
+    release();
+    return fail(h, rc, std::string("expand: ") + g_last_error);
+  }
+  g->opt_contact_prec = h->opt_contact_prec;
+  g->opt_contact_engine = h->opt_contact_engine;
+  g->opt_footprint_fine = h->opt_footprint_fine;
+  g->opt_footprint_occ = h->opt_footprint_occ;
+  launch_expand_scatter(P, D, n, k, L, B, below, rb, above, ra, nb, g->gb.bP + g->desc[0].rod_off,
+                        g->gb.bD + g->desc[0].rod_off, s);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = sync_stream(s, "expand scatter");
+  release();
+  if (e != hipSuccess) {
+    nf_destroy(g);
+    return fail(h, NF_E_HIP, std::string("expand: ") + hipGetErrorString(e));
+  }
+  g->n_rods[0] = (uint32_t)total;
+  g->generated = false;
+  trace("expand: axis %d, buffer %.6f um, %u rods + %u below + %u above", k, B, n, nb, na);
+  *out = g;
   return NF_OK;
 }

@@ -74,6 +74,8 @@ def lib() -> C.CDLL:
                                          C.c_int64, P(HistSpec), C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p]),
             ("orc_atan", C.c_double, [C.c_double]),
+            ("orc_expand", C.c_int64, [C.c_void_p, C.c_uint32, C.c_int, C.c_double, C.c_double,
+                                       C.c_void_p, C.c_int64]),
             ("orc_reduce", C.c_uint32, [P(Params), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_int64, C.c_double, C.c_int32, C.c_void_p,
                                         P(C.c_uint32)]),
@@ -242,6 +244,16 @@ def network(p, rods, I, J, gap64, image):
                       im.ctypes.data, len(I), labels.ctypes.data, row.ctypes.data, col.ctypes.data,
                       val.ctypes.data, nnz, C.byref(info))
     return info.to_dict(), {"labels": labels, "row_ptr": row, "col": col, "val": val}
+
+
+def expand(rods, axis, L, B):
+    """Periodic copies of the rods within B of the two faces normal to
+    `axis`, every coordinate along it shifted by B (orc_expand)."""
+    r = np.ascontiguousarray(rods, np.float32)
+    n = lib().orc_expand(r.ctypes.data, r.shape[0], int(axis), float(L), float(B), None, 0)
+    out = np.zeros((max(n, 1), 8), np.float32)
+    lib().orc_expand(r.ctypes.data, r.shape[0], int(axis), float(L), float(B), out.ctypes.data, n)
+    return out[:n]
 
 
 def reduce(p, rods, I, J, image, gcut, min_cont=2):

@@ -1609,3 +1609,36 @@ void orc_node_points(const nf_params* p, const float* rods, uint32_t n, const ui
     flag[q] = scr ? 1u : 0u;
   }
 }
+
+/* ===================================================================== */
+/* expand                                                                  */
+/* ===================================================================== */
+int64_t orc_expand(const float* rods, uint32_t n, int k, double L, double B, float* out, int64_t cap) {
+  int64_t nb = 0, na = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const double x = (double)rods[8 * (size_t)i + k];
+    if (x >= L - B) ++nb;
+    if (x < B) ++na;
+  }
+  const int64_t total = (int64_t)n + nb + na;
+  if (!out || cap < total) return total;
+  int64_t ib = n, ia = n + nb;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* r = rods + 8 * (size_t)i;
+    const double x = (double)r[k];
+    float* o = out + 8 * (size_t)i;
+    memcpy(o, r, 8 * sizeof(float));
+    o[k] = (float)(x + B);
+    if (x >= L - B) {
+      o = out + 8 * (size_t)ib++;
+      memcpy(o, r, 8 * sizeof(float));
+      o[k] = (float)(x + (B - L));
+    }
+    if (x < B) {
+      o = out + 8 * (size_t)ia++;
+      memcpy(o, r, 8 * sizeof(float));
+      o[k] = (float)(x + (B + L));
+    }
+  }
+  return total;
+}

@@ -103,6 +103,14 @@ int64_t orc_network(const nf_params* p, const float* rods, uint32_t n,
 
 /* Dead-end pruning (reduce): keep[i] = 1 for the kept rods; returns their
  * number, *rounds the iterations including the last, empty one. */
+/* expand (simPrms::expand, src/simPrms.cpp:744-852, its copies chosen by
+ * centre): periodic copies along axis k of the rods whose centre x_k lies
+ * within B of the far face (x_k >= L - B, copied below the box) and of the
+ * near face (x_k < B, copied above it); every coordinate along k shifted by
+ * B, each as one FP64 sum rounded to FP32 (x + B, x + (B - L), x + (B + L)).
+ * out: the n rods in id order, then the copies below, then those above,
+ * each in id order.  Returns the rod count; writes only when cap suffices. */
+int64_t orc_expand(const float* rods, uint32_t n, int k, double L, double B, float* out, int64_t cap);
 uint32_t orc_reduce(const nf_params* p, const float* rods, uint32_t n, const uint32_t* I,
                     const uint32_t* J, const int32_t* image, int64_t nc, double gcut,
                     int32_t min_cont, uint8_t* keep, uint32_t* rounds);

@@ -90,3 +90,34 @@ def test_cli_reduce(tmp_path):
     # the state file stores rods by cell: the same set
     key = lambda a: np.lexsort(a.view(np.uint32).T[::-1])
     np.testing.assert_array_equal(back[key(back)].view(np.uint32), pruned[key(pruned)].view(np.uint32))
+
+
+def test_cli_milestones_are_read(tmp_path):
+    """VOLUME_PERCENT_LIST is parsed (count, then percent values)."""
+    (tmp_path / "params.txt").write_text(PARAMS + "VOLUME_PERCENT_LIST 3 0.01 0.02 0.03\n")
+    assert nf.params_milestones(str(tmp_path / "params.txt")) == [0.01, 0.02, 0.03]
+    (tmp_path / "none.txt").write_text(PARAMS)
+    assert nf.params_milestones(str(tmp_path / "none.txt")) == []
+    (tmp_path / "bad.txt").write_text(PARAMS + "VOLUME_PERCENT_LIST 3 0.01\n")
+    with pytest.raises(nf.NFError):
+        nf.params_milestones(str(tmp_path / "bad.txt"))
+
+
+@pytest.mark.gpu
+def test_cli_milestones_and_expand(tmp_path):
+    """generate with VOLUME_PERCENT_LIST writes state_<vf>.bin dumps; expand 1
+    hands the expanded packing to the later verbs (the same as nf_expand)."""
+    (tmp_path / "params.txt").write_text(PARAMS + "VOLUME_PERCENT_LIST 2 0.02 0.04\n")
+    r = subprocess.run([CLI, "params.txt", "generate", "expand", "1", "stat_contacts"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    dumps = [f for f in os.listdir(tmp_path) if f.startswith("state_")]
+    assert dumps
+    p = nf.normalize(nf.load_params(str(tmp_path / "params.txt")))
+    with nf.Packing(p) as pk:
+        pk.generate()
+        with pk.expand(1) as ex:
+            q = ex.params()
+            n = len(ex.rods())
+    assert f"new cnt number: {n}" in r.stdout
+    assert "New box: %f - %f - %f" % (q.box[0], q.box[1], q.box[2]) in r.stdout
