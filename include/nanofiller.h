@@ -176,7 +176,8 @@ int nf_set_rods(nf_handle* h, int inst, const float* rods, uint32_t n);
  *      (src/get_contact_data.cu:71-128) ----------------------------------- */
 
 /* Finds every pair (I < J, periodic image) whose surface gap is <= gap on
- * instance 0.  Result stays on the device; *n_contacts receives its size. */
+ * the post-processing instance (NF_OPT_POST_INSTANCE, default 0).  Result
+ * stays on the device; *n_contacts receives its size. */
 int nf_contacts(nf_handle* h, double gap, uint64_t* n_contacts);
 
 /* Copies the contact list, sorted by (I, J, image): row_ptr has N+1 entries
@@ -230,7 +231,7 @@ int nf_histograms_noalloc(nf_handle* h, const nf_hist_spec* spec, int64_t* out_c
 
 /* Centre-centre distance statistics, statKernel_CENT (inc/statKernel_CENT.h:
  * 8-122) driven by collectStats (src/statistics_collection.cu:866-1033), on
- * instance 0: every unordered pair of distinct rods over all periodic images
+ * the post-processing instance: every unordered pair of distinct rods over all periodic images
  * whose centre distance d is below bins * width (at most 8 box edges).
  * counts[k] (int64, exact) holds d in [k width, (k+1) width); *n_pairs the
  * pair count; *sum_d_q24 and *sum_d2_q16 the exact sums of round(d * 2^24)
@@ -283,7 +284,7 @@ int nf_get_network(nf_handle* h, uint32_t* labels, uint32_t* row_ptr,
  * those crossing the electrode axis (params.elec_axis); a coupling to an
  * electrode plane (the rule of nf_network) counts as one contact - the
  * reference instead keeps every rod of its electrode cell layers.  The kept
- * rods replace instance 0's packing in id order (new ids 0..n_after-1); the
+ * rods replace the post-processing instance's (NF_OPT_POST_INSTANCE, default 0) packing in id order (new ids 0..n_after-1); the
  * contact list and everything built on it are released, so nf_contacts
  * must run again for the pruned packing. */
 typedef struct nf_reduce_info {
@@ -360,7 +361,7 @@ int nf_get_node_electrodes(nf_handle* h, uint32_t* node, double* gap, int32_t* s
 
 /* ---- output writers ---------------------------------------------------- */
 
-/* Legacy-ASCII VTK of instance 0 as write_CNTvtk (src/IO.cpp:38-76). */
+/* Legacy-ASCII VTK of the post-processing instance as write_CNTvtk (src/IO.cpp:38-76). */
 int nf_write_vtk(nf_handle* h, const char* path);
 /* Contact lists as save_coo (src/conductivity.cu:838-879): <prefix>internal.srt.coo,
  * <prefix>sink.elec.coo, <prefix>source.elec.coo. */
@@ -401,14 +402,14 @@ typedef struct nf_state_header {
   float tol;        /* TOL: slots with l <= tol are empty           */
 } nf_state_header;
 
-/* Writes instance 0 as save_state_BIN after simPrms::repack
+/* Writes the post-processing instance as save_state_BIN after simPrms::repack
  * (src/stateSR.cpp:54-88, src/simPrms.cpp:691-743): the generation cell
  * grid, per-cell occupancy and slot prefix, and the nine field-major
  * arrays x y z cx cy cz l a k (k = cell index), one slot per rod. */
 int nf_save_state(nf_handle* h, const char* path);
 /* Reads the header of a state file and checks the file size against it. */
 int nf_read_state_header(const char* path, nf_state_header* out);
-/* Replaces instance 0's rods with the state file's (read_state_BIN,
+/* Replaces the post-processing instance's (NF_OPT_POST_INSTANCE, default 0) rods with the state file's (read_state_BIN,
  * src/stateSR.cpp:90-134): every slot with l > TOL, in slot order, as
  * simPrms::asmbl_cnt_data (src/simPrms.cpp:329-391).  The file's box must
  * equal the handle's; the rod count must not exceed the handle's target. */
@@ -431,7 +432,7 @@ int nf_params_milestones(const char* path, double* vf_percent, int32_t cap);
 
 /* expand (simPrms::expand, src/simPrms.cpp:744-852; verb at
  * src/NanoFiller.cpp:67-69): a new single-instance handle *out holding
- * instance 0's packing with its box grown along axis k = dir >> 1 (dir the
+ * the post-processing instance's (NF_OPT_POST_INSTANCE, default 0) packing with its box grown along axis k = dir >> 1 (dir the
  * reference's bit mask: 1 = x, 2 = y, 4 = z) by a buffer of periodic copies
  * on each side.  The buffer is B = ceil((l_max + contact_gap) / E) E, E the
  * generation cell edge along k (the reference's NeiOrder cells of the
@@ -459,8 +460,12 @@ enum {
                                 footprint)                                                */
   NF_OPT_FOOTPRINT_FINE = 6, /* footprint engine: fine cell edge in 1/1000 of the typical
                                 rod length (0: default 200)                               */
-  NF_OPT_FOOTPRINT_OCC = 7   /* footprint engine: rods crossing a home cell at the mean
+  NF_OPT_FOOTPRINT_OCC = 7,  /* footprint engine: rods crossing a home cell at the mean
                                 density, which sizes the home cells (0: default 64)       */
+  NF_OPT_POST_INSTANCE = 8   /* the instance of a batch that contacts, statistics,
+                                network, reduce, conductance, the writers, the state
+                                files and expand act on (default 0); changing it drops
+                                the results computed for the previous one              */
 };
 /* Candidate pairs of nf_contacts.  Both give the same contact list.
  * FOOTPRINT: each rod binned into the fine cells its capsule (radius

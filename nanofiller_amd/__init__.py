@@ -31,7 +31,7 @@ NF_PREC_FP64, NF_PREC_FP32_RECHECK = 0, 1
 NF_STATUS_RUNNING, NF_STATUS_TARGET, NF_STATUS_MAX_ROUNDS, NF_STATUS_STALLED, NF_STATUS_OVERFLOW = 0, 1, 2, 3, 4
 STATUS = {0: "running", 1: "target", 2: "max_rounds", 3: "stalled", 4: "overflow"}
 NF_OPT_GRAPH, NF_OPT_PROFILE, NF_OPT_DIAG, NF_OPT_CONTACT_PREC = 1, 2, 3, 4
-NF_OPT_CONTACT_ENGINE, NF_OPT_FOOTPRINT_FINE, NF_OPT_FOOTPRINT_OCC = 5, 6, 7
+NF_OPT_CONTACT_ENGINE, NF_OPT_FOOTPRINT_FINE, NF_OPT_FOOTPRINT_OCC, NF_OPT_POST_INSTANCE = 5, 6, 7, 8
 NF_ENGINE_FOOTPRINT, NF_ENGINE_CELLS = 0, 1
 KERNEL_CLASSES = ["gen", "bin", "reject_old", "reject_new", "commit", "control", "contact", "other"]
 
@@ -419,6 +419,7 @@ class Packing:
         if _handle is not None:
             self._h = _handle  # a handle the library made (nf_expand)
             self.n_inst = 1
+            self.post_inst = 0
             return
         plist = [params] if isinstance(params, Params) else list(params)
         arr = (Params * len(plist))(*plist)
@@ -426,6 +427,7 @@ class Packing:
         _check(lib().nf_create_batch(arr, len(plist), C.byref(h)))
         self._h = h
         self.n_inst = len(plist)
+        self.post_inst = 0
         for k, v in (options or {}).items():
             self.set_option(k, v)
 
@@ -467,6 +469,13 @@ class Packing:
 
     def set_option(self, key: int, value: int) -> None:
         _check(lib().nf_set_option(self._h, int(key), int(value)), self._h)
+        if key == NF_OPT_POST_INSTANCE:
+            self.post_inst = int(value)
+
+    def select(self, inst: int) -> "Packing":
+        """Post-process instance `inst` of a batch (NF_OPT_POST_INSTANCE)."""
+        self.set_option(NF_OPT_POST_INSTANCE, inst)
+        return self
 
     def generate(self) -> list[GenStats]:
         st = (GenStats * self.n_inst)()
@@ -489,7 +498,7 @@ class Packing:
         nc = C.c_uint64()
         _check(lib().nf_contacts(self._h, float(gap), C.byref(nc)), self._h)
         nrod = C.c_uint32()
-        _check(lib().nf_get_rods(self._h, 0, None, 0, C.byref(nrod)), self._h)
+        _check(lib().nf_get_rods(self._h, self.post_inst, None, 0, C.byref(nrod)), self._h)
         row = np.zeros(nrod.value + 1, dtype=np.uint32)
         col = np.zeros(nc.value, dtype=np.uint32)
         g = np.zeros(nc.value, dtype=np.float32)
@@ -632,7 +641,7 @@ class Packing:
         (info dict, rod potentials or None)."""
         info = ConductanceInfo()
         n = C.c_uint32()
-        _check(lib().nf_get_rods(self._h, 0, None, 0, C.byref(n)), self._h)
+        _check(lib().nf_get_rods(self._h, self.post_inst, None, 0, C.byref(n)), self._h)
         v = np.zeros(max(n.value, 1), np.float64) if potentials else None
         _check(lib().nf_conductance(self._h, tol, max_iter, C.byref(info),
                                     v.ctypes.data if v is not None else None), self._h)
@@ -649,7 +658,7 @@ class Packing:
         _check(lib().nf_save_state(self._h, path.encode()), self._h)
 
     def load_state(self, path: str) -> int:
-        """Replaces instance 0's rods with a state file's; returns the count."""
+        """Replaces the post-processing instance's rods with a state file's; returns the count."""
         n = C.c_uint32()
         _check(lib().nf_load_state(self._h, path.encode(), C.byref(n)), self._h)
         return n.value

@@ -643,6 +643,13 @@ extern "C" int nf_set_option(nf_handle* h, int key, int64_t value) {
       if (value < 0 || value > 1000000) return fail(h, NF_E_INVALID, "footprint fine cell factor out of range");
       h->opt_footprint_fine = (int)value;
       break;
+    case NF_OPT_POST_INSTANCE:
+      if (value < 0 || value >= (int64_t)h->params.size()) return fail(h, NF_E_INVALID, "no such instance");
+      if ((int)value != h->post_inst) {
+        release_contacts(h);  // contacts, network and cell lists belong to the previous instance
+        h->post_inst = (int)value;
+      }
+      break;
     case NF_OPT_FOOTPRINT_OCC:
       if (value < 0 || value > 200) return fail(h, NF_E_INVALID, "footprint occupancy out of range (0..200)");
       h->opt_footprint_occ = (int)value;

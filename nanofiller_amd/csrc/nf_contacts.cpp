@@ -23,8 +23,8 @@ namespace nf {
 // Reach bounds of the contact rods: the instance's bounds after generation,
 // else the largest half length and radius of the rods (copied once).
 int contact_bounds(nf_handle* h, uint32_t n, const float4* P, const float4* D, float* hlmax, float* amax) {
-  *hlmax = h->desc[0].hl_max;
-  *amax = (float)h->params[0].radius;
+  *hlmax = h->desc[h->post_inst].hl_max;
+  *amax = (float)h->params[h->post_inst].radius;
   if (h->generated) return NF_OK;
   std::vector<float4> hP(n), hD(n);
   if (n) {
@@ -50,10 +50,10 @@ static double contact_reach(const ContactBuffers& c) {
 // (half, bx = 1: each cell pair once).
 static int build_cell_list(nf_handle* h, double edge, double reach, bool half, int bx = 1) {
   ContactBuffers& c = h->cb;
-  const nf_params& p = h->params[0];
+  const nf_params& p = h->params[h->post_inst];
   const uint32_t n = c.n_rods;
-  const float4* P = h->gb.bP + h->desc[0].rod_off;
-  const float4* D = h->gb.bD + h->desc[0].rod_off;
+  const float4* P = h->gb.bP + h->desc[h->post_inst].rod_off;
+  const float4* D = h->gb.bD + h->desc[h->post_inst].rod_off;
   hipStream_t s = h->stream;
   uint64_t ncells = 1;
   double scale[3];
@@ -105,7 +105,7 @@ static int build_cell_list(nf_handle* h, double edge, double reach, bool half, i
 int build_contact_cells(nf_handle* h) {
   ContactBuffers& c = h->cb;
   if (c.cP) return NF_OK;
-  const nf_params& p = h->params[0];
+  const nf_params& p = h->params[h->post_inst];
   const double Rmax = ((double)c.a_max + (double)c.a_max) + c.gap_cut;
   const double reach = contact_reach(c);
   // cell edge about one typical pair reach; the sorted stencil reaches the
@@ -126,7 +126,7 @@ int build_contact_cells(nf_handle* h) {
 // agglomerates: occupancy 32-64 and fine fraction 0.2 are within 20% of
 // the best for each; from 96 on, units are redone cell by cell too often.
 static void footprint_cells(nf_handle* h, uint32_t n, double* edge, int* fine) {
-  const nf_params& p = h->params[0];
+  const nf_params& p = h->params[h->post_inst];
   const ContactBuffers& c = h->cb;
   const double typ = p.len_law == NF_LEN_LOGNORMAL ? p.len : (double)c.hl_max * 2.0;
   const double V = p.box[0] * p.box[1] * p.box[2];
@@ -150,9 +150,9 @@ static void footprint_cells(nf_handle* h, uint32_t n, double* edge, int* fine) {
 // when the footprint engine found a home cell with too many survivors.
 static int contacts_pass(nf_handle* h, double gap, uint64_t* n_out, int prec, bool cells) {
   ContactBuffers& c = h->cb;
-  const nf_params& p = h->params[0];
-  const InstDesc& d = h->desc[0];
-  const uint32_t n = h->n_rods[0];
+  const nf_params& p = h->params[h->post_inst];
+  const InstDesc& d = h->desc[h->post_inst];
+  const uint32_t n = h->n_rods[h->post_inst];
   const float4* P = h->gb.bP + d.rod_off;
   const float4* D = h->gb.bD + d.rod_off;
   hipStream_t s = h->stream;
@@ -421,8 +421,7 @@ extern "C" int nf_contacts(nf_handle* h, double gap, uint64_t* n_contacts) {
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   h->err.clear();
   if (!(gap >= 0.0)) return fail(h, NF_E_INVALID, "gap must be >= 0");
-  if (h->params.size() != 1) return fail(h, NF_E_INVALID, "contacts run on single-instance handles");
-  const int prec = h->opt_contact_prec >= 0 ? h->opt_contact_prec : h->params[0].precision;
+  const int prec = h->opt_contact_prec >= 0 ? h->opt_contact_prec : h->params[h->post_inst].precision;
   return contacts_run(h, gap, n_contacts, prec, nullptr);
 }
 
@@ -544,12 +543,12 @@ int hist_run(nf_handle* h, const HistPlan& pl, const double* edges, double* dE,
     fill_u32((uint32_t*)dOut, 0u, 2 * pl.n_out, s);
     e = hipGetLastError();
   }
-  const InstDesc& d = h->desc[0];
-  const uint32_t n = h->n_rods[0];
+  const InstDesc& d = h->desc[h->post_inst];
+  const uint32_t n = h->n_rods[h->post_inst];
   if (e == hipSuccess && n) {
     launch_hist(h->gb.bP + d.rod_off, h->gb.bD + d.rod_off, n, pl.mutual ? h->cb.rowptr : nullptr,
                 h->cb.J, dE, dE + (nb + 1), pl.npb ? dE + 2 * (nb + 1) : nullptr, nb, pl.npb, pl.ntb,
-                pl.axis, h->params[0].box[pl.axis], pl.mu, dOut, s);
+                pl.axis, h->params[h->post_inst].box[pl.axis], pl.mu, dOut, s);
     e = hipGetLastError();
   }
   // int64 and u64 share the layout; counts never reach 2^63
@@ -596,15 +595,14 @@ extern "C" int nf_centre_distance(nf_handle* h, int32_t bins, double width, int6
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   if (bins < 1 || bins > CENT_BINS_MAX || !(width > 0.0))
     return fail(h, NF_E_INVALID, "need 1 <= bins <= 1024 and width > 0");
-  if (h->params.size() != 1) return fail(h, NF_E_INVALID, "statistics run on single-instance handles");
-  const nf_params& p = h->params[0];
-  const uint32_t n = h->n_rods[0];
+  const nf_params& p = h->params[h->post_inst];
+  const uint32_t n = h->n_rods[h->post_inst];
   const double rmax = bins * width;
   const double bmin = std::min(p.box[0], std::min(p.box[1], p.box[2]));
   if (rmax > 8.0 * bmin) return fail(h, NF_E_INVALID, "bins * width exceeds 8 box edges");
   hipStream_t s = h->stream;
-  const float4* P = h->gb.bP + h->desc[0].rod_off;
-  const float4* D = h->gb.bD + h->desc[0].rod_off;
+  const float4* P = h->gb.bP + h->desc[h->post_inst].rod_off;
+  const float4* D = h->gb.bD + h->desc[h->post_inst].rod_off;
   // grid: edge about rmax (half stencil of +-1 cells), finer when crowded
   const double edge = default_cell_edge(rmax, (double)n / (p.box[0] * p.box[1] * p.box[2]), 1.0);
   int32_t nc[3];
@@ -692,8 +690,8 @@ extern "C" int nf_vdw(nf_handle* h, double A, double B, double* energy, uint64_t
   if (!(c.gap_cut > 0.0)) return fail(h, NF_E_INVALID, "the contact gap must be > 0");
   hipStream_t s = h->stream;
   const uint32_t n = c.n_rods;
-  const float4* P = h->gb.bP + h->desc[0].rod_off;
-  const float4* D = h->gb.bD + h->desc[0].rod_off;
+  const float4* P = h->gb.bP + h->desc[h->post_inst].rod_off;
+  const float4* D = h->gb.bD + h->desc[h->post_inst].rod_off;
   double *rowsum = nullptr, *bsum = nullptr, *tot = nullptr;
   unsigned long long* pairs = nullptr;
   int rc = NF_OK;
@@ -713,7 +711,7 @@ extern "C" int nf_vdw(nf_handle* h, double A, double B, double* energy, uint64_t
     return rc;
   }
   fill_u32((uint32_t*)pairs, 0u, 2, s);
-  launch_vdw(P, D, n, c.rowptr, c.J, c.img, h->params[0].box, c.gap_cut, A, B, rowsum, bsum, tot,
+  launch_vdw(P, D, n, c.rowptr, c.J, c.img, h->params[h->post_inst].box, c.gap_cut, A, B, rowsum, bsum, tot,
              pairs, s);
   double e_h = 0.0;
   unsigned long long p_h = 0;

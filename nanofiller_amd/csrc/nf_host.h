@@ -26,6 +26,7 @@ struct nf_handle {
   int opt_contact_engine = NF_ENGINE_FOOTPRINT;
   int opt_footprint_fine = 0;  // footprint fine cell edge in 1/1000 of the typical rod length (0: 200)
   int opt_footprint_occ = 0;   // footprint home cell: rods crossing it at the mean density (0: 64)
+  int post_inst = 0;  // instance that contacts, statistics, network, reduce, writers, state files and expand use
   int device = 0;
   std::vector<uint32_t> n_rods;  // current rod count per instance
   std::vector<double> milestones;  // volume fractions (percent) of the state dumps, ascending

@@ -308,10 +308,10 @@ extern "C" int nf_write_density_mesh(const char* path, const double dim[3], cons
 // ===========================================================================
 extern "C" int nf_write_vtk(nf_handle* h, const char* path) {
   if (!h || !path) return fail(h, NF_E_INVALID, "null argument");
-  const uint32_t n = h->n_rods[0];
+  const uint32_t n = h->n_rods[h->post_inst];
   std::vector<float> rods((size_t)n * 8);
   uint32_t got = 0;
-  int rc = nf_get_rods(h, 0, rods.data(), n, &got);
+  int rc = nf_get_rods(h, h->post_inst, rods.data(), n, &got);
   if (rc != NF_OK) return rc;
   std::ofstream f(path, std::ios::out | std::ios::trunc);
   if (!f.is_open()) return fail(h, NF_E_IO, std::string("cannot write ") + path);
@@ -382,12 +382,12 @@ extern "C" int nf_write_coo(nf_handle* h, const char* prefix) {
   if (rc != NF_OK) return rc;
   std::vector<float> rods((size_t)n * 8);
   uint32_t got = 0;
-  rc = nf_get_rods(h, 0, rods.data(), n, &got);
+  rc = nf_get_rods(h, h->post_inst, rods.data(), n, &got);
   if (rc != NF_OK) return rc;
   std::vector<double> area(C);
   rc = nf_get_contact_area(h, area.data(), C);
   if (rc != NF_OK) return rc;
-  const int ax = h->params[0].elec_axis;
+  const int ax = h->params[h->post_inst].elec_axis;
   const double rlim = c.gap_cut;
   {
     std::ofstream f(pre + "internal.srt.coo");
@@ -401,7 +401,7 @@ extern "C" int nf_write_coo(nf_handle* h, const char* prefix) {
       }
   }
   // electrode couplings: planes at 0 and box[ax]
-  const double L = h->params[0].box[ax];
+  const double L = h->params[h->post_inst].box[ax];
   std::ofstream fk(pre + "sink.elec.coo"), fs(pre + "source.elec.coo");
   if (!fk.is_open() || !fs.is_open()) return fail(h, NF_E_IO, "cannot write electrode COO files");
   fk << "#\tI\tJ\tD\tS\n";
@@ -462,7 +462,7 @@ extern "C" int nf_write_node_coo(nf_handle* h, const char* prefix) {
   if (rc != NF_OK) return rc;
   std::vector<float> rods((size_t)n * 8);
   uint32_t got = 0;
-  rc = nf_get_rods(h, 0, rods.data(), n, &got);
+  rc = nf_get_rods(h, h->post_inst, rods.data(), n, &got);
   if (rc != NF_OK) return rc;
   const double rlim = c.gap_cut;
   std::ofstream fn(pre + "Nodes.tbl.dat"), fsg(pre + "segment.LA.coo"), fi(pre + "internal.DS.coo"),

@@ -19,7 +19,7 @@ extern "C" int nf_network(nf_handle* h, nf_network_info* info) {
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   ContactBuffers& c = h->cb;
   if (!c.valid) return fail(h, NF_E_STATE, "no contacts computed");
-  const nf_params& p = h->params[0];
+  const nf_params& p = h->params[h->post_inst];
   const int ax = p.elec_axis;
   if (ax < 0 || ax > 2) return fail(h, NF_E_INVALID, "elec_axis must be 0..2");
   const uint32_t n = c.n_rods;
@@ -40,8 +40,8 @@ extern "C" int nf_network(nf_handle* h, nf_network_info* info) {
   a.J = c.J;
   a.img = c.img;
   a.gap64 = c.gap64;
-  a.P = h->gb.bP + h->desc[0].rod_off;
-  a.D = h->gb.bD + h->desc[0].rod_off;
+  a.P = h->gb.bP + h->desc[h->post_inst].rod_off;
+  a.D = h->gb.bD + h->desc[h->post_inst].rod_off;
   a.n = n;
   a.axis = ax;
   a.L = p.box[ax];

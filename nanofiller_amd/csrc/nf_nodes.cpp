@@ -34,7 +34,7 @@ extern "C" int nf_node_network(nf_handle* h, nf_node_info* info) {
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   ContactBuffers& c = h->cb;
   if (!c.valid) return fail(h, NF_E_STATE, "no contacts computed");
-  const nf_params& p = h->params[0];
+  const nf_params& p = h->params[h->post_inst];
   if (p.elec_axis < 0 || p.elec_axis > 2) return fail(h, NF_E_INVALID, "elec_axis must be 0..2");
   free_nodes(h);
   // the screening walk runs over the centre cell list of the contact rods
@@ -48,8 +48,8 @@ extern "C" int nf_node_network(nf_handle* h, nf_node_info* info) {
   a.rowptr = c.rowptr;
   a.J = c.J;
   a.img = c.img;
-  a.P = h->gb.bP + h->desc[0].rod_off;
-  a.D = h->gb.bD + h->desc[0].rod_off;
+  a.P = h->gb.bP + h->desc[h->post_inst].rod_off;
+  a.D = h->gb.bD + h->desc[h->post_inst].rod_off;
   a.axis = p.elec_axis;
   for (int d = 0; d < 3; ++d) {
     a.box[d] = p.box[d];

@@ -159,13 +159,12 @@ __global__ void k_red_scatter(const uint32_t* __restrict__ state, const uint32_t
 extern "C" int nf_reduce(nf_handle* h, int32_t min_cont, nf_reduce_info* info) {
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   h->err.clear();
-  if (h->params.size() != 1) return fail(h, NF_E_INVALID, "reduce runs on single-instance handles");
   if (min_cont < 0) return fail(h, NF_E_INVALID, "min_cont must be >= 0");
   ContactBuffers& c = h->cb;
   if (!c.valid) return fail(h, NF_E_STATE, "nf_reduce needs nf_contacts first");
-  const nf_params& p = h->params[0];
+  const nf_params& p = h->params[h->post_inst];
   const uint32_t n = c.n_rods;
-  const uint32_t off = h->desc[0].rod_off;
+  const uint32_t off = h->desc[h->post_inst].rod_off;
   float4* P = h->gb.bP + off;
   float4* D = h->gb.bD + off;
   hipStream_t s = h->stream;
@@ -257,7 +256,7 @@ extern "C" int nf_reduce(nf_handle* h, int32_t min_cont, nf_reduce_info* info) {
   // the kept rods are the packing now (a subset of the old one: the length
   // and radius bounds of the handle still hold); every contact-derived
   // result refers to the old ids
-  h->n_rods[0] = kept;
+  h->n_rods[h->post_inst] = kept;
   release_contacts(h);
   h->reduce_ids.swap(ids);
   if (info) {

@@ -98,12 +98,12 @@ int read_file(nf_handle* h, const char* path, std::vector<char>& buf, RawHeader&
 
 extern "C" int nf_save_state(nf_handle* h, const char* path) {
   if (!h || !path) return fail(h, NF_E_INVALID, "null argument");
-  const nf_params& p = h->params[0];
-  const nf::InstDesc& d = h->desc[0];
-  const uint32_t n = h->n_rods[0];
+  const nf_params& p = h->params[h->post_inst];
+  const nf::InstDesc& d = h->desc[h->post_inst];
+  const uint32_t n = h->n_rods[h->post_inst];
   std::vector<float> rods((size_t)n * 8);
   uint32_t got = 0;
-  int rc = nf_get_rods(h, 0, rods.data(), n, &got);
+  int rc = nf_get_rods(h, h->post_inst, rods.data(), n, &got);
   if (rc != NF_OK) return rc;
   const int32_t ext[3] = {d.ncell[0], d.ncell[1], d.ncell[2]};
   const uint64_t G = (uint64_t)ext[0] * ext[1] * ext[2];
@@ -178,7 +178,7 @@ extern "C" int nf_load_state(nf_handle* h, const char* path, uint32_t* n_rods) {
   uint64_t G = 0;
   int rc = read_file(h, path, buf, hd, G);
   if (rc != NF_OK) return rc;
-  const nf_params& p = h->params[0];
+  const nf_params& p = h->params[h->post_inst];
   for (int d = 0; d < 3; ++d)
     if (hd.box[d] != (float)p.box[d])
       return fail(h, NF_E_INVALID, std::string(path) + ": box " + std::to_string(hd.box[d]) + " on axis " +
@@ -194,7 +194,7 @@ extern "C" int nf_load_state(nf_handle* h, const char* path, uint32_t* n_rods) {
     for (int fi = 0; fi < 8; ++fi) rods.push_back(fieldp(fi, i));
   }
   const uint32_t n = (uint32_t)(rods.size() / 8);
-  rc = nf_set_rods(h, 0, n ? rods.data() : nullptr, n);
+  rc = nf_set_rods(h, h->post_inst, n ? rods.data() : nullptr, n);
   if (rc != NF_OK) return rc;
   if (n_rods) *n_rods = n;
   return NF_OK;
@@ -214,9 +214,9 @@ extern "C" int nf_expand(nf_handle* h, int dir, nf_handle** out) {
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   if (dir != 1 && dir != 2 && dir != 4) return fail(h, NF_E_INVALID, "dir must be 1 (x), 2 (y) or 4 (z)");
   const int k = dir >> 1;
-  const nf_params& p = h->params[0];
-  const nf::InstDesc& d = h->desc[0];
-  const uint32_t n = h->n_rods[0];
+  const nf_params& p = h->params[h->post_inst];
+  const nf::InstDesc& d = h->desc[h->post_inst];
+  const uint32_t n = h->n_rods[h->post_inst];
   const float4* P = h->gb.bP + d.rod_off;
   const float4* D = h->gb.bD + d.rod_off;
   float hlmax = 0.0f, amax = 0.0f;
