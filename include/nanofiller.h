@@ -107,6 +107,10 @@ typedef struct nf_gen_stats {
                              (probes and neighbour tiles; roofline figure) */
   uint64_t tile_bytes_new;/* same for the reject-vs-same-round kernel      */
   uint64_t rounds_launched; /* rounds launched (incl. no-op tail rounds)   */
+  uint32_t slot_growths;  /* times the cell slots of the accepted rods were
+                             doubled and the packing generated again (a
+                             cell held more rods than planned)              */
+  uint32_t reserved;
 } nf_gen_stats;
 
 /* Kernel classes reported in nf_gen_stats.kern_ms. */
@@ -462,10 +466,14 @@ enum {
                                 rod length (0: default 200)                               */
   NF_OPT_FOOTPRINT_OCC = 7,  /* footprint engine: rods crossing a home cell at the mean
                                 density, which sizes the home cells (0: default 64)       */
-  NF_OPT_POST_INSTANCE = 8   /* the instance of a batch that contacts, statistics,
+  NF_OPT_POST_INSTANCE = 8,  /* the instance of a batch that contacts, statistics,
                                 network, reduce, conductance, the writers, the state
                                 files and expand act on (default 0); changing it drops
                                 the results computed for the previous one              */
+  NF_OPT_SLOT_SCALE = 9      /* per-cell slots of the accepted rods in 1/1000 of the
+                                planned capacity (default 1000); a generation whose
+                                slots overflow is repeated with twice as many, with
+                                the same packing (nf_gen_stats.slot_growths)           */
 };
 /* Candidate pairs of nf_contacts.  Both give the same contact list.
  * FOOTPRINT: each rod binned into the fine cells its capsule (radius

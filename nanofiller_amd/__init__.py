@@ -32,6 +32,7 @@ NF_STATUS_RUNNING, NF_STATUS_TARGET, NF_STATUS_MAX_ROUNDS, NF_STATUS_STALLED, NF
 STATUS = {0: "running", 1: "target", 2: "max_rounds", 3: "stalled", 4: "overflow"}
 NF_OPT_GRAPH, NF_OPT_PROFILE, NF_OPT_DIAG, NF_OPT_CONTACT_PREC = 1, 2, 3, 4
 NF_OPT_CONTACT_ENGINE, NF_OPT_FOOTPRINT_FINE, NF_OPT_FOOTPRINT_OCC, NF_OPT_POST_INSTANCE = 5, 6, 7, 8
+NF_OPT_SLOT_SCALE = 9
 NF_ENGINE_FOOTPRINT, NF_ENGINE_CELLS = 0, 1
 KERNEL_CLASSES = ["gen", "bin", "reject_old", "reject_new", "commit", "control", "contact", "other"]
 
@@ -128,6 +129,8 @@ class GenStats(C.Structure):
         ("tile_bytes", C.c_uint64),
         ("tile_bytes_new", C.c_uint64),
         ("rounds_launched", C.c_uint64),
+        ("slot_growths", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
     def to_dict(self) -> dict:

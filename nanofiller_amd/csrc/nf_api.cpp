@@ -595,6 +595,8 @@ extern "C" int nf_create_batch(const nf_params* ps, int n, nf_handle** out) {
     return fail(nullptr, NF_E_HIP, std::string("upload failed: ") + hipGetErrorString(e));
   }
   h->n_rods.assign(n, 0);
+  h->planned_cap.resize(n);
+  for (int i = 0; i < n; ++i) h->planned_cap[i] = h->desc[i].cap;
   *out = h;
   return NF_OK;
 }
@@ -623,6 +625,8 @@ extern "C" int nf_num_instances(const nf_handle* h) { return h ? (int)h->params.
 
 extern "C" const char* nf_version(void) { return "nanofiller-mi355x 0.1 (gfx950)"; }
 
+static int scale_slots(nf_handle* h, double scale);
+
 extern "C" int nf_set_option(nf_handle* h, int key, int64_t value) {
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   switch (key) {
@@ -643,6 +647,9 @@ extern "C" int nf_set_option(nf_handle* h, int key, int64_t value) {
       if (value < 0 || value > 1000000) return fail(h, NF_E_INVALID, "footprint fine cell factor out of range");
       h->opt_footprint_fine = (int)value;
       break;
+    case NF_OPT_SLOT_SCALE:
+      if (value < 1 || value > 1000000) return fail(h, NF_E_INVALID, "slot scale out of range (1..1000000)");
+      return scale_slots(h, (double)value * 1e-3);
     case NF_OPT_POST_INSTANCE:
       if (value < 0 || value >= (int64_t)h->params.size()) return fail(h, NF_E_INVALID, "no such instance");
       if ((int)value != h->post_inst) {
@@ -734,8 +741,93 @@ extern "C" int nf_set_milestones(nf_handle* h, const double* vf_percent, int32_t
   return NF_OK;
 }
 
+// Per-cell slots of the accepted rods (the "old" set): instance i gets
+// caps[i] slots per cell.  Reallocates the slot arrays, rewrites each
+// instance's offset and every cell's first slot, and drops a captured round
+// graph (its kernel arguments name the old arrays).  The packing does not
+// depend on the capacity, only whether it suffices.
+static int relayout_old(nf_handle* h, const std::vector<uint32_t>& caps) {
+  GenBuffers& b = h->gb;
+  const int n = (int)h->desc.size();
+  uint64_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    InstDesc& d = h->desc[i];
+    d.cap = caps[i];
+    d.old_off = off;
+    off += (uint64_t)d.ncells * d.cap;
+  }
+  if (off > 0xFFFFFFF0ull) return fail(h, NF_E_INVALID, "cell slots exceed the 32-bit index space");
+  dfree(h, b.oP);
+  dfree(h, b.oD);
+  dfree(h, b.oid);
+  b.oP = b.oD = nullptr;
+  b.oid = nullptr;
+  int rc = dalloc(h, &b.oP, std::max<uint64_t>(off, 1));
+  if (rc == NF_OK) rc = dalloc(h, &b.oD, std::max<uint64_t>(off, 1));
+  if (rc == NF_OK) rc = dalloc(h, &b.oid, std::max<uint64_t>(off, 1));
+  if (rc != NF_OK) return rc;
+  b.total_old = off;
+  std::vector<uint32_t> ostart(b.total_cells);
+  for (int i = 0; i < n; ++i) {
+    const InstDesc& d = h->desc[i];
+    for (uint32_t c = 0; c < d.ncells; ++c) ostart[d.cell_base + c] = (uint32_t)(d.old_off + (uint64_t)c * d.cap);
+  }
+  HIPCHK(h, hipMemcpy(b.desc, h->desc.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(b.ostart, ostart.data(), sizeof(uint32_t) * b.total_cells, hipMemcpyHostToDevice));
+  if (h->graph) {
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return NF_OK;
+}
+
+// The planned slots per cell scaled by scale (at least one, at most what
+// cannot overflow: the instance's target + 16).
+static int scale_slots(nf_handle* h, double scale) {
+  std::vector<uint32_t> caps(h->desc.size());
+  for (size_t i = 0; i < caps.size(); ++i) {
+    const double c = std::ceil((double)h->planned_cap[i] * scale);
+    caps[i] = (uint32_t)std::max(1.0, std::min(c, (double)h->desc[i].n_target + 16.0));
+  }
+  return relayout_old(h, caps);
+}
+
+static int generate_once(nf_handle* h, nf_gen_stats* stats, std::vector<int>* overflowed);
+
+// Generation with slot growth: an instance whose cell slots overflowed is
+// generated again from the start with twice the slots per cell (the packing
+// is the same; only its storage was too small), up to the capacity that
+// cannot overflow.
 extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
+  uint32_t growths = 0;
+  for (;;) {
+    std::vector<int> over;
+    const int rc = generate_once(h, stats, &over);
+    if (rc != NF_E_OVERFLOW || over.empty()) {
+      if (stats)
+        for (int i = 0; i < h->gb.n_inst; ++i) stats[i].slot_growths = growths;
+      return rc;
+    }
+    std::vector<uint32_t> caps(h->desc.size());
+    bool grew = false;
+    for (size_t i = 0; i < caps.size(); ++i) caps[i] = h->desc[i].cap;
+    for (int i : over) {
+      const uint32_t lim = h->desc[i].n_target + 16u;
+      const uint32_t c = (uint32_t)std::min<uint64_t>(2ull * caps[i], lim);
+      grew = grew || c > caps[i];
+      caps[i] = c;
+    }
+    if (!grew) return rc;
+    const int r2 = relayout_old(h, caps);
+    if (r2 != NF_OK) return r2;
+    ++growths;
+    trace("generate: cell slots overflowed, regenerating with %u slots per cell (instance %d)", caps[over[0]],
+          over[0]);
+  }
+}
+
+static int generate_once(nf_handle* h, nf_gen_stats* stats, std::vector<int>* overflowed) {
   h->err.clear();
   const int prec = h->params[0].precision;
   for (auto& p : h->params)
@@ -822,7 +914,10 @@ extern "C" int nf_generate(nf_handle* h, nf_gen_stats* stats) {
   int status_rc = NF_OK;
   for (int i = 0; i < h->gb.n_inst; ++i) {
     h->n_rods[i] = st[i].n_acc;
-    if (st[i].status == NF_STATUS_OVERFLOW) status_rc = NF_E_OVERFLOW;
+    if (st[i].status == NF_STATUS_OVERFLOW) {
+      status_rc = NF_E_OVERFLOW;
+      overflowed->push_back(i);
+    }
     if (stats) {
       nf_gen_stats& s = stats[i];
       std::memset(&s, 0, sizeof(s));

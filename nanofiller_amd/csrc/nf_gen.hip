@@ -258,6 +258,11 @@ __global__ void __launch_bounds__(GEN_THREADS) k_commit(const InstDesc* __restri
       oid[dst] = id;
       atomicMax((unsigned int*)&ohl[c], __float_as_uint(P.w));
     } else {
+      // no slot: give the count back, so that once this kernel ends every
+      // cell's count is at most its capacity (later passes of the other
+      // instances of a batch read a cell's first `count` slots), and stop
+      // the instance (the host generates it again with more slots)
+      atomicSub(&ocount[c], 1u);
       atomicOr(&st[inst].overflow, 1u);
     }
   }

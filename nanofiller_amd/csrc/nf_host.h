@@ -29,6 +29,7 @@ struct nf_handle {
   int post_inst = 0;  // instance that contacts, statistics, network, reduce, writers, state files and expand use
   int device = 0;
   std::vector<uint32_t> n_rods;  // current rod count per instance
+  std::vector<uint32_t> planned_cap;  // cell slots per instance as planned (NF_OPT_SLOT_SCALE scales them)
   std::vector<double> milestones;  // volume fractions (percent) of the state dumps, ascending
   std::string milestone_dir;       // where nf_generate writes them
   bool generated = false;

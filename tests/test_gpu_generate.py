@@ -143,3 +143,33 @@ def test_long_rods_in_small_box_default_cells():
         rg = pk.rods()
     np.testing.assert_array_equal(rg.view(np.uint32), ro.view(np.uint32))
     assert st.pair_tests == so["pair_tests"]
+
+
+@pytest.mark.parametrize("scale", [20, 60])
+def test_slot_overflow_regenerates_the_same_packing(scale):
+    """Cell slots planned far too small (2% or 6% of the plan) overflow;
+    the packing is generated again with twice the slots until they suffice,
+    identical to the default run, alone and in a batch.  The grown slots are
+    kept for the next generation."""
+    ps = [cases.small_cfg2(nf.NF_PREC_FP32_RECHECK), cases.small_iso(nf.NF_PREC_FP32_RECHECK)]
+    refs = []
+    for p in ps:
+        with nf.Packing(p) as pk:
+            st = pk.generate()[0]
+            refs.append((st, pk.rods()))
+        assert st.slot_growths == 0
+    with nf.Packing(ps[0], {nf.NF_OPT_SLOT_SCALE: scale}) as pk:
+        st2 = pk.generate()[0]
+        rods = pk.rods()
+        st3 = pk.generate()[0]
+    assert st2.slot_growths > 0 and st3.slot_growths == 0
+    for s in (st2, st3):
+        assert nf.STATUS[s.status] == nf.STATUS[refs[0][0].status]
+        assert s.n_accepted == refs[0][0].n_accepted and s.pair_tests == refs[0][0].pair_tests
+    np.testing.assert_array_equal(rods.view(np.uint32), refs[0][1].view(np.uint32))
+    with nf.Packing(ps, {nf.NF_OPT_SLOT_SCALE: scale}) as pk:
+        sts = pk.generate()
+        for i in range(2):
+            assert sts[i].n_accepted == refs[i][0].n_accepted and sts[i].rounds == refs[i][0].rounds
+            np.testing.assert_array_equal(pk.rods(i).view(np.uint32), refs[i][1].view(np.uint32))
+    assert sts[0].slot_growths > 0
