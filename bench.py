@@ -131,10 +131,10 @@ def timed_generation(p, steps: int, warmup: int, options=None):
     return dt, sts, rods, cont
 
 
-def profiled_generation(p) -> dict:
+def profiled_generation(p, tiles=None) -> dict:
     """One generation with HIP events per kernel class: the two pair passes'
     times and streamed bytes (probes and neighbour tiles)."""
-    with nf.Packing(p, {nf.NF_OPT_PROFILE: 1}) as pk:
+    with nf.Packing(p, {nf.NF_OPT_PROFILE: 1, **(tiles or {})}) as pk:
         sp = pk.generate()[0]
     return {"wall_ms": sp.wall_ms, "rounds": sp.rounds, "launches": sp.rounds_launched,
             "kernel_ms": {k: sp.kern_ms[i] for i, k in enumerate(nf.KERNEL_CLASSES)},
@@ -150,15 +150,16 @@ def same_contacts(a, b) -> bool:
 
 def leg_main(a, out: dict) -> None:
     p32 = configs.cfg2(nf.NF_PREC_FP32_RECHECK)
-    log("headline: FP32 + FP64 re-check")
-    dt, sts, rods32, c32 = timed_generation(p32, a.steps, a.warmup)
+    tiles = configs.tile_options("500k")  # launch shape picked by nanofiller_amd/autotune.py
+    log(f"headline: FP32 + FP64 re-check (tile options {tiles})")
+    dt, sts, rods32, c32 = timed_generation(p32, a.steps, a.warmup, tiles)
     st = sts[-1]
     rods_s = st.n_accepted * a.steps / dt
     pair_s = sum(s.pair_tests for s in sts) / dt
     ms_step = dt / a.steps * 1e3
     log(f"FP32: {rods_s:.4e} rods/s, {ms_step:.2f} ms/step, {st.rounds} rounds")
     # one profiled generation for the per-kernel times and the roofline
-    with nf.Packing(p32, {nf.NF_OPT_PROFILE: 1}) as pk:
+    with nf.Packing(p32, {nf.NF_OPT_PROFILE: 1, **tiles}) as pk:
         sp = pk.generate()[0]
     launches = max(sp.rounds_launched, 1)
     k_old = sp.kern_ms[K_OLD]
@@ -175,6 +176,7 @@ def leg_main(a, out: dict) -> None:
         "launches": launches, "valu": valu_roof("k_pair<0, 1> 500k", k_old / launches),
     }
     out["extra"]["fp32_recheck"] = {
+        "tile_options": tiles,
         "accepted_rods_per_s": rods_s, "pair_tests_per_s": pair_s, "ms_per_step": ms_step,
         "rounds": st.rounds, "candidates": st.candidates, "proposals": st.proposals,
         "pair_tests": st.pair_tests, "rechecks": st.rechecks,
@@ -184,7 +186,7 @@ def leg_main(a, out: dict) -> None:
     }
     # FP32-vs-FP64 distance error: one generation that also evaluates every
     # FP32-decided pair in FP64 (NF_OPT_DIAG); its packing must not change
-    with nf.Packing(p32, {nf.NF_OPT_DIAG: 1}) as pk:
+    with nf.Packing(p32, {nf.NF_OPT_DIAG: 1, **tiles}) as pk:
         sd = pk.generate()[0]
         rods_diag = pk.rods()
     out["extra"]["fp32_recheck"]["max_abs_d32_d64_um"] = sd.max_abs_d32_d64
@@ -198,7 +200,7 @@ def leg_main(a, out: dict) -> None:
     p64 = configs.cfg2(nf.NF_PREC_FP64)
     log("headline: plain FP64")
     steps64 = max(1, min(a.steps, 3))
-    dt64, sts64, rods64, c64 = timed_generation(p64, steps64, 1)
+    dt64, sts64, rods64, c64 = timed_generation(p64, steps64, 1, tiles)
     rods64_s = sts64[-1].n_accepted * steps64 / dt64
     same_rods = rods64.shape == rods32.shape and np.array_equal(rods64.view(np.uint32),
                                                                   rods32.view(np.uint32))
@@ -231,7 +233,7 @@ def leg_cfg1(a, out: dict) -> None:
     rods = {}
     for name, prec in (("fp64_plain", nf.NF_PREC_FP64), ("fp32_recheck", nf.NF_PREC_FP32_RECHECK)):
         p = configs.cfg1(prec)
-        dt, sts, r, c = timed_generation(p, a.steps, a.warmup)
+        dt, sts, r, c = timed_generation(p, a.steps, a.warmup, configs.tile_options("20k"))
         rods[name] = (r, c)
         res[name] = {"accepted_rods_per_s": sts[-1].n_accepted * a.steps / dt,
                      "pair_tests_per_s": sum(s.pair_tests for s in sts) / dt,
@@ -262,7 +264,7 @@ def leg_batch(a, out: dict) -> None:
 
     def batched(ps, label):
         log(f"batch: {len(ps)} packings, {total} rods, batched {label}")
-        with nf.Packing(ps) as pk:
+        with nf.Packing(ps, configs.tile_options("batch256")) as pk:
             pk.generate()  # warm-up
             t0 = time.perf_counter()
             sts = pk.generate()
@@ -297,6 +299,7 @@ def leg_batch(a, out: dict) -> None:
     log(f"sequential FP64: {s_rate:.4e} rods/s ({t_seq * 1e3:.1f} ms); identical packings {same}")
     status = collections.Counter(nf.STATUS[s.status] for s in sts)
     out["extra"]["batch"] = {
+        "tile_options": configs.tile_options("batch256"),
         "workload": f"{len(ps32)} packings, N log-uniform 2k-200k (sum {total}), aspect 20-500, "
                     "vf 0.1-2%, half isotropic / half vMF kappa 1-20 about random axes, "
                     f"round cap {configs.CFG3_MAX_ROUNDS}",
@@ -317,7 +320,7 @@ def leg_batch(a, out: dict) -> None:
 def leg_contacts(a, out: dict) -> None:
     p = configs.cfg4(nf.NF_PREC_FP32_RECHECK)
     log("contacts: 4M rods, generate")
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, configs.tile_options("4M")) as pk:
         pk.generate()  # warm-up
         t0 = time.perf_counter()
         st = pk.generate()[0]
@@ -350,7 +353,7 @@ def leg_contacts(a, out: dict) -> None:
         c64 = pk.contacts(p.contact_gap)
         t_c64 = time.perf_counter() - t0
         cs64 = pk.contact_stats()
-    gen_prof = profiled_generation(p)
+    gen_prof = profiled_generation(p, configs.tile_options("4M"))
     km = cs["kernel_ms"]
     # the algorithmic bytes of the contact pass: N * 32 B * 14 tiles (27-cell
     # stencil with Newton's third law) + C * 12 B (I, J, gap)

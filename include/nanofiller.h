@@ -470,10 +470,19 @@ enum {
                                 network, reduce, conductance, the writers, the state
                                 files and expand act on (default 0); changing it drops
                                 the results computed for the previous one              */
-  NF_OPT_SLOT_SCALE = 9      /* per-cell slots of the accepted rods in 1/1000 of the
+  NF_OPT_SLOT_SCALE = 9,     /* per-cell slots of the accepted rods in 1/1000 of the
                                 planned capacity (default 1000); a generation whose
                                 slots overflow is repeated with twice as many, with
                                 the same packing (nf_gen_stats.slot_growths)           */
+  /* Launch shape of the generation's pair passes (the packing does not
+   * depend on it; nanofiller_amd/autotune.py picks it per workload): */
+  NF_OPT_WAVES_OLD = 10,     /* waves per CU, reject vs accepted (0: default 16)     */
+  NF_OPT_WAVES_NEW = 11,     /* waves per CU, reject vs same-round (0: default 16)   */
+  NF_OPT_PART_SHIFT = 12,    /* reject vs accepted: a dense cell is dealt over
+                                1 << shift work items in launches with few cells per
+                                wave (-1: default 1)                                    */
+  NF_OPT_SPLIT_MIN = 13      /* probes per work item of a dealt cell, at least
+                                (0: default 32)                                         */
 };
 /* Candidate pairs of nf_contacts.  Both give the same contact list.
  * FOOTPRINT: each rod binned into the fine cells its capsule (radius

@@ -1,4 +1,7 @@
-"""Cell-size autotuner for the batched packings (256 uneven instances).
+"""Autotuner: cell size of the batched packings and the launch shape of the
+generation's pair passes.
+
+Cell size (the 256 uneven packings).
 
 Instances are grouped in buckets (decade of N x aspect-ratio tercile, see
 configs.bucket).  The tuner times the batched FP32 + recheck generation of
@@ -9,8 +12,18 @@ shortens the batch by more than 2%.  The packing does not depend on the
 cell edge - every accept decision is a function of the candidate stream
 alone - so tuning changes speed only; the tests check that.
 
+Launch shape (--tiles): per workload (the 20k-, 500k- and 4M-rod packings
+and the 256-packing batch), the waves per CU of the reject-vs-accepted and
+same-round passes, and for launches with few cells per wave the split of a
+dense cell (work items as a shift, and the probes per item at least), by one
+pass of coordinate descent over the generation time from the library's
+defaults, keeping a change for a gain over 2%.  The batch's launches have a
+million cells, so only its wave counts are tuned.  The packing does not
+depend on the launch shape either (test_tile_shape_does_not_change_packing).
+
 Usage: python -m nanofiller_amd.autotune [--factors 0.7,0.85,1,1.2,1.4]
-Writes nanofiller_amd/cell_tuning.json.
+       python -m nanofiller_amd.autotune --tiles
+Writes nanofiller_amd/cell_tuning.json (bucket cell factors; "tiles").
 """
 from __future__ import annotations
 
@@ -21,8 +34,65 @@ import time
 
 import numpy as np
 
-from . import NF_PREC_FP32_RECHECK, NFError, Packing
+from . import (NF_OPT_PART_SHIFT, NF_OPT_SPLIT_MIN, NF_OPT_WAVES_NEW, NF_OPT_WAVES_OLD,
+               NF_PREC_FP32_RECHECK, NFError, Packing)
 from . import configs
+
+# option keys of a tile entry in cell_tuning.json
+TILE_KEYS = {"waves_old": NF_OPT_WAVES_OLD, "waves_new": NF_OPT_WAVES_NEW, "part_shift": NF_OPT_PART_SHIFT,
+             "split_min": NF_OPT_SPLIT_MIN}
+TILE_DEFAULTS = {"waves_old": 16, "waves_new": 16, "part_shift": 1, "split_min": 32}
+TILE_CHOICES = {"waves_old": [8, 12, 16], "waves_new": [8, 12, 16], "part_shift": [0, 1, 2],
+                "split_min": [16, 32, 64]}
+
+
+def _gen_ms(params, tiles: dict, repeats: int) -> float:
+    """Best-of-`repeats` generation time with the tile options set."""
+    opts = {TILE_KEYS[k]: v for k, v in tiles.items()}
+    best = float("inf")
+    with Packing(params, opts) as pk:
+        pk.generate()  # warm-up
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            pk.generate()
+            best = min(best, time.perf_counter() - t0)
+    return best * 1e3
+
+
+def tile_workloads() -> dict:
+    """The workloads the launch shape is tuned on: parameters, keys tuned,
+    repeats."""
+    return {
+        "20k": (configs.cfg1(NF_PREC_FP32_RECHECK), list(TILE_CHOICES), 3),
+        "500k": (configs.cfg2(NF_PREC_FP32_RECHECK), list(TILE_CHOICES), 5),
+        "4M": (configs.cfg4(NF_PREC_FP32_RECHECK), list(TILE_CHOICES), 3),
+        "batch256": (configs.cfg3(NF_PREC_FP32_RECHECK), ["waves_old", "waves_new"], 1),
+    }
+
+
+def tune_tiles(names=None, log=print) -> dict:
+    out = {}
+    for name, (params, keys, reps) in tile_workloads().items():
+        if names and name not in names:
+            continue
+        cur = dict(TILE_DEFAULTS)
+        cur_ms = _gen_ms(params, cur, reps)
+        tried = {json.dumps(cur, sort_keys=True): cur_ms}
+        log(f"{name}: defaults {cur_ms:.2f} ms")
+        for k in keys:
+            for v in TILE_CHOICES[k]:
+                if v == cur[k]:
+                    continue
+                trial = dict(cur, **{k: v})
+                t = _gen_ms(params, trial, reps)
+                tried[json.dumps(trial, sort_keys=True)] = t
+                log(f"{name}: {k}={v} {t:.2f} ms")
+                if t < cur_ms * 0.98:
+                    cur, cur_ms = trial, t
+        out[name] = dict(cur, ms=cur_ms, ms_defaults=tried[json.dumps(TILE_DEFAULTS, sort_keys=True)],
+                         tried={kk: vv for kk, vv in tried.items()})
+        log(f"{name}: {cur} {cur_ms:.2f} ms")
+    return out
 
 
 def _batch_ms(shapes, base, fmap, repeats: int = 2) -> float:
@@ -83,13 +153,23 @@ def tune(factors, n_inst: int = 256, seed: int = 3, log=print) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--factors", default="0.7,0.85,1.0,1.2,1.4,1.7")
+    ap.add_argument("--tiles", action="store_true", help="tune the pair passes' launch shape")
+    ap.add_argument("--workloads", default="", help="with --tiles: comma-separated subset")
     ap.add_argument("--out", default=configs.TUNING_FILE)
     a = ap.parse_args()
-    factors = [float(x) for x in a.factors.split(",")]
-    table = tune(factors, log=lambda m: print(m, file=sys.stderr, flush=True))
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    current = configs.load_tuning() if a.out == configs.TUNING_FILE else {}
+    if a.tiles:
+        tiles = tune_tiles([w for w in a.workloads.split(",") if w] or None, log=log)
+        current.setdefault("tiles", {}).update(tiles)
+        print(json.dumps({k: {kk: v[kk] for kk in TILE_DEFAULTS} for k, v in tiles.items()}))
+    else:
+        factors = [float(x) for x in a.factors.split(",")]
+        table = tune(factors, log=log)
+        current.update(table)
+        print(json.dumps({b: t["cell_factor"] for b, t in table.items()}))
     with open(a.out, "w") as f:
-        json.dump(table, f, indent=1, sort_keys=True)
-    print(json.dumps({b: t["cell_factor"] for b, t in table.items()}))
+        json.dump(current, f, indent=1, sort_keys=True)
     return 0
 
 

@@ -110,6 +110,18 @@ def load_tuning() -> dict:
     return {}
 
 
+def tile_options(workload: str) -> dict:
+    """Handle options of the launch shape the autotuner picked for a
+    workload ("20k", "500k", "4M", "batch256"; nanofiller_amd/autotune.py
+    --tiles), {} when untuned."""
+    from . import NF_OPT_PART_SHIFT, NF_OPT_SPLIT_MIN, NF_OPT_WAVES_NEW, NF_OPT_WAVES_OLD
+
+    t = load_tuning().get("tiles", {}).get(workload, {})
+    keys = {"waves_old": NF_OPT_WAVES_OLD, "waves_new": NF_OPT_WAVES_NEW, "part_shift": NF_OPT_PART_SHIFT,
+            "split_min": NF_OPT_SPLIT_MIN}
+    return {keys[k]: int(v) for k, v in t.items() if k in keys}
+
+
 def cfg3(precision=NF_PREC_FP32_RECHECK, n_inst: int = 256, tuned: bool = True, seed: int = 3,
          cell_factor: float | None = None, n_scale: float = 1.0,
          max_rounds: int = CFG3_MAX_ROUNDS):

@@ -627,6 +627,14 @@ extern "C" const char* nf_version(void) { return "nanofiller-mi355x 0.1 (gfx950)
 
 static int scale_slots(nf_handle* h, double scale);
 
+// A captured round graph holds the launch shapes and buffers of its capture.
+static void drop_graph(nf_handle* h) {
+  if (h->graph) {
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+}
+
 extern "C" int nf_set_option(nf_handle* h, int key, int64_t value) {
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   switch (key) {
@@ -646,6 +654,22 @@ extern "C" int nf_set_option(nf_handle* h, int key, int64_t value) {
     case NF_OPT_FOOTPRINT_FINE:
       if (value < 0 || value > 1000000) return fail(h, NF_E_INVALID, "footprint fine cell factor out of range");
       h->opt_footprint_fine = (int)value;
+      break;
+    case NF_OPT_WAVES_OLD:
+    case NF_OPT_WAVES_NEW:
+      if (value < 0 || value > 16) return fail(h, NF_E_INVALID, "waves per CU out of range (0..16)");
+      (key == NF_OPT_WAVES_OLD ? h->gb.tiles.waves_old : h->gb.tiles.waves_new) = (uint32_t)value;
+      drop_graph(h);
+      break;
+    case NF_OPT_PART_SHIFT:
+      if (value < -1 || value > 3) return fail(h, NF_E_INVALID, "part shift out of range (-1..3)");
+      h->gb.tiles.part_shift = (int32_t)value;
+      drop_graph(h);
+      break;
+    case NF_OPT_SPLIT_MIN:
+      if (value < 0 || value > 4096) return fail(h, NF_E_INVALID, "split minimum out of range (0..4096)");
+      h->gb.tiles.split_min = (uint32_t)value;
+      drop_graph(h);
       break;
     case NF_OPT_SLOT_SCALE:
       if (value < 1 || value > 1000000) return fail(h, NF_E_INVALID, "slot scale out of range (1..1000000)");

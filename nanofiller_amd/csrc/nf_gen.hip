@@ -382,7 +382,7 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   a.nbHl = b.ohl;
   a.qctr = b.queues + Q_OLD * QWORDS;
   a.alive_out = b.palive;
-  launch_pair(PM_GEN_OLD, prec, a, s);
+  launch_pair(PM_GEN_OLD, prec, a, s, &b.tiles);
   if (ev) (void)hipEventRecord(ev[4], s);
   // stage 2: survivors vs higher-priority survivors, over the survivors
   // compacted per cell (a cell's longest rod stays that of all its
@@ -406,7 +406,7 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
   a.qctr = b.queues + Q_NEW * QWORDS;
   a.alive_out = nullptr;
   a.surv = b.surv;
-  launch_pair(PM_GEN_NEW, prec, a, s);
+  launch_pair(PM_GEN_NEW, prec, a, s, &b.tiles);
   if (ev) (void)hipEventRecord(ev[5], s);
   scan_u32(b.surv, b.srank, b.T_cap, b.scan_tmp, s);
   hipLaunchKernelGGL(k_commit, dim3(gen_grid(b.T_cap)), dim3(GEN_THREADS), 0, s, b.desc, b.st,

@@ -84,7 +84,17 @@ constexpr int QWORDS = QUEUES_PER_KERNEL * QSTRIDE;  // words per launch
 enum { Q_OLD = 0, Q_NEW = 1, Q_CNT = 2, Q_REC = 3, Q_N = 4 };
 
 // Device buffers of a generation batch.
+// Launch shape of the generation's pair passes (0 / -1: the built-in
+// choice of pair_grid, pair_part_shift and pair_split_min).
+struct PairTiles {
+  uint32_t waves_old = 0;  // waves per CU, reject vs accepted rods
+  uint32_t waves_new = 0;  // waves per CU, reject vs same-round proposals
+  int32_t part_shift = -1; // work items per dense cell (reject vs accepted), as a shift
+  uint32_t split_min = 0;  // probes per part of a split cell, at least
+};
+
 struct GenBuffers {
+  PairTiles tiles;
   InstDesc* desc = nullptr;
   InstState* st = nullptr;
   RoundInfo* ri = nullptr;
@@ -276,7 +286,7 @@ void launch_round(GenBuffers& b, int prec, int diag, hipStream_t s, hipEvent_t* 
 void launch_ctl_init(GenBuffers& b, hipStream_t s);
 
 // ---- launchers (nf_pair.hip) ----
-void launch_pair(int mode, int prec, const PairArgs& a, hipStream_t s);
+void launch_pair(int mode, int prec, const PairArgs& a, hipStream_t s, const PairTiles* tiles = nullptr);
 // Persistent grid of a pair pass (waves): 16 per CU, the most that the
 // register budget lets reside (measured on MI355X, 500k-rod workload in 3
 // rounds: same-round pass 1.72 ms per generation at 16 per CU, 1.82 at 12;

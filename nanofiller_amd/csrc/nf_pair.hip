@@ -861,18 +861,23 @@ int pair_grid(int mode) {
   return g_cus * 16;
 }
 
-void launch_pair(int mode, int prec, const PairArgs& args, hipStream_t s) {
-  const dim3 grid(pair_grid(mode)), block(PAIR_THREADS);
+void launch_pair(int mode, int prec, const PairArgs& args, hipStream_t s, const PairTiles* t) {
+  // the handle's tuned launch shape where set (nanofiller_amd/autotune.py)
+  uint32_t waves = 0;
+  if (t) waves = mode == PM_GEN_OLD ? t->waves_old : (mode == PM_GEN_NEW ? t->waves_new : 0u);
+  const int cus = pair_grid(mode) / 16;
+  const dim3 grid(waves ? (uint32_t)cus * waves : (uint32_t)pair_grid(mode)), block(PAIR_THREADS);
   PairArgs a = args;
   // launches with few cells per wave: costliest cells first (under 64 per
   // wave), dense cells split (under 16 per wave)
   const bool small = a.ncells_total < 16u * grid.x;
-  a.part_shift = small ? (uint32_t)pair_part_shift(mode) : 0u;
+  const int ps = (t && mode == PM_GEN_OLD && t->part_shift >= 0) ? t->part_shift : pair_part_shift(mode);
+  a.part_shift = small ? (uint32_t)ps : 0u;
   if (a.ncells_total < 64u * grid.x && a.order != nullptr)
     hipLaunchKernelGGL(k_order, dim3(QUEUES_PER_KERNEL), dim3(ORDER_THREADS), 0, s, a);
   else
     a.order = nullptr;
-  a.split_min = (uint32_t)pair_split_min(mode);
+  a.split_min = (t && t->split_min) ? t->split_min : (uint32_t)pair_split_min(mode);
 #define NF_LAUNCH(M, P) hipLaunchKernelGGL((k_pair<M, P>), grid, block, 0, s, a)
   if (prec == NF_PREC_FP64) {
     switch (mode) {

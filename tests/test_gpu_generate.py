@@ -173,3 +173,18 @@ def test_slot_overflow_regenerates_the_same_packing(scale):
             assert sts[i].n_accepted == refs[i][0].n_accepted and sts[i].rounds == refs[i][0].rounds
             np.testing.assert_array_equal(pk.rods(i).view(np.uint32), refs[i][1].view(np.uint32))
     assert sts[0].slot_growths > 0
+
+
+@pytest.mark.parametrize("tiles", [{nf.NF_OPT_WAVES_OLD: 4, nf.NF_OPT_WAVES_NEW: 8},
+                                   {nf.NF_OPT_PART_SHIFT: 0, nf.NF_OPT_SPLIT_MIN: 8},
+                                   {nf.NF_OPT_PART_SHIFT: 3, nf.NF_OPT_SPLIT_MIN: 16, nf.NF_OPT_WAVES_OLD: 12}])
+def test_tile_shape_does_not_change_packing(tiles):
+    """The launch shape the autotuner picks (waves per CU, dense-cell split)
+    changes speed only: the same packing and counters as the oracle."""
+    p = cases.small_cfg2(nf.NF_PREC_FP32_RECHECK)
+    ro, so, _ = oracle.generate(p, threads=16)
+    with nf.Packing(p, tiles) as pk:
+        st = pk.generate()[0]
+        rods = pk.rods()
+    assert st.n_accepted == so["n_accepted"] and st.pair_tests == so["pair_tests"]
+    np.testing.assert_array_equal(rods.view(np.uint32), ro.view(np.uint32))
