@@ -12,14 +12,16 @@ shortens the batch by more than 2%.  The packing does not depend on the
 cell edge - every accept decision is a function of the candidate stream
 alone - so tuning changes speed only; the tests check that.
 
-Launch shape (--tiles): per workload (the 20k-, 500k- and 4M-rod packings
-and the 256-packing batch), the waves per CU of the reject-vs-accepted and
-same-round passes, and for launches with few cells per wave the split of a
-dense cell (work items as a shift, and the probes per item at least), by one
-pass of coordinate descent over the generation time from the library's
-defaults, keeping a change for a gain over 2%.  The batch's launches have a
-million cells, so only its wave counts are tuned.  The packing does not
-depend on the launch shape either (test_tile_shape_does_not_change_packing).
+Launch shape (--tiles): per workload (the 20k-, 500k-, 4M- and 8M-rod
+packings and the 256-packing batch), the generation cell edge as a factor of
+the library's default (single packings without a set cell size), the waves
+per CU of the reject-vs-accepted and same-round passes, and for launches
+with few cells per wave the split of a dense cell (work items as a shift,
+and the probes per item at least), by one pass of coordinate descent over
+the generation time from the library's defaults, keeping a change for a
+gain over 2%.  The batch's launches have a million cells, so only its wave
+counts are tuned.  The packing does not depend on any of these
+(test_tile_shape_does_not_change_packing, test_cell_size_does_not_change_packing).
 
 Usage: python -m nanofiller_amd.autotune [--factors 0.7,0.85,1,1.2,1.4]
        python -m nanofiller_amd.autotune --tiles
@@ -41,14 +43,20 @@ from . import configs
 # option keys of a tile entry in cell_tuning.json
 TILE_KEYS = {"waves_old": NF_OPT_WAVES_OLD, "waves_new": NF_OPT_WAVES_NEW, "part_shift": NF_OPT_PART_SHIFT,
              "split_min": NF_OPT_SPLIT_MIN}
-TILE_DEFAULTS = {"waves_old": 16, "waves_new": 16, "part_shift": 1, "split_min": 32}
-TILE_CHOICES = {"waves_old": [8, 12, 16], "waves_new": [8, 12, 16], "part_shift": [0, 1, 2],
-                "split_min": [16, 32, 64]}
+TILE_DEFAULTS = {"cell_factor": 1.0, "waves_old": 16, "waves_new": 16, "part_shift": 1, "split_min": 32}
+# cell_factor scales the library's default generation cell edge (0.85 of a
+# typical pair reach; 1.18 makes it one reach), passed as cell_size = -f
+TILE_CHOICES = {"cell_factor": [0.85, 1.0, 1.18, 1.35], "waves_old": [8, 12, 16], "waves_new": [8, 12, 16],
+                "part_shift": [0, 1, 2], "split_min": [16, 32, 64]}
 
 
 def _gen_ms(params, tiles: dict, repeats: int) -> float:
-    """Best-of-`repeats` generation time with the tile options set."""
-    opts = {TILE_KEYS[k]: v for k, v in tiles.items()}
+    """Best-of-`repeats` generation time with the tile options set (and the
+    cell factor, when tuned, as cell_size = -factor)."""
+    opts = {TILE_KEYS[k]: v for k, v in tiles.items() if k in TILE_KEYS}
+    if "cell_factor" in tiles:
+        params = params.copy()
+        params.cell_size = -float(tiles["cell_factor"])
     best = float("inf")
     with Packing(params, opts) as pk:
         pk.generate()  # warm-up
@@ -62,10 +70,14 @@ def _gen_ms(params, tiles: dict, repeats: int) -> float:
 def tile_workloads() -> dict:
     """The workloads the launch shape is tuned on: parameters, keys tuned,
     repeats."""
+    launch = ["waves_old", "waves_new", "part_shift", "split_min"]
     return {
-        "20k": (configs.cfg1(NF_PREC_FP32_RECHECK), list(TILE_CHOICES), 3),
-        "500k": (configs.cfg2(NF_PREC_FP32_RECHECK), list(TILE_CHOICES), 5),
-        "4M": (configs.cfg4(NF_PREC_FP32_RECHECK), list(TILE_CHOICES), 3),
+        # the 20k case keeps its 1 um cells (the reference's domain grid)
+        "20k": (configs.cfg1(NF_PREC_FP32_RECHECK), launch, 3),
+        "500k": (configs.cfg2(NF_PREC_FP32_RECHECK, tuned=False), ["cell_factor"] + launch, 5),
+        "4M": (configs.cfg4(NF_PREC_FP32_RECHECK, tuned=False), ["cell_factor"] + launch, 3),
+        "8M": (configs.cfg5(NF_PREC_FP32_RECHECK, tuned=False), ["cell_factor", "waves_old", "waves_new"], 1),
+        # per-bucket cell factors (tune()); its launches hold a million cells
         "batch256": (configs.cfg3(NF_PREC_FP32_RECHECK), ["waves_old", "waves_new"], 1),
     }
 
@@ -75,7 +87,7 @@ def tune_tiles(names=None, log=print) -> dict:
     for name, (params, keys, reps) in tile_workloads().items():
         if names and name not in names:
             continue
-        cur = dict(TILE_DEFAULTS)
+        cur = {k: v for k, v in TILE_DEFAULTS.items() if k in keys or k in TILE_KEYS}
         cur_ms = _gen_ms(params, cur, reps)
         tried = {json.dumps(cur, sort_keys=True): cur_ms}
         log(f"{name}: defaults {cur_ms:.2f} ms")
@@ -89,7 +101,8 @@ def tune_tiles(names=None, log=print) -> dict:
                 log(f"{name}: {k}={v} {t:.2f} ms")
                 if t < cur_ms * 0.98:
                     cur, cur_ms = trial, t
-        out[name] = dict(cur, ms=cur_ms, ms_defaults=tried[json.dumps(TILE_DEFAULTS, sort_keys=True)],
+        d0 = {k: v for k, v in TILE_DEFAULTS.items() if k in keys or k in TILE_KEYS}
+        out[name] = dict(cur, ms=cur_ms, ms_defaults=tried[json.dumps(d0, sort_keys=True)],
                          tried={kk: vv for kk, vv in tried.items()})
         log(f"{name}: {cur} {cur_ms:.2f} ms")
     return out
@@ -162,7 +175,7 @@ def main() -> int:
     if a.tiles:
         tiles = tune_tiles([w for w in a.workloads.split(",") if w] or None, log=log)
         current.setdefault("tiles", {}).update(tiles)
-        print(json.dumps({k: {kk: v[kk] for kk in TILE_DEFAULTS} for k, v in tiles.items()}))
+        print(json.dumps({k: {kk: v[kk] for kk in TILE_DEFAULTS if kk in v} for k, v in tiles.items()}))
     else:
         factors = [float(x) for x in a.factors.split(",")]
         table = tune(factors, log=log)

@@ -1,5 +1,9 @@
 """Parameter sets of the benchmark workloads (all lengths in um).
 
+The single-packing sets take the generation cell factor the autotuner
+recorded for them in cell_tuning.json ("tiles"; tuned=False: the library's
+default edge); the packing does not depend on it.
+
 cfg1  20,000 rods, l = 1, d = 10 nm, 5 um periodic cube, uniform, isotropic,
       FP64, seed 1, 1 um cells, contacts <= 2 nm.
 cfg2  500,000 rods in a 20 um cube: log-normal l (median 1, sigma 0.3, in
@@ -47,30 +51,43 @@ def cfg1(precision=NF_PREC_FP64, **kw):
     return normalize(default_params(**base))
 
 
-def cfg2(precision=NF_PREC_FP32_RECHECK, **kw):
+def tuned_cell_factor(workload: str):
+    """The generation cell factor the autotuner picked for a workload
+    (cell_size = -factor), or None."""
+    f = load_tuning().get("tiles", {}).get(workload, {}).get("cell_factor")
+    return float(f) if f is not None else None
+
+
+def _tuned(base: dict, workload: str, tuned: bool, kw: dict) -> dict:
+    if tuned and "cell_size" not in kw:
+        f = tuned_cell_factor(workload)
+        if f is not None:
+            base["cell_size"] = -f
+    base.update(kw)
+    return base
+
+
+def cfg2(precision=NF_PREC_FP32_RECHECK, tuned: bool = True, **kw):
     base = dict(box=(20, 20, 20), n_rods=500000, len=1.0, len_law=NF_LEN_LOGNORMAL, len_sigma=0.3,
                 len_min=0.3, len_max=3.0, radius=0.005, seed=2, den_law=NF_DEN_AGGLOM, n_agg=8,
                 agg_sigma=1.5, agg_peak=3.0, ort_law=NF_ORT_VMF, kappa=5.0, axis=(0, 0, 1),
                 contact_gap=0.002, precision=precision)
-    base.update(kw)
-    return normalize(default_params(**base))
+    return normalize(default_params(**_tuned(base, "500k", tuned, kw)))
 
 
-def cfg4(precision=NF_PREC_FP32_RECHECK, **kw):
+def cfg4(precision=NF_PREC_FP32_RECHECK, tuned: bool = True, **kw):
     base = dict(box=(50, 50, 50), n_rods=4000000, len=1.0, radius=0.005, seed=4,
                 ort_law=NF_ORT_VMF, kappa=2.0, axis=(1, 0, 0), contact_gap=0.002,
                 precision=precision)
-    base.update(kw)
-    return normalize(default_params(**base))
+    return normalize(default_params(**_tuned(base, "4M", tuned, kw)))
 
 
-def cfg5(precision=NF_PREC_FP32_RECHECK, **kw):
+def cfg5(precision=NF_PREC_FP32_RECHECK, tuned: bool = True, **kw):
     base = dict(box=(40, 40, 40), n_rods=8000000, len=1.0, radius=0.005, seed=5,
                 den_law=NF_DEN_AGGLOM, n_agg=16, agg_sigma=2.0, agg_peak=3.0, ort_law=NF_ORT_VMF,
                 kappa=3.0, axis=(1, 0, 0), contact_gap=0.002, elec_axis=0, tunnel_lambda=1e-4,
                 tunnel_g0=1.0, precision=precision)
-    base.update(kw)
-    return normalize(default_params(**base))
+    return normalize(default_params(**_tuned(base, "8M", tuned, kw)))
 
 
 def cfg3_instances(n_inst: int = 256, seed: int = 3, n_scale: float = 1.0, n_min: int = 200):
