@@ -444,7 +444,8 @@ void nf::release_contacts(nf_handle* h) {
                   c.rowptr, c.fill, c.J, c.gap, c.gap64, c.img, c.area, c.labels, c.nrow, c.ncol, c.nval,
                   c.nd_pflag, c.nd_edge_a, c.nd_edge_b, c.nd_esrc, c.nd_esnk, c.nd_src_node,
                   c.nd_snk_node, c.nd_egs, c.nd_egk, c.nd_node_rod, c.nd_node_t, c.nd_seg_a,
-                  c.nd_seg_b, c.nd_seg_L, c.nd_seg_A};
+                  c.nd_seg_b, c.nd_seg_L, c.nd_seg_A, c.nd_ce_a, c.nd_ce_b, c.nd_ce_k, c.nd_cel_node,
+                  c.nd_cel_gap, c.nd_cel_side};
   for (void* p : ptrs) dfree(h, p);
   c = ContactBuffers();
 }

@@ -104,7 +104,17 @@ static int build_cell_list(nf_handle* h, double edge, double reach, bool half, i
 // half stencil of the longest pair reach.
 int build_contact_cells(nf_handle* h) {
   ContactBuffers& c = h->cb;
-  if (c.cP) return NF_OK;
+  if (c.cP && c.cell_list == 1) return NF_OK;
+  if (c.cP) {
+    // the footprint engine's list (home-cell grid, run stencil): rebuild
+    void* ptrs[] = {c.cP, c.cD, c.cid, c.cstart, c.ccount, c.chl, c.ccur, c.stencil};
+    for (void* q : ptrs) dfree(h, q);
+    c.cP = c.cD = nullptr;
+    c.cid = c.cstart = c.ccount = c.ccur = nullptr;
+    c.chl = nullptr;
+    c.stencil = nullptr;
+    c.cell_list = 0;
+  }
   const nf_params& p = h->params[h->post_inst];
   const double Rmax = ((double)c.a_max + (double)c.a_max) + c.gap_cut;
   const double reach = contact_reach(c);
@@ -113,7 +123,9 @@ int build_contact_cells(nf_handle* h) {
   double edge = (p.len_law == NF_LEN_LOGNORMAL ? p.len : (double)c.hl_max * 2.0) * REACH_PAD + Rmax;
   if (edge > reach) edge = reach;
   edge = default_cell_edge(edge, (double)c.n_rods / (p.box[0] * p.box[1] * p.box[2]), 1.0);
-  return build_cell_list(h, edge, reach, true);
+  const int rc = build_cell_list(h, edge, reach, true);
+  if (rc == NF_OK) c.cell_list = 1;
+  return rc;
 }
 
 // Footprint engine: home cells sized so that about 64 rods (the occupancy
@@ -210,6 +222,7 @@ static int contacts_pass(nf_handle* h, double gap, uint64_t* n_out, int prec, bo
     xa.g = seg_grid(p.box, edge, (double)amax, gap, ew);
     const double rpad = (double)xa.g.r + 2.0 * (double)CELL_MARGIN;
     rc = build_cell_list(h, edge, (double)hlmax * REACH_PAD + rpad, false, SEG_ROW);
+    if (rc == NF_OK) c.cell_list = 2;
     xa.fine = fine;
     xa.rpad = (float)rpad;
     const float rr = (float)((double)hlmax * REACH_PAD + rpad) * 1.0001f;

@@ -170,6 +170,7 @@ struct ContactBuffers {
   double gap_cut = 0.0;
   float hl_max = 0.0f, a_max = 0.0f;  // bounds on half length and radius of the rods
   int engine = 0;                     // NF_ENGINE_* that found the last contact list
+  int cell_list = 0;                  // the centre cell list held: 1 cell-pair engine's, 2 footprint engine's
   double fp_edge = 0.0;               // footprint engine: home cell edge (um) and fine cells
   int fp_fine = 0;                    // per axis of the last pass (0: cell engine)
   bool valid = false;
@@ -188,6 +189,11 @@ struct ContactBuffers {
   double* nd_node_t = nullptr;
   uint32_t *nd_seg_a = nullptr, *nd_seg_b = nullptr;
   double *nd_seg_L = nullptr, *nd_seg_A = nullptr;
+  uint32_t *nd_ce_a = nullptr, *nd_ce_b = nullptr, *nd_ce_k = nullptr;  // kept edges, compacted
+  uint32_t* nd_cel_node = nullptr;                                      // electrode points, compacted
+  double* nd_cel_gap = nullptr;
+  int32_t* nd_cel_side = nullptr;
+  uint64_t nd_n_edges = 0, nd_n_elec = 0;
   uint64_t nd_nodes = 0, nd_segments = 0, nd_points = 0;
   bool nd_valid = false;
 };
@@ -419,6 +425,11 @@ void launch_node_points(const NodeArgs& a, hipStream_t s);
 void launch_node_lists(const NodeArgs& a, hipStream_t s);
 void launch_node_nodes(const NodeArgs& a, hipStream_t s);
 // out[4] (zeroed by the caller): kept contacts, screened contacts, source and sink points.
+void launch_node_kept_flag(const uint32_t* pflag, uint32_t nc, uint32_t* f, hipStream_t s);
+void launch_node_tables(const NodeArgs& a, uint32_t nc, const uint32_t* kept, const uint32_t* rank_kept,
+                        uint32_t* oa, uint32_t* ob, uint32_t* ok, const uint32_t* rank_src,
+                        const uint32_t* rank_snk, uint32_t n_src, uint32_t* onode, double* ogap, int32_t* oside,
+                        hipStream_t s);
 void launch_node_count(const NodeArgs& a, uint32_t nc, unsigned long long* out, hipStream_t s);
 
 // ---- network extraction (nf_net.hip) ----

@@ -1,7 +1,8 @@
 // nf_nodes.cpp - host orchestration of the node-level segment network (see
 // nf_nodes.hip) on the contacts of the last nf_contacts call, and its
 // getters.  Three device phases; a synchronisation after each of the first
-// two sizes the next one's arrays.
+// two sizes the next one's arrays, and after the third, the compact tables
+// the getters copy (kept edges, electrode points) are built on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,11 +20,16 @@ void free_nodes(nf_handle* h) {
   ContactBuffers& c = h->cb;
   void* ptrs[] = {c.nd_pflag, c.nd_edge_a, c.nd_edge_b, c.nd_esrc,    c.nd_esnk,
                   c.nd_src_node, c.nd_snk_node, c.nd_egs, c.nd_egk, c.nd_node_rod,
-                  c.nd_node_t, c.nd_seg_a,  c.nd_seg_b, c.nd_seg_L, c.nd_seg_A};
+                  c.nd_node_t, c.nd_seg_a,  c.nd_seg_b, c.nd_seg_L, c.nd_seg_A,
+                  c.nd_ce_a, c.nd_ce_b, c.nd_ce_k, c.nd_cel_node, c.nd_cel_gap, c.nd_cel_side};
   for (void* p : ptrs) dfree(h, p);
   c.nd_pflag = c.nd_edge_a = c.nd_edge_b = c.nd_esrc = c.nd_esnk = nullptr;
   c.nd_src_node = c.nd_snk_node = c.nd_node_rod = c.nd_seg_a = c.nd_seg_b = nullptr;
   c.nd_egs = c.nd_egk = c.nd_node_t = c.nd_seg_L = c.nd_seg_A = nullptr;
+  c.nd_ce_a = c.nd_ce_b = c.nd_ce_k = c.nd_cel_node = nullptr;
+  c.nd_cel_gap = nullptr;
+  c.nd_cel_side = nullptr;
+  c.nd_n_edges = c.nd_n_elec = 0;
   c.nd_nodes = c.nd_segments = c.nd_points = 0;
   c.nd_valid = false;
 }
@@ -98,7 +104,7 @@ extern "C" int nf_node_network(nf_handle* h, nf_node_info* info) {
   ALLOC(c.nd_snk_node, n1);
   ALLOC(c.nd_egs, n1);
   ALLOC(c.nd_egk, n1);
-  ALLOC(tmp, scan_tmp_words(n1));
+  ALLOC(tmp, scan_tmp_words(std::max<uint64_t>(n1, ncs)));  // scans over rods and over contacts
   ALLOC(cnt, 4);
   auto fail_out = [&](int code, const std::string& m) {
     release();
@@ -167,11 +173,48 @@ extern "C" int nf_node_network(nf_handle* h, nf_node_info* info) {
   if (n > 0) launch_node_nodes(a, s);
   fill_u32((uint32_t*)cnt, 0u, 8, s);
   if (n > 0) launch_node_count(a, (uint32_t)nc, cnt, s);
-  hipEventRecord(h->ev[1], s);
   unsigned long long hc[4] = {0, 0, 0, 0};
   if (hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s) != hipSuccess ||
       sync_stream(s, "node network") != hipSuccess)
     return fail_out(NF_E_HIP, "node network: node phase failed");
+  // compact tables: kept edges (contact order), electrode points (source
+  // then sink, rod order)
+  uint32_t *kept = nullptr, *rkept = nullptr, *rsrc = nullptr, *rsnk = nullptr;
+  const uint64_t n_elec = hc[2] + hc[3];
+  if (rc == NF_OK) rc = dalloc(h, &kept, ncs);
+  if (rc == NF_OK) rc = dalloc(h, &rkept, ncs + 1);
+  if (rc == NF_OK) rc = dalloc(h, &rsrc, n1);
+  if (rc == NF_OK) rc = dalloc(h, &rsnk, n1);
+  if (rc == NF_OK) rc = dalloc(h, &c.nd_ce_a, std::max<uint64_t>(hc[0], 1));
+  if (rc == NF_OK) rc = dalloc(h, &c.nd_ce_b, std::max<uint64_t>(hc[0], 1));
+  if (rc == NF_OK) rc = dalloc(h, &c.nd_ce_k, std::max<uint64_t>(hc[0], 1));
+  if (rc == NF_OK) rc = dalloc(h, &c.nd_cel_node, std::max<uint64_t>(n_elec, 1));
+  if (rc == NF_OK) rc = dalloc(h, &c.nd_cel_gap, std::max<uint64_t>(n_elec, 1));
+  if (rc == NF_OK) rc = dalloc(h, &c.nd_cel_side, std::max<uint64_t>(n_elec, 1));
+  auto free_ranks = [&]() {
+    dfree(h, kept);
+    dfree(h, rkept);
+    dfree(h, rsrc);
+    dfree(h, rsnk);
+  };
+  if (rc != NF_OK) {
+    free_ranks();
+    return fail_out(rc, "node network: out of device memory");
+  }
+  launch_node_kept_flag(c.nd_pflag, (uint32_t)nc, kept, s);
+  scan_u32(kept, rkept, nc, tmp, s);
+  scan_u32(c.nd_esrc, rsrc, n, tmp, s);
+  scan_u32(c.nd_esnk, rsnk, n, tmp, s);
+  launch_node_tables(a, (uint32_t)nc, kept, rkept, c.nd_ce_a, c.nd_ce_b, c.nd_ce_k, rsrc, rsnk, (uint32_t)hc[2],
+                     c.nd_cel_node, c.nd_cel_gap, c.nd_cel_side, s);
+  hipEventRecord(h->ev[1], s);
+  if (hipGetLastError() != hipSuccess || sync_stream(s, "node tables") != hipSuccess) {
+    free_ranks();
+    return fail_out(NF_E_HIP, "node network: table phase failed");
+  }
+  free_ranks();
+  c.nd_n_edges = hc[0];
+  c.nd_n_elec = n_elec;
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
   release();
@@ -224,26 +267,14 @@ extern "C" int nf_get_node_edges(nf_handle* h, uint32_t* na, uint32_t* nb, uint3
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   const ContactBuffers& c = h->cb;
   if (!c.nd_valid) return fail(h, NF_E_STATE, "no node network built");
-  const uint64_t nc = c.n_contacts;
-  std::vector<uint32_t> flag(nc), ea(nc), eb(nc);
-  if (nc) {
-    HIPCHK(h, hipMemcpy(flag.data(), c.nd_pflag, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost));
-    HIPCHK(h, hipMemcpy(ea.data(), c.nd_edge_a, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost));
-    HIPCHK(h, hipMemcpy(eb.data(), c.nd_edge_b, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost));
-  }
-  uint64_t m = 0;
-  for (uint64_t k = 0; k < nc; ++k) m += flag[k] == NODE_KEPT ? 1u : 0u;
+  const uint64_t m = c.nd_n_edges;
   if (count) *count = m;
   if (!na && !nb && !contact) return NF_OK;
   if (cap < m) return fail(h, NF_E_CAPACITY, "edge buffer too small");
-  uint64_t e = 0;
-  for (uint64_t k = 0; k < nc; ++k) {
-    if (flag[k] != NODE_KEPT) continue;
-    if (na) na[e] = ea[k];
-    if (nb) nb[e] = eb[k];
-    if (contact) contact[e] = (uint32_t)k;
-    ++e;
-  }
+  if (m == 0) return NF_OK;
+  if (na) HIPCHK(h, hipMemcpy(na, c.nd_ce_a, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+  if (nb) HIPCHK(h, hipMemcpy(nb, c.nd_ce_b, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+  if (contact) HIPCHK(h, hipMemcpy(contact, c.nd_ce_k, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
   return NF_OK;
 }
 
@@ -252,30 +283,13 @@ extern "C" int nf_get_node_electrodes(nf_handle* h, uint32_t* node, double* gap,
   if (!h) return fail(nullptr, NF_E_INVALID, "null handle");
   const ContactBuffers& c = h->cb;
   if (!c.nd_valid) return fail(h, NF_E_STATE, "no node network built");
-  const uint32_t n = c.n_rods;
-  std::vector<uint32_t> es(n), ek(n), ns(n), nk(n);
-  std::vector<double> gs(n), gk(n);
-  if (n) {
-    HIPCHK(h, hipMemcpy(es.data(), c.nd_esrc, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-    HIPCHK(h, hipMemcpy(ek.data(), c.nd_esnk, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-    HIPCHK(h, hipMemcpy(ns.data(), c.nd_src_node, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-    HIPCHK(h, hipMemcpy(nk.data(), c.nd_snk_node, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-    HIPCHK(h, hipMemcpy(gs.data(), c.nd_egs, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(h, hipMemcpy(gk.data(), c.nd_egk, sizeof(double) * n, hipMemcpyDeviceToHost));
-  }
-  uint64_t m = 0;
-  for (uint32_t i = 0; i < n; ++i) m += es[i] + ek[i];
+  const uint64_t m = c.nd_n_elec;
   if (count) *count = m;
   if (!node && !gap && !side) return NF_OK;
   if (cap < m) return fail(h, NF_E_CAPACITY, "electrode buffer too small");
-  uint64_t e = 0;
-  for (int sd = 0; sd < 2; ++sd)
-    for (uint32_t i = 0; i < n; ++i) {
-      if (!(sd == 0 ? es[i] : ek[i])) continue;
-      if (node) node[e] = sd == 0 ? ns[i] : nk[i];
-      if (gap) gap[e] = sd == 0 ? gs[i] : gk[i];
-      if (side) side[e] = sd;
-      ++e;
-    }
+  if (m == 0) return NF_OK;
+  if (node) HIPCHK(h, hipMemcpy(node, c.nd_cel_node, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+  if (gap) HIPCHK(h, hipMemcpy(gap, c.nd_cel_gap, sizeof(double) * m, hipMemcpyDeviceToHost));
+  if (side) HIPCHK(h, hipMemcpy(side, c.nd_cel_side, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
   return NF_OK;
 }

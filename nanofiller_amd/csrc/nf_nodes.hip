@@ -325,6 +325,61 @@ __global__ void __launch_bounds__(NT) k_nd_count(NodeArgs a, uint32_t nc,
   }
 }
 
+// Compact tables for the getters, so that they copy only what they return:
+// the kept contacts' edges in contact order, and the electrode points,
+// source side then sink side, each in rod order.  Ranks come from scans of
+// the flags.
+__global__ void __launch_bounds__(NT) k_nd_kept_flag(const uint32_t* __restrict__ pflag, uint32_t nc,
+                                                     uint32_t* __restrict__ f) {
+  for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < nc; k += gridDim.x * NT) f[k] = pflag[k] == NODE_KEPT ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(NT) k_nd_edges_out(NodeArgs a, uint32_t nc, const uint32_t* __restrict__ f,
+                                                     const uint32_t* __restrict__ rank, uint32_t* __restrict__ oa,
+                                                     uint32_t* __restrict__ ob, uint32_t* __restrict__ ok) {
+  for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < nc; k += gridDim.x * NT) {
+    if (!f[k]) continue;
+    const uint32_t j = rank[k];
+    oa[j] = a.edge_a[k];
+    ob[j] = a.edge_b[k];
+    ok[j] = k;
+  }
+}
+
+__global__ void __launch_bounds__(NT) k_nd_elec_out(NodeArgs a, const uint32_t* __restrict__ rank_src,
+                                                    const uint32_t* __restrict__ rank_snk, uint32_t n_src,
+                                                    uint32_t* __restrict__ onode, double* __restrict__ ogap,
+                                                    int32_t* __restrict__ oside) {
+  for (uint32_t i = blockIdx.x * NT + threadIdx.x; i < a.n; i += gridDim.x * NT) {
+    if (a.esrc[i]) {
+      const uint32_t j = rank_src[i];
+      onode[j] = a.src_node[i];
+      ogap[j] = a.egs[i];
+      oside[j] = 0;
+    }
+    if (a.esnk[i]) {
+      const uint32_t j = n_src + rank_snk[i];
+      onode[j] = a.snk_node[i];
+      ogap[j] = a.egk[i];
+      oside[j] = 1;
+    }
+  }
+}
+
+void launch_node_kept_flag(const uint32_t* pflag, uint32_t nc, uint32_t* f, hipStream_t s) {
+  if (nc) hipLaunchKernelGGL(k_nd_kept_flag, dim3(grid_n(nc)), dim3(NT), 0, s, pflag, nc, f);
+}
+
+void launch_node_tables(const NodeArgs& a, uint32_t nc, const uint32_t* kept, const uint32_t* rank_kept,
+                        uint32_t* oa, uint32_t* ob, uint32_t* ok, const uint32_t* rank_src,
+                        const uint32_t* rank_snk, uint32_t n_src, uint32_t* onode, double* ogap, int32_t* oside,
+                        hipStream_t s) {
+  if (nc) hipLaunchKernelGGL(k_nd_edges_out, dim3(grid_n(nc)), dim3(NT), 0, s, a, nc, kept, rank_kept, oa, ob, ok);
+  if (a.n)
+    hipLaunchKernelGGL(k_nd_elec_out, dim3(grid_n(a.n)), dim3(NT), 0, s, a, rank_src, rank_snk, n_src, onode, ogap,
+                       oside);
+}
+
 void launch_node_count(const NodeArgs& a, uint32_t nc, unsigned long long* out, hipStream_t s) {
   hipLaunchKernelGGL(k_nd_count, dim3(grid_n(nc > a.n ? nc : a.n)), dim3(NT), 0, s, a, nc, out);
 }
