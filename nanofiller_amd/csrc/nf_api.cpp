@@ -440,7 +440,7 @@ static void release_gen(nf_handle* h) {
 
 void nf::release_contacts(nf_handle* h) {
   ContactBuffers& c = h->cb;
-  void* ptrs[] = {c.cP, c.cD, c.cid, c.cstart, c.ccount, c.chl, c.ccur, c.stencil, c.cnt,
+  void* ptrs[] = {c.cP, c.cD, c.cid, c.cstart, c.ccount, c.chl, c.cmask, c.stencil, c.cnt,
                   c.rowptr, c.fill, c.J, c.gap, c.gap64, c.img, c.area, c.labels, c.nrow, c.ncol, c.nval,
                   c.nd_pflag, c.nd_edge_a, c.nd_edge_b, c.nd_esrc, c.nd_esnk, c.nd_src_node,
                   c.nd_snk_node, c.nd_egs, c.nd_egk, c.nd_node_rod, c.nd_node_t, c.nd_seg_a,

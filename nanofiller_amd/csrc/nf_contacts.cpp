@@ -48,6 +48,45 @@ static double contact_reach(const ContactBuffers& c) {
 // Centre cell list of the contact rods on cells of about `edge`, with the
 // stencil of every cell within `reach` of a home block of bx cells along x
 // (half, bx = 1: each cell pair once).
+// Scratch of bin_rods, given back by the caller once the stream has
+// synchronized.
+struct BinScratch {
+  uint32_t* M = nullptr;
+  uint32_t* tmp = nullptr;
+  float4* tP = nullptr;
+  float4* tD = nullptr;
+  uint32_t* tid = nullptr;
+};
+
+static void free_bin_scratch(nf_handle* h, BinScratch* b) {
+  dfree(h, b->M);
+  dfree(h, b->tmp);
+  dfree(h, b->tP);
+  dfree(h, b->tD);
+  dfree(h, b->tid);
+  *b = BinScratch{};
+}
+
+// Cell list of n rods on the stream (launch_bin_rods): rods in cell order
+// and each cell's start (ncells + 1), count and largest half length.
+static int bin_rods(nf_handle* h, const float4* P, const float4* D, uint32_t n, const int32_t nc[3],
+                    const double scale[3], float4* cP, float4* cD, uint32_t* cid, uint32_t* cstart,
+                    uint32_t* ccount, float* chl, BinScratch* b) {
+  const uint64_t ncells = (uint64_t)nc[0] * nc[1] * nc[2];
+  const BinPlan bp = bin_plan(n, ncells);
+  const uint64_t n1 = std::max<uint64_t>(n, 1);
+  int rc = NF_OK;
+  if (rc == NF_OK) rc = dalloc(h, &b->M, bp.m_words + 1);
+  if (rc == NF_OK) rc = dalloc(h, &b->tmp, scan_tmp_words(bp.m_words));
+  if (rc == NF_OK) rc = dalloc(h, &b->tP, n1);
+  if (rc == NF_OK) rc = dalloc(h, &b->tD, n1);
+  if (rc == NF_OK) rc = dalloc(h, &b->tid, n1);
+  if (rc != NF_OK) return rc;
+  launch_bin_rods(bp, P, D, n, nc, scale, b->M, b->tmp, b->tP, b->tD, b->tid, cP, cD, cid, cstart, ccount, chl,
+                  h->stream);
+  return NF_OK;
+}
+
 static int build_cell_list(nf_handle* h, double edge, double reach, bool half, int bx = 1) {
   ContactBuffers& c = h->cb;
   const nf_params& p = h->params[h->post_inst];
@@ -63,13 +102,13 @@ static int build_cell_list(nf_handle* h, double edge, double reach, bool half, i
     if (nn > 1024.0) nn = 1024.0;
     c.ncell[a] = (int32_t)nn;
     scale[a] = (double)c.ncell[a] / p.box[a];
+    c.cscale[a] = scale[a];
     ncells *= (uint64_t)c.ncell[a];
   }
   if (ncells > (1ull << 26)) return fail(h, NF_E_INVALID, "contact grid too large");
   c.ncells = (uint32_t)ncells;
   std::vector<int4> st = build_stencil(c.ncell, p.box, reach, half, bx);
   c.st_n = (uint32_t)st.size();
-  uint32_t* tmp = nullptr;
   int rc = NF_OK;
 #define ALLOC(ptr, cnt) \
   if (rc == NF_OK) rc = dalloc(h, &(ptr), (cnt));
@@ -79,23 +118,18 @@ static int build_cell_list(nf_handle* h, double edge, double reach, bool half, i
   ALLOC(c.cstart, ncells + 1);
   ALLOC(c.ccount, ncells);
   ALLOC(c.chl, ncells);
-  ALLOC(c.ccur, ncells);
   ALLOC(c.stencil, st.size());
-  ALLOC(tmp, scan_tmp_words(ncells));
 #undef ALLOC
+  if (rc != NF_OK) return rc;
+  HIPCHK(h, hipMemcpyAsync(c.stencil, st.data(), sizeof(int4) * st.size(), hipMemcpyHostToDevice, s));
+  BinScratch bs;
+  rc = bin_rods(h, P, D, n, c.ncell, scale, c.cP, c.cD, c.cid, c.cstart, c.ccount, c.chl, &bs);
   if (rc != NF_OK) {
-    dfree(h, tmp);
+    free_bin_scratch(h, &bs);
     return rc;
   }
-  HIPCHK(h, hipMemcpyAsync(c.stencil, st.data(), sizeof(int4) * st.size(), hipMemcpyHostToDevice, s));
-  fill_u32(c.ccount, 0u, ncells, s);
-  fill_u32((uint32_t*)c.chl, 0u, ncells, s);
-  fill_u32(c.ccur, 0u, ncells, s);
-  launch_cell_count(P, n, c.ncell, scale, c.ccount, c.chl, s);
-  scan_u32(c.ccount, c.cstart, ncells, tmp, s);
-  launch_cell_scatter(P, D, n, c.ncell, scale, c.cstart, c.ccur, c.cP, c.cD, c.cid, s);
   const hipError_t e = sync_stream(s, "contact cell list");
-  dfree(h, tmp);
+  free_bin_scratch(h, &bs);
   if (e != hipSuccess) return fail(h, NF_E_HIP, std::string("contact cell list: ") + hipGetErrorString(e));
   return NF_OK;
 }
@@ -107,10 +141,10 @@ int build_contact_cells(nf_handle* h) {
   if (c.cP && c.cell_list == 1) return NF_OK;
   if (c.cP) {
     // the footprint engine's list (home-cell grid, run stencil): rebuild
-    void* ptrs[] = {c.cP, c.cD, c.cid, c.cstart, c.ccount, c.chl, c.ccur, c.stencil};
+    void* ptrs[] = {c.cP, c.cD, c.cid, c.cstart, c.ccount, c.chl, c.cmask, c.stencil};
     for (void* q : ptrs) dfree(h, q);
     c.cP = c.cD = nullptr;
-    c.cid = c.cstart = c.ccount = c.ccur = nullptr;
+    c.cid = c.cstart = c.ccount = c.cmask = nullptr;
     c.chl = nullptr;
     c.stencil = nullptr;
     c.cell_list = 0;
@@ -290,6 +324,14 @@ static int contacts_pass(nf_handle* h, double gap, uint64_t* n_out, int prec, bo
     xa.ctr = ctr;
     xa.next = queues;  // eight counters, QSTRIDE words apart
     xa.overflow = (uint32_t*)(ctr + 6);
+    // each rod's crossing of its neighbour cells, once per rod
+    rc = dalloc(h, &c.cmask, std::max<uint64_t>(n, 1));
+    if (rc != NF_OK) {
+      free_scratch();
+      return rc;
+    }
+    launch_seg_marks(c.cP, c.cD, n, c.ncell, c.cscale, xa.g.h, xa.g.r, 4.0f * xa.x.ew, c.cmask, s);
+    xa.cmask = c.cmask;
   }
   hipEventRecord(ev[1], s);
   // one pair pass appends every contact to an unsorted list.  Append
@@ -635,7 +677,7 @@ extern "C" int nf_centre_distance(nf_handle* h, int32_t bins, double width, int6
   for (int k = 0; k <= bins; ++k) e2[k] = ((double)k * width) * ((double)k * width);
   float4* cP = nullptr;
   float4* cD = nullptr;
-  uint32_t *cid = nullptr, *cstart = nullptr, *ccount = nullptr, *ccur = nullptr, *tmp = nullptr;
+  uint32_t *cid = nullptr, *cstart = nullptr, *ccount = nullptr;
   float* chl = nullptr;
   int4* dst = nullptr;
   double* de2 = nullptr;
@@ -647,15 +689,15 @@ extern "C" int nf_centre_distance(nf_handle* h, int32_t bins, double width, int6
   if (rc == NF_OK) rc = dalloc(h, &cid, n1);
   if (rc == NF_OK) rc = dalloc(h, &cstart, ncells + 1);
   if (rc == NF_OK) rc = dalloc(h, &ccount, ncells);
-  if (rc == NF_OK) rc = dalloc(h, &ccur, ncells);
   if (rc == NF_OK) rc = dalloc(h, &chl, ncells);
-  if (rc == NF_OK) rc = dalloc(h, &tmp, scan_tmp_words(ncells));
   if (rc == NF_OK) rc = dalloc(h, &dst, st.size());
   if (rc == NF_OK) rc = dalloc(h, &de2, (size_t)bins + 1);
   if (rc == NF_OK) rc = dalloc(h, &out, (size_t)bins + 3);
+  BinScratch bs;
   auto release = [&]() {
-    void* w[] = {cP, cD, cid, cstart, ccount, ccur, chl, tmp, dst, de2, out};
+    void* w[] = {cP, cD, cid, cstart, ccount, chl, dst, de2, out};
     for (void* q : w) dfree(h, q);
+    free_bin_scratch(h, &bs);
   };
   if (rc != NF_OK) {
     release();
@@ -668,13 +710,13 @@ extern "C" int nf_centre_distance(nf_handle* h, int32_t bins, double width, int6
     release();
     return fail(h, NF_E_HIP, std::string("statistics upload: ") + hipGetErrorString(e));
   }
-  fill_u32(ccount, 0u, ncells, s);
-  fill_u32((uint32_t*)chl, 0u, ncells, s);
-  fill_u32(ccur, 0u, ncells, s);
   fill_u32((uint32_t*)out, 0u, 2 * ((uint64_t)bins + 3), s);
-  launch_cell_count(P, n, nc, scale, ccount, chl, s);
-  scan_u32(ccount, cstart, ncells, tmp, s);
-  launch_cell_scatter(P, D, n, nc, scale, cstart, ccur, cP, cD, cid, s);
+  rc = bin_rods(h, P, D, n, nc, scale, cP, cD, cid, cstart, ccount, chl, &bs);
+  if (rc != NF_OK) {
+    (void)sync_stream(s, "centre distance");
+    release();
+    return rc;
+  }
   const double Lmax = std::max(p.box[0], std::max(p.box[1], p.box[2]));
   const float ew = (float)(5.9604644775390625e-8 * (Lmax + rmax) * 1.01);
   const float rf = (float)rmax + 8.0f * ew;

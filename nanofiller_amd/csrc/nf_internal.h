@@ -60,6 +60,24 @@ struct RoundInfo {
 
 // Wave reductions for the end of a kernel (every lane converged): sum and
 // max over the 64 lanes, result in every lane.
+// Cell of a centre in a cell list binned with nc cells and scale = nc / box
+// per axis (FP64 product, truncated, clamped to the grid): the binning and
+// every pass that needs a listed rod's cell use this one function.
+__device__ __forceinline__ void cell_coords(const float4 P, const int32_t nc[3], const double sc[3], int c[3]) {
+  const float p[3] = {P.x, P.y, P.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int i = (int)((double)p[a] * sc[a]);
+    c[a] = i < 0 ? 0 : (i >= nc[a] ? nc[a] - 1 : i);
+  }
+}
+
+__device__ __forceinline__ uint32_t cell_index(const float4 P, const int32_t nc[3], const double sc[3]) {
+  int c[3];
+  cell_coords(P, nc, sc, c);
+  return (uint32_t)(c[0] + nc[0] * (c[1] + nc[1] * c[2]));
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
@@ -151,7 +169,8 @@ struct ContactBuffers {
   uint32_t* cstart = nullptr;  // ncells + 1
   uint32_t* ccount = nullptr;
   float* chl = nullptr;
-  uint32_t* ccur = nullptr;
+  uint32_t* cmask = nullptr;  // footprint engine: near mask of each rod of the list
+  double cscale[3] = {0.0, 0.0, 0.0};  // cells per unit length (the list's binning)
   int4* stencil = nullptr;
   uint32_t st_n = 0;
   int32_t ncell[3] = {0, 0, 0};
@@ -262,6 +281,7 @@ struct SegCrossArgs {
   const uint32_t* cstart;
   const uint32_t* ccount;
   const float* chl;
+  const uint32_t* cmask;  // near mask of each rod of the list (seg_near_mask)
   const int4* stencil;  // stencil of a SEG_ROW-cell home run, sorted by minimum distance
   uint32_t st_n;
   float reach2;  // (hl_max + rpad)^2: the stencil walk stops past it
@@ -307,18 +327,34 @@ int pair_split_min(int mode);
 
 // ---- launchers (nf_seg.hip) ----
 void launch_seg_cross(int prec, const SegCrossArgs& a, hipStream_t s);
+// Near masks (seg_near_mask) of the n rods of a cell list binned with
+// (nc, scale); h and r are the footprint grid's cell edges and radius.
+void launch_seg_marks(const float4* cP, const float4* cD, uint32_t n, const int32_t nc[3], const double scale[3],
+                      const float h[3], float r, float mx, uint32_t* cmask, hipStream_t s);
 
 // ---- launchers (nf_post.hip) ----
-void launch_cell_count(const float4* P, uint32_t n, const int32_t ncell[3], const double scale[3],
-                       uint32_t* count, float* hl, hipStream_t s);
+// Cell list of n rods (two-level counting sort, nf_post.hip): bin_plan
+// sizes the scratch (M: m_words + 1 words, scan_tmp: scan_tmp_words(m_words),
+// tP/tD/tid: n each), launch_bin_rods writes the rods in cell order (oP, oD,
+// oid), each cell's start (ncells + 1 entries), count and largest half
+// length.
+struct BinPlan {
+  uint32_t shift;  // log2 of the cells per bucket
+  uint32_t nb;     // buckets
+  uint32_t nblk;   // blocks of the histogram and scatter passes
+  uint32_t chunk;  // rods per block
+  uint64_t m_words;
+};
+BinPlan bin_plan(uint32_t n, uint64_t ncells);
+void launch_bin_rods(const BinPlan& bp, const float4* P, const float4* D, uint32_t n, const int32_t ncell[3],
+                     const double scale[3], uint32_t* M, uint32_t* scan_tmp, float4* tP, float4* tD,
+                     uint32_t* tid, float4* oP, float4* oD, uint32_t* oid, uint32_t* start, uint32_t* count,
+                     float* hl, hipStream_t s);
 void launch_expand_flags(const float4* P, uint32_t n, int k, double L, double B, uint32_t* below, uint32_t* above,
                          hipStream_t s);
 void launch_expand_scatter(const float4* P, const float4* D, uint32_t n, int k, double L, double B,
                            const uint32_t* below, const uint32_t* rank_below, const uint32_t* above,
                            const uint32_t* rank_above, uint32_t n_below, float4* oP, float4* oD, hipStream_t s);
-void launch_cell_scatter(const float4* P, const float4* D, uint32_t n, const int32_t ncell[3],
-                         const double scale[3], const uint32_t* start, uint32_t* cur, float4* oP,
-                         float4* oD, uint32_t* oid, hipStream_t s);
 void launch_row_sort(const uint32_t* rowptr, uint32_t n, uint32_t* J, float* gap, double* gap64,
                      int32_t* img, hipStream_t s);
 // Contact area S (contact_area_f64) of every contact of the CSR, rod J at

@@ -4,49 +4,6 @@
 
 namespace nf {
 
-__device__ __forceinline__ uint32_t cell_index(const float4 P, const int32_t nc[3],
-                                               const double sc[3]) {
-  const float p[3] = {P.x, P.y, P.z};
-  int c[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    int i = (int)((double)p[a] * sc[a]);
-    c[a] = i < 0 ? 0 : (i >= nc[a] ? nc[a] - 1 : i);
-  }
-  return (uint32_t)(c[0] + nc[0] * (c[1] + nc[1] * c[2]));
-}
-
-struct GridArg {
-  int32_t nc[3];
-  double sc[3];
-};
-
-// Cell binning of a packing (count pass): the role of writeGlbMem's
-// per-cell placement (inc/memoryIO.h:361-429) for an existing packing.
-__global__ void k_cell_count(const float4* __restrict__ P, uint32_t n, GridArg g,
-                             uint32_t* __restrict__ count, float* __restrict__ hl) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float4 p = P[i];
-    const uint32_t c = cell_index(p, g.nc, g.sc);
-    atomicAdd(&count[c], 1u);
-    atomicMax((unsigned int*)&hl[c], __float_as_uint(p.w));
-  }
-}
-
-__global__ void k_cell_scatter(const float4* __restrict__ P, const float4* __restrict__ D, uint32_t n,
-                               GridArg g, const uint32_t* __restrict__ start,
-                               uint32_t* __restrict__ cur, float4* __restrict__ oP,
-                               float4* __restrict__ oD, uint32_t* __restrict__ oid) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float4 p = P[i];
-    const uint32_t c = cell_index(p, g.nc, g.sc);
-    const uint32_t dst = start[c] + atomicAdd(&cur[c], 1u);
-    oP[dst] = p;
-    oD[dst] = D[i];
-    oid[dst] = i;
-  }
-}
-
 static int grid_for(uint64_t n, int threads) {
   uint64_t b = (n + threads - 1) / threads;
   if (b > 8192) b = 8192;
@@ -54,26 +11,190 @@ static int grid_for(uint64_t n, int threads) {
   return (int)b;
 }
 
-void launch_cell_count(const float4* P, uint32_t n, const int32_t ncell[3], const double scale[3],
-                       uint32_t* count, float* hl, hipStream_t s) {
-  GridArg g;
-  for (int a = 0; a < 3; ++a) {
-    g.nc[a] = ncell[a];
-    g.sc[a] = scale[a];
-  }
-  hipLaunchKernelGGL(k_cell_count, dim3(grid_for(n, 256)), dim3(256), 0, s, P, n, g, count, hl);
+// ---------------------------------------------------------------------------
+// Cell list of a packing (the role of writeGlbMem's per-cell placement,
+// inc/memoryIO.h:361-429, for an existing packing) as a two-level counting
+// sort with no device-scope atomics: those resolve beyond the XCD's L2 and
+// cap a one-atomic-per-rod binning at a few tens of G/s.  Buckets are runs
+// of 2^shift consecutive cells in linear cell order, so a bucket's cells
+// own one contiguous range of the output.
+//   1. k_bin_count: each block counts its chunk of rods per bucket in LDS
+//      and writes its column of the bucket-major matrix M[bucket][block];
+//   2. scan_u32 of M gives each (bucket, block) its range in bucket order;
+//   3. k_bin_scatter: each block places its rods there (LDS cursors);
+//   4. k_bin_cells: one workgroup per bucket counts its rods per cell in
+//      LDS, scans the counts, writes the cells' start, count and largest
+//      half length, and places the rods in cell order.
+// The order of the rods inside a cell is not fixed (LDS atomics); every
+// pass reading the list is independent of it.
+// ---------------------------------------------------------------------------
+constexpr int BIN_THREADS = 1024;
+
+struct BinArg {
+  int32_t nc[3];
+  double sc[3];
+  uint32_t shift;  // log2 of the cells per bucket
+  uint32_t nb;     // buckets
+  uint32_t nblk;   // blocks of passes 1 and 3
+  uint32_t chunk;  // rods per block of passes 1 and 3
+  uint32_t ncells;
+};
+
+__global__ void __launch_bounds__(BIN_THREADS) k_bin_count(const float4* __restrict__ P, uint32_t n, BinArg g,
+                                                           uint32_t* __restrict__ M) {
+  extern __shared__ uint32_t hist[];
+  for (uint32_t b = threadIdx.x; b < g.nb; b += BIN_THREADS) hist[b] = 0u;
+  __syncthreads();
+  const uint64_t lo = (uint64_t)blockIdx.x * g.chunk;
+  const uint64_t hi = lo + g.chunk < (uint64_t)n ? lo + g.chunk : (uint64_t)n;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += BIN_THREADS)
+    atomicAdd(&hist[cell_index(P[i], g.nc, g.sc) >> g.shift], 1u);
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < g.nb; b += BIN_THREADS)
+    M[(uint64_t)b * g.nblk + blockIdx.x] = hist[b];
 }
 
-void launch_cell_scatter(const float4* P, const float4* D, uint32_t n, const int32_t ncell[3],
-                         const double scale[3], const uint32_t* start, uint32_t* cur, float4* oP,
-                         float4* oD, uint32_t* oid, hipStream_t s) {
-  GridArg g;
+__global__ void __launch_bounds__(BIN_THREADS) k_bin_scatter(const float4* __restrict__ P,
+                                                             const float4* __restrict__ D, uint32_t n,
+                                                             BinArg g, const uint32_t* __restrict__ M,
+                                                             float4* __restrict__ tP, float4* __restrict__ tD,
+                                                             uint32_t* __restrict__ tid) {
+  extern __shared__ uint32_t cur[];
+  for (uint32_t b = threadIdx.x; b < g.nb; b += BIN_THREADS) cur[b] = M[(uint64_t)b * g.nblk + blockIdx.x];
+  __syncthreads();
+  const uint64_t lo = (uint64_t)blockIdx.x * g.chunk;
+  const uint64_t hi = lo + g.chunk < (uint64_t)n ? lo + g.chunk : (uint64_t)n;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += BIN_THREADS) {
+    const float4 p = P[i];
+    const uint32_t dst = atomicAdd(&cur[cell_index(p, g.nc, g.sc) >> g.shift], 1u);
+    tP[dst] = p;
+    tD[dst] = D[i];
+    tid[dst] = (uint32_t)i;
+  }
+}
+
+__global__ void __launch_bounds__(BIN_THREADS) k_bin_cells(const float4* __restrict__ tP,
+                                                           const float4* __restrict__ tD,
+                                                           const uint32_t* __restrict__ tid, BinArg g,
+                                                           const uint32_t* __restrict__ M,
+                                                           float4* __restrict__ oP, float4* __restrict__ oD,
+                                                           uint32_t* __restrict__ oid,
+                                                           uint32_t* __restrict__ start,
+                                                           uint32_t* __restrict__ count,
+                                                           float* __restrict__ hl) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t K = 1u << g.shift;
+  uint32_t* cnt = lds;      // rods per cell
+  uint32_t* mx = lds + K;   // largest half length per cell (float bits, >= 0)
+  uint32_t* off = mx + K;   // exclusive prefix, then the placement cursor
+  __shared__ uint32_t wsum[BIN_THREADS / WAVE];
+  const uint32_t b = blockIdx.x;
+  const uint32_t base = M[(uint64_t)b * g.nblk];
+  const uint32_t end = M[(uint64_t)(b + 1) * g.nblk];
+  const uint32_t c0 = b << g.shift;
+  for (uint32_t k = threadIdx.x; k < K; k += BIN_THREADS) {
+    cnt[k] = 0u;
+    mx[k] = 0u;
+  }
+  __syncthreads();
+  for (uint32_t i = base + threadIdx.x; i < end; i += BIN_THREADS) {
+    const float4 p = tP[i];
+    const uint32_t k = cell_index(p, g.nc, g.sc) - c0;
+    atomicAdd(&cnt[k], 1u);
+    atomicMax(&mx[k], __float_as_uint(p.w));
+  }
+  __syncthreads();
+  // exclusive scan of cnt: each thread sums a run of K / BIN_THREADS cells
+  // (at least one), then a wave scan and the wave totals
+  const uint32_t per = (K + BIN_THREADS - 1) / BIN_THREADS;
+  const uint32_t k0 = threadIdx.x * per;
+  uint32_t sum = 0;
+  for (uint32_t k = k0; k < k0 + per && k < K; ++k) sum += cnt[k];
+  const int lane = threadIdx.x & (WAVE - 1);
+  uint32_t inc = sum;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o, WAVE);
+    inc += lane >= o ? t : 0u;
+  }
+  if (lane == WAVE - 1) wsum[threadIdx.x / WAVE] = inc;
+  __syncthreads();
+  uint32_t run = inc - sum;
+  for (int w = 0; w < (int)(threadIdx.x / WAVE); ++w) run += wsum[w];
+  for (uint32_t k = k0; k < k0 + per && k < K; ++k) {
+    off[k] = run;
+    run += cnt[k];
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < K; k += BIN_THREADS) {
+    const uint32_t c = c0 + k;
+    if (c < g.ncells) {
+      start[c] = base + off[k];
+      count[c] = cnt[k];
+      hl[c] = __uint_as_float(mx[k]);
+    }
+  }
+  if (b + 1 == g.nb && threadIdx.x == 0) start[g.ncells] = end;
+  __syncthreads();
+  for (uint32_t i = base + threadIdx.x; i < end; i += BIN_THREADS) {
+    const float4 p = tP[i];
+    const uint32_t dst = base + atomicAdd(&off[cell_index(p, g.nc, g.sc) - c0], 1u);
+    oP[dst] = p;
+    oD[dst] = tD[i];
+    oid[dst] = tid[i];
+  }
+}
+
+// Buckets of 2^shift cells, shift the smallest with at most BIN_BUCKETS
+// buckets (at least 32 cells, at most 4096: 48 KB of LDS per bucket
+// workgroup), and about BIN_BLOCKS blocks of at least 2048 rods for the
+// histogram and scatter passes, so that each (bucket, block) range written
+// by the scatter holds a few tens of rods.
+constexpr uint32_t BIN_BUCKETS = 1024;
+constexpr uint32_t BIN_BLOCKS = 256;
+
+BinPlan bin_plan(uint32_t n, uint64_t ncells) {
+  BinPlan bp{};
+  uint32_t shift = 5;
+  while (shift < 12 && ((ncells + (1ull << shift) - 1) >> shift) > BIN_BUCKETS) ++shift;
+  bp.shift = shift;
+  bp.nb = (uint32_t)((ncells + (1ull << shift) - 1) >> shift);
+  uint64_t chunk = ((uint64_t)n + BIN_BLOCKS - 1) / BIN_BLOCKS;
+  if (chunk < 2048) chunk = 2048;
+  chunk = (chunk + BIN_THREADS - 1) / BIN_THREADS * BIN_THREADS;
+  bp.chunk = (uint32_t)chunk;
+  bp.nblk = n == 0 ? 0u : (uint32_t)(((uint64_t)n + chunk - 1) / chunk);
+  bp.m_words = (uint64_t)bp.nb * bp.nblk;
+  return bp;
+}
+
+void launch_bin_rods(const BinPlan& bp, const float4* P, const float4* D, uint32_t n, const int32_t ncell[3],
+                     const double scale[3], uint32_t* M, uint32_t* scan_tmp, float4* tP, float4* tD,
+                     uint32_t* tid, float4* oP, float4* oD, uint32_t* oid, uint32_t* start, uint32_t* count,
+                     float* hl, hipStream_t s) {
+  BinArg g;
   for (int a = 0; a < 3; ++a) {
     g.nc[a] = ncell[a];
     g.sc[a] = scale[a];
   }
-  hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, P, D, n, g, start,
-                     cur, oP, oD, oid);
+  g.shift = bp.shift;
+  g.nb = bp.nb;
+  g.nblk = bp.nblk;
+  g.chunk = bp.chunk;
+  g.ncells = (uint32_t)((uint64_t)ncell[0] * ncell[1] * ncell[2]);
+  if (n == 0) {
+    fill_u32(start, 0u, (uint64_t)g.ncells + 1, s);
+    fill_u32(count, 0u, g.ncells, s);
+    fill_u32((uint32_t*)hl, 0u, g.ncells, s);
+    return;
+  }
+  const size_t hist_bytes = sizeof(uint32_t) * bp.nb;
+  hipLaunchKernelGGL(k_bin_count, dim3(bp.nblk), dim3(BIN_THREADS), hist_bytes, s, P, n, g, M);
+  scan_u32(M, M, bp.m_words, scan_tmp, s);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(bp.nblk), dim3(BIN_THREADS), hist_bytes, s, P, D, n, g, M, tP, tD,
+                     tid);
+  hipLaunchKernelGGL(k_bin_cells, dim3(bp.nb), dim3(BIN_THREADS), sizeof(uint32_t) * 3u * (1u << bp.shift), s,
+                     tP, tD, tid, g, M, oP, oD, oid, start, count, hl);
 }
 
 // ---------------------------------------------------------------------------

@@ -168,6 +168,47 @@ NF_HD uint64_t seg_fine_mask(const float p[3], const float c[3], float hl, const
   return m;
 }
 
+// Near mask of a rod binned in cell cc: the cells of the 3 x 3 x 3 block
+// around cc (bit (dx + 1) + 3 (dy + 1) + 9 (dz + 1), cell boundaries k h)
+// that its capsule of radius r can touch, the fine mask of that block with
+// f = 3 and e = h.  SEG_LONG instead when the capsule may leave the block
+// (a rod longer than about a cell, or a centre outside its cell): such a
+// rod is tested against each work unit (seg_row_cells).  mx covers the
+// rounding of the capsule's extent.  Depends on the rod alone, so it is
+// computed once per rod instead of once per work unit that examines it.
+constexpr uint32_t SEG_LONG = 1u << 31;
+
+NF_HD uint32_t seg_near_mask(const float p[3], const float c[3], float hl, const int cc[3], const float h[3],
+                             float r, float mx) {
+  float lo[3], e[3], ie[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = (float)(cc[k] - 1) * h[k];
+    const float hi = (float)(cc[k] + 2) * h[k];
+    const float ext = fabsf(c[k]) * hl + r + mx;
+    if (!(p[k] - ext >= lo[k] && p[k] + ext <= hi)) return SEG_LONG;
+    e[k] = h[k];
+    ie[k] = 1.0f / h[k];
+  }
+  return (uint32_t)seg_fine_mask(p, c, hl, lo, e, ie, 3, r);
+}
+
+// Home cells of a run of w cells (bit i: the run's cell i) crossed by a rod
+// with near mask m (not SEG_LONG) in the cell at offset (ox, oy, oz) from
+// the run's first cell.
+NF_HD uint32_t seg_near_cells(uint32_t m, int ox, int oy, int oz, int w) {
+  if (oy < -1 || oy > 1 || oz < -1 || oz > 1) return 0u;
+  // the row of the block at the run's y and z: dx = -1, 0, 1 of the rod's cell
+  const uint32_t row = (m >> (3 * (1 - oy) + 9 * (1 - oz))) & 7u;
+  // bit dx + 1 is the run's cell ox + dx
+  const int sh = ox - 1;
+  uint32_t cells;
+  if (sh >= 0)
+    cells = sh < 32 ? row << sh : 0u;
+  else
+    cells = -sh < 32 ? row >> -sh : 0u;
+  return cells & ((1u << w) - 1u);
+}
+
 struct SegHit {
   uint32_t I, J;  // lower and higher rod id
   int32_t img;    // J's periodic image (signed bytes x, y, z)

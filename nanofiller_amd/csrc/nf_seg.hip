@@ -15,8 +15,14 @@
 //    the next batch's loads are in flight while one is tested.  Rods whose
 //    capsule (radius r) crosses one of the run's cells are the unit's
 //    survivors (at most SMAX, kept in LDS with their periodic image and the
-//    cells they cross, seg_row_cells).  A run shares one gather among its
-//    cells: about 4x fewer rods examined than a gather per cell.
+//    cells they cross).  The test reads each rod's near mask (k_seg_marks,
+//    seg_near_mask: the cells of the 3 x 3 x 3 block around its own cell
+//    that it crosses, computed once per rod) and the listed cell's offset
+//    from the run (seg_near_cells): 4 bytes and a few bit operations per
+//    examined rod.  Only a rod that may leave its block (SEG_LONG) loads its
+//    axis for the crossing test (seg_row_cells); the survivors' rods are
+//    loaded after the walk.  A run shares one gather among its cells:
+//    about 4x fewer rods examined than a gather per cell.
 // Steps 2 and 3 run per home cell of the unit, on its survivors.
 // 2. Fine masks: each survivor's footprint on the f x f x f fine cells of the
 //    home cell (seg_fine_mask), and per fine cell a bit set of its survivors.
@@ -64,7 +70,8 @@ struct XLds {
       uint32_t q2[XQ];  // screened pairs: i | j << 8 | seg_screen code << 16
     };
     struct {
-      uint32_t gpre[WAVE], gst[WAVE], gmeta[WAVE];  // gather: prefix counts, starts, images
+      // gather: prefix counts, starts, images, offsets from the run's first cell
+      uint32_t gpre[WAVE], gst[WAVE], gmeta[WAVE], gwin[WAVE];
     };
   };
 };
@@ -164,6 +171,9 @@ __device__ __forceinline__ uint32_t x_gather(const SegCrossArgs& a, XLds& L, int
     const bool rel = sv && n > 0u && md2 <= rr * rr;
     const bool beyond = sv && md2 > a.reach2;
     const uint32_t meta = seg_pack(im[0], im[1], im[2]);
+    // the listed cell's offset from the run's first cell, clamped to what a
+    // near mask can tell apart (a rod crosses only cells within one of its own)
+    const uint32_t win = seg_pack(max(-2, min(so.x, SEG_ROW + 1)), max(-2, min(so.y, 2)), max(-2, min(so.z, 2)));
     // the block's relevant cells back to back: inclusive prefix of counts
     const uint32_t cnt = rel ? n : 0u;
     uint32_t incl = cnt;
@@ -176,13 +186,13 @@ __device__ __forceinline__ uint32_t x_gather(const SegCrossArgs& a, XLds& L, int
     L.gpre[lane] = incl - cnt;
     L.gst[lane] = st0;
     L.gmeta[lane] = meta;
+    L.gwin[lane] = win;
     __syncthreads();
     // rod pos of the packed list: the last lane whose exclusive prefix is
     // <= pos holds it (lanes without rods share the next lane's prefix).
     // An empty cell's start can be one past the last rod, so idle lanes
-    // load rod 0, which exists whenever total > 0.
-    float4 P = make_float4(0.f, 0.f, 0.f, 0.f), D = P;
-    uint32_t id = 0u, cm = 0u;
+    // load rod 0's mask, which exists whenever total > 0.
+    uint32_t src = 0u, mk = 0u, cm = 0u, wn = 0u;
     bool v = false;
     auto fetch = [&](uint32_t b0) {
       const uint32_t pos = b0 + lane;
@@ -192,30 +202,36 @@ __device__ __forceinline__ uint32_t x_gather(const SegCrossArgs& a, XLds& L, int
       for (uint32_t step = WAVE / 2; step > 0; step >>= 1)
         if (L.gpre[k + step] <= pos) k += step;
       cm = L.gmeta[k];
-      const uint32_t src = v ? L.gst[k] + (pos - L.gpre[k]) : 0u;
-      P = a.cP[src];
-      D = a.cD[src];
-      id = a.cid[src];
+      wn = L.gwin[k];
+      src = v ? L.gst[k] + (pos - L.gpre[k]) : 0u;
+      mk = a.cmask[src];
     };
     if (total > 0u) fetch(0u);
     for (uint32_t b0 = 0; b0 < total; b0 += WAVE) {
       // this batch, then the next batch's loads in flight while it is tested
-      const float4 bP = P, bD = D;
-      const uint32_t bid = id, bcm = cm;
+      const uint32_t bsrc = src, bmk = mk, bcm = cm, bwn = wn;
       const bool bv = v;
       if (b0 + WAVE < total) fetch(b0 + WAVE);
-      bytes += bv ? 36ull : 0ull;
-      const float p[3] = {bP.x + (float)ximg(bcm, 0) * a.x.boxf[0], bP.y + (float)ximg(bcm, 1) * a.x.boxf[1],
-                          bP.z + (float)ximg(bcm, 2) * a.x.boxf[2]};
-      const float c[3] = {bD.x, bD.y, bD.z};
-      const uint32_t cells = bv ? seg_row_cells(p, c, bP.w, lo, hi, xs, ws, a.g.h[0], a.g.r, mx) : 0u;
+      uint32_t cells = 0u;
+      if (bv) {
+        bytes += 4ull;
+        if ((bmk & SEG_LONG) == 0u) {
+          cells = seg_near_cells(bmk, ximg(bwn, 0), ximg(bwn, 1), ximg(bwn, 2), ws);
+        } else {
+          // a rod that may reach beyond its neighbour cells: the crossing test
+          const float4 bP = a.cP[bsrc], bD = a.cD[bsrc];
+          bytes += 32ull;
+          const float p[3] = {bP.x + (float)ximg(bcm, 0) * a.x.boxf[0], bP.y + (float)ximg(bcm, 1) * a.x.boxf[1],
+                              bP.z + (float)ximg(bcm, 2) * a.x.boxf[2]};
+          const float c[3] = {bD.x, bD.y, bD.z};
+          cells = seg_row_cells(p, c, bP.w, lo, hi, xs, ws, a.g.h[0], a.g.r, mx);
+        }
+      }
       const bool keep = cells != 0u;
       const unsigned long long b = __ballot(keep);
       const uint32_t at = S + xprefix(b);
       if (keep && at < (uint32_t)SMAX) {
-        L.sP[at] = bP;
-        L.sD[at] = bD;
-        L.sX[at] = bid;
+        L.sX[at] = bsrc;  // list position until the survivors are loaded
         L.sM[at] = bcm;
         L.sC[at] = (uint8_t)cells;
       }
@@ -223,6 +239,17 @@ __device__ __forceinline__ uint32_t x_gather(const SegCrossArgs& a, XLds& L, int
     }
     __syncthreads();
     if (__ballot(beyond) != 0ull) break;
+  }
+  // the survivors' rods, from their list positions
+  if (S <= (uint32_t)SMAX) {
+    for (uint32_t k = lane; k < S; k += WAVE) {
+      const uint32_t at = L.sX[k];
+      L.sP[k] = a.cP[at];
+      L.sD[k] = a.cD[at];
+      L.sX[k] = a.cid[at];
+    }
+    bytes += 36ull * (uint64_t)((S > lane) ? (S - lane + WAVE - 1u) / WAVE : 0u);
+    __syncthreads();
   }
   return S;
 }
@@ -413,7 +440,43 @@ __global__ void __launch_bounds__(WAVE) k_seg_cross(SegCrossArgs a) {
   }
 }
 
+struct MarkArg {
+  int32_t nc[3];
+  double sc[3];
+  float h[3];
+  float r, mx;
+};
+
+// Near mask of every rod of the cell list (seg_near_mask), in list order.
+__global__ void __launch_bounds__(256) k_seg_marks(const float4* __restrict__ cP, const float4* __restrict__ cD,
+                                                   uint32_t n, MarkArg m, uint32_t* __restrict__ cmask) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float4 P = cP[i], D = cD[i];
+    int cc[3];
+    cell_coords(P, m.nc, m.sc, cc);
+    const float p[3] = {P.x, P.y, P.z};
+    const float c[3] = {D.x, D.y, D.z};
+    cmask[i] = seg_near_mask(p, c, P.w, cc, m.h, m.r, m.mx);
+  }
+}
+
 }  // namespace
+
+void launch_seg_marks(const float4* cP, const float4* cD, uint32_t n, const int32_t nc[3], const double scale[3],
+                      const float h[3], float r, float mx, uint32_t* cmask, hipStream_t s) {
+  if (n == 0) return;
+  MarkArg m;
+  for (int k = 0; k < 3; ++k) {
+    m.nc[k] = nc[k];
+    m.sc[k] = scale[k];
+    m.h[k] = h[k];
+  }
+  m.r = r;
+  m.mx = mx;
+  uint64_t blocks = ((uint64_t)n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_seg_marks, dim3((unsigned)blocks), dim3(256), 0, s, cP, cD, n, m, cmask);
+}
 
 // Persistent grid: 10 waves per CU, what the LDS holds (15.9 KB per wave;
 // the registers would allow 12).

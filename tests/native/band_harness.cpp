@@ -78,6 +78,16 @@ uint64_t nfh_fine_mask(const float* rod, const float lo[3], const float e[3], in
   return nf::seg_fine_mask(p, c, P.w, lo, e, ie, f, r);
 }
 
+// Near mask (seg_near_mask) of a rod binned in cell cc of a grid of cell
+// edges h: the 27-bit mask, or SEG_LONG.
+uint32_t nfh_near_mask(const float* rod, const int cc[3], const float h[3], float r, float mx) {
+  float4 P, D;
+  rod_f4(rod, &P, &D);
+  const float p[3] = {P.x, P.y, P.z};
+  const float c[3] = {D.x, D.y, D.z};
+  return nf::seg_near_mask(p, c, P.w, cc, h, r, mx);
+}
+
 // Host emulation of the footprint contact pass of nf_contacts on home cells
 // of about `edge` with f fine cells per axis: every rod's capsule (radius
 // from seg_grid) is tested against every home cell it can reach, each home
@@ -88,7 +98,7 @@ uint64_t nfh_fine_mask(const float* rod, const float lo[3], const float e[3], in
 // image, gap64 (unsorted, up to cap) and returns the number of reported
 // contacts; *max_surv the largest survivor count of a home cell.
 int64_t nfh_footprint_contacts(const float* rods, uint32_t n, const double box[3], double edge, int f,
-                               double gap, int prec, int row, uint32_t* I, uint32_t* J, int32_t* img,
+                               double gap, int prec, int row, int near, uint32_t* I, uint32_t* J, int32_t* img,
                                double* gap64, int64_t cap, uint64_t* max_surv) {
   std::vector<float4> P(n), D(n);
   float hl_max = 0.0f, a_max = 0.0f;
@@ -110,6 +120,33 @@ int64_t nfh_footprint_contacts(const float* rods, uint32_t n, const double box[3
     for (int k = 0; k < 3; ++k) {
       lo_u[k] = (int)std::floor((pp[k] - hl - r) * g.inv_h[k]) - 1;
       hi_u[k] = (int)std::floor((pp[k] + hl + r) * g.inv_h[k]) + 1;
+    }
+    if (near) {
+      // the kernel's near masks: the rod's binned cell (FP64 product,
+      // truncated, clamped), the cells of its 3 x 3 x 3 block it crosses;
+      // a rod that may leave the block is tested per work unit below
+      int cc[3];
+      for (int k = 0; k < 3; ++k) {
+        const int q = (int)((double)pp[k] * ((double)g.nc[k] / box[k]));
+        cc[k] = q < 0 ? 0 : (q >= g.nc[k] ? g.nc[k] - 1 : q);
+      }
+      const float c[3] = {D[i].x, D[i].y, D[i].z};
+      const uint32_t m = nf::seg_near_mask(pp, c, hl, cc, g.h, r, 4.0f * (float)ew);
+      if (m != nf::SEG_LONG) {
+        for (int b = 0; b < 27; ++b) {
+          if (((m >> b) & 1u) == 0u) continue;
+          const int u[3] = {cc[0] + b % 3 - 1, cc[1] + (b / 3) % 3 - 1, cc[2] + b / 9 - 1};
+          int w[3], t[3];
+          for (int k = 0; k < 3; ++k) {
+            const int qd = nf::floor_div(u[k], g.nc[k]);
+            w[k] = u[k] - qd * g.nc[k];
+            t[k] = -qd;
+          }
+          cells[(size_t)w[0] + (size_t)g.nc[0] * ((size_t)w[1] + (size_t)g.nc[1] * (size_t)w[2])].push_back(
+              {i, nf::seg_pack(t[0], t[1], t[2])});
+        }
+        continue;
+      }
     }
     if (row > 1) {
       // the kernel's work units: runs of `row` cells along x, one crossing
