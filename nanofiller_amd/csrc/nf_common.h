@@ -581,6 +581,35 @@ NF_HD bool vdw_pair(const float4 P1, const float4 D1, const float4 P2, const flo
 // algorithm's worst case.
 // --------------------------------------------------------------------------
 constexpr float U32 = 5.9604644775390625e-8f;  // 2^-24
+
+// Square root and quotient where only a bound or a search point is needed.
+// The device uses the hardware instructions (v_sqrt_f32 and v_rcp_f32,
+// each within 1 ulp) instead of the correctly rounded sequences of about a
+// dozen instructions; the host uses the library functions.
+//   sqrt_up(x) >= sqrt(x) for x >= 0: a root within 1 ulp (2^-23
+//   relative) times 1 + 2^-22 (rounding to nearest can only raise the
+//   product past that); a zero or denormal x may give 0 on the device,
+//   below every margin it is added to.
+//   quot(a, b) is a / b within a few ulps: the closest-point parameters of
+//   the FP32 band, whose error bounds hold for any parameters in [-1, 1],
+//   and the slab parameters of the footprint clips, widened by 8 ulps.
+NF_HD float sqrt_up(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x) * (1.0f + 4.0f * U32);
+#else
+  return sqrtf(x) * (1.0f + 4.0f * U32);
+#endif
+}
+
+NF_HD float recip(float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(b);
+#else
+  return 1.0f / b;
+#endif
+}
+
+NF_HD float quot(float a, float b) { return a * recip(b); }
 constexpr float EPS64_REL = 1e-12f;
 
 struct F32Pair {
@@ -601,7 +630,7 @@ NF_HD F32Pair pair_f32(const float4 P1, const float4 D1, const float4 P2, const 
   const float rr2 = rr * rr;
   // reach band: w components err <= ew; rr differs from the FP64 reach by
   // < 3.5e-7 relative (5 roundings + the pad constant), so rr^2 by < 8e-7.
-  const float nw = sqrtf(w2) + 2.0f * ew;
+  const float nw = sqrt_up(w2) + 2.0f * ew;
   const float ew2 = 1.7321f * ew;
   const float e_w2 = 4.0f * U32 * w2 + (2.0f * nw + ew2) * ew2;
   const float rlo = rr2 * (1.0f - 1.5e-6f), rhi = rr2 * (1.0f + 1.5e-6f);
@@ -628,16 +657,16 @@ NF_HD F32Pair pair_f32(const float4 P1, const float4 D1, const float4 P2, const 
   const float denom = fmaf(a, e, -(b * b));
   float s = 0.0f;
   if (denom > 0.0f) {
-    s = fmaf(b, f, -(c * e)) / denom;
+    s = quot(fmaf(b, f, -(c * e)), denom);
     s = fminf(fmaxf(s, -1.0f), 1.0f);
   }
-  float t = fmaf(b, s, f) / e;
+  float t = quot(fmaf(b, s, f), e);
   if (t < -1.0f) {
     t = -1.0f;
-    s = fminf(fmaxf((-b - c) / a, -1.0f), 1.0f);
+    s = fminf(fmaxf(quot(-b - c, a), -1.0f), 1.0f);
   } else if (t > 1.0f) {
     t = 1.0f;
-    s = fminf(fmaxf((b - c) / a, -1.0f), 1.0f);
+    s = fminf(fmaxf(quot(b - c, a), -1.0f), 1.0f);
   }
   const float vx = fmaf(-t, h2x, fmaf(s, h1x, wx));
   const float vy = fmaf(-t, h2y, fmaf(s, h1y, wy));
@@ -648,7 +677,7 @@ NF_HD F32Pair pair_f32(const float4 P1, const float4 D1, const float4 P2, const 
   const float nh1 = P1.w * 1.000001f, nh2 = P2.w * 1.000001f;
   const float S = nw + nh1 + nh2;
   const float Ev = ew2 + U32 * (nh1 + nh2) + 4.1f * U32 * S;
-  const float nv = sqrtf(d2) * 1.000001f + Ev;
+  const float nv = sqrt_up(d2) * 1.000001f + Ev;
   const float EF = 4.0f * U32 * d2 + (2.0f * nv + Ev) * Ev;
   const float gs = fmaf(h1z, vz, fmaf(h1y, vy, h1x * vx));
   const float gt = -fmaf(h2z, vz, fmaf(h2y, vy, h2x * vx));
@@ -702,7 +731,7 @@ NF_HD int pair_f32_screen(const float4 P1, const float4 D1, const float4 P2, con
   const float w2 = fmaf(wz, wz, fmaf(wy, wy, wx * wx));
   const float rr = (P1.w + P2.w) * 1.000001f + R;
   const float rr2 = rr * rr;
-  const float nw = sqrtf(w2) + 2.0f * ew;
+  const float nw = sqrt_up(w2) + 2.0f * ew;
   const float ew2 = 1.7321f * ew;
   const float e_w2 = 4.0f * U32 * w2 + (2.0f * nw + ew2) * ew2;
   const float rlo = rr2 * (1.0f - 1.5e-6f), rhi = rr2 * (1.0f + 1.5e-6f);
@@ -713,7 +742,7 @@ NF_HD int pair_f32_screen(const float4 P1, const float4 D1, const float4 P2, con
   const float vz = fmaf(D1.x, D2.y, -(D1.y * D2.x));
   const float v2 = fmaf(vz, vz, fmaf(vy, vy, vx * vx));
   const float t = fmaf(wz, vz, fmaf(wy, vy, wx * vx));
-  const float nv = sqrtf(v2) * (1.0f + 4.0f * U32) + 3.5f * U32;
+  const float nv = sqrt_up(v2) * (1.0f + 4.0f * U32) + 3.5f * U32;
   const float Et = (3.5f * U32 * nw + ew2 * nv + 3.0f * U32 * nw * nv) * 1.01f;
   const float A = fabsf(t) - Et;
   const float S = nw + P1.w * 1.000001f + P2.w * 1.000001f;

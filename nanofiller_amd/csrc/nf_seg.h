@@ -77,7 +77,7 @@ NF_HD bool seg_clip(const float p[3], const float c[3], float hl, const float lo
       if (p[k] < l || p[k] > u) return false;
       continue;
     }
-    const float rc = 1.0f / c[k];
+    const float rc = recip(c[k]);
     float t0 = (l - p[k]) * rc, t1 = (u - p[k]) * rc;
     if (t0 > t1) {
       const float t = t0;
@@ -132,7 +132,7 @@ NF_HD uint64_t seg_fine_mask(const float p[3], const float c[3], float hl, const
   const int d0 = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
   const int d1 = d0 == 0 ? 1 : 0;
   const int d2 = d0 == 2 ? 1 : 2;
-  const float rc = 1.0f / c[d0];
+  const float rc = recip(c[d0]);
   uint64_t m = 0ull;
   for (int i = 0; i < f; ++i) {
     const float xa = lo[d0] + (float)i * e[d0] - r, xb = lo[d0] + (float)(i + 1) * e[d0] + r;
