@@ -133,6 +133,11 @@ NF_HD uint64_t seg_fine_mask(const float p[3], const float c[3], float hl, const
   const int d1 = d0 == 0 ? 1 : 0;
   const int d2 = d0 == 2 ? 1 : 2;
   const float rc = recip(c[d0]);
+  // bit x + f (y + f z) of fine cell (x, y, z): index strides of the slab
+  // axis and the two others (d1 is x or y, d2 is y or z)
+  const int t0 = d0 == 0 ? 1 : (d0 == 1 ? f : f * f);
+  const int t1 = d1 == 0 ? 1 : f;
+  const int t2 = d2 == 1 ? f : f * f;
   uint64_t m = 0ull;
   for (int i = 0; i < f; ++i) {
     const float xa = lo[d0] + (float)i * e[d0] - r, xb = lo[d0] + (float)(i + 1) * e[d0] + r;
@@ -156,14 +161,11 @@ NF_HD uint64_t seg_fine_mask(const float p[3], const float c[3], float hl, const
     k0 = k0 < 0 ? 0 : k0;
     j1 = j1 > f - 1 ? f - 1 : j1;
     k1 = k1 > f - 1 ? f - 1 : k1;
-    for (int j = j0; j <= j1; ++j)
-      for (int k = k0; k <= k1; ++k) {
-        int u[3];
-        u[d0] = i;
-        u[d1] = j;
-        u[d2] = k;
-        m |= 1ull << (u[0] + f * (u[1] + f * u[2]));
-      }
+    // the rectangle [j0, j1] x [k0, k1] of slab i: a row of bits along d1,
+    // shifted along d2
+    uint64_t row = 0ull;
+    for (int j = j0; j <= j1; ++j) row |= 1ull << (j * t1);
+    for (int k = k0; k <= k1; ++k) m |= row << (i * t0 + k * t2);
   }
   return m;
 }
