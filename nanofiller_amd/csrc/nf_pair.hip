@@ -359,9 +359,13 @@ __device__ void refill_cells(const PairArgs& a, const CellGeom& g, PairLds& L, S
     }
     const uint32_t nc = g.cell_base + (uint32_t)(m[0] + g.nc[0] * (m[1] + g.nc[1] * m[2]));
     uint32_t n = g.nCount[nc];
-    if (MODE == PM_GEN_OLD && !g.inv) n = min(n, g.cap);  // accepted cells hold at most cap rods
     const float hl = g.nHl[nc];
     const uint32_t st0 = g.nStart[nc];
+    // the three loads in flight together: the compiler would otherwise sink
+    // the second behind a branch on the count and the third past the loop,
+    // three dependent round trips instead of one
+    asm volatile("" ::"v"(n), "v"(hl), "v"(st0));
+    if (MODE == PM_GEN_OLD && !g.inv) n = min(n, g.cap);  // accepted cells hold at most cap rods
     const float rr_c = (chunk_hl + hl) * 1.000001f + g.R_max + CELL_MARGIN;
     const bool beyond = valid && md2 > rr_chunk2;
     const bool rel = valid && !beyond && n > 0u && md2 <= rr_c * rr_c;

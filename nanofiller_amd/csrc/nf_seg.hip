@@ -167,6 +167,9 @@ __device__ __forceinline__ uint32_t x_gather(const SegCrossArgs& a, XLds& L, int
     const uint32_t n = a.ccount[nc];
     const float hl = a.chl[nc];
     const uint32_t st0 = a.cstart[nc];
+    // the three loads in flight together (the compiler would otherwise load
+    // the longest rod only behind a branch on the count: a second round trip)
+    asm volatile("" ::"v"(n), "v"(hl), "v"(st0));
     const float rr = hl * 1.000001f + a.rpad;
     const bool rel = sv && n > 0u && md2 <= rr * rr;
     const bool beyond = sv && md2 > a.reach2;
