@@ -520,6 +520,20 @@ int nf_contacts_engine(nf_handle* h, int* engine, double* home_edge, int* fine, 
  * plain copy (out[i] = in[i]; a known-size kernel for counter calibration). */
 int nf_selftest(int which, const double* in, double* out, uint64_t n);
 
+/* Diagnostics (no counterpart in the reference): the FP32 screen and band of
+ * the pair passes, evaluated on the GPU for n pairs, so that tests can check
+ * the device's decisions (its hardware square root and reciprocal) against
+ * FP64.  ra, rb: 8 floats per rod (centre, direction, length, radius);
+ * shift: 3 doubles per pair, added to rod b's image; contact: 1 for the
+ * contact cut (a_1 + a_2 + sep_or_gap), 0 for the generation's
+ * (soft_frac (a_1 + a_2) + sep_or_gap); ew: the FP32 bound on a
+ * coordinate-difference component.  Out per pair: screen (-1 undecided,
+ * 0 out of reach, 1 far, 2 full test), decided (-1 undecided, else 1 cut,
+ * 0 not cut) and within (reach, when decided). */
+int nf_selftest_pairs(const float* ra, const float* rb, const double* shift, uint64_t n, int contact,
+                      double sep_or_gap, double soft_frac, float ew, int32_t* screen, int32_t* decided,
+                      int32_t* within);
+
 #ifdef __cplusplus
 }
 #endif

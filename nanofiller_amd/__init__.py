@@ -301,6 +301,8 @@ _SIGS = [
     ("nf_version", C.c_char_p, []),
     ("nf_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
     ("nf_selftest", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("nf_selftest_pairs", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_double,
+                                    C.c_double, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
 
 
@@ -686,6 +688,23 @@ def selftest(which: str, x: np.ndarray) -> np.ndarray:
     out = np.zeros(n, np.float64)
     _check(lib().nf_selftest(k, a.ctypes.data, out.ctypes.data, n))
     return out
+
+
+def selftest_pairs(ra: np.ndarray, rb: np.ndarray, shift: np.ndarray, contact: bool, sep_or_gap: float,
+                   ew: float, soft_frac: float = 1.0) -> dict:
+    """The FP32 screen and band of the pair passes evaluated on the GPU for
+    each pair (rods as rows of 8 floats, rod b moved by its shift)."""
+    a = np.ascontiguousarray(ra, np.float32)
+    b = np.ascontiguousarray(rb, np.float32)
+    sh = np.ascontiguousarray(shift, np.float64)
+    n = a.shape[0]
+    scr = np.zeros(n, np.int32)
+    dec = np.zeros(n, np.int32)
+    wi = np.zeros(n, np.int32)
+    _check(lib().nf_selftest_pairs(a.ctypes.data, b.ctypes.data, sh.ctypes.data, n, int(contact),
+                                   float(sep_or_gap), float(soft_frac), float(ew), scr.ctypes.data,
+                                   dec.ctypes.data, wi.ctypes.data))
+    return {"screen": scr, "decided": dec, "within": wi.astype(bool)}
 
 
 def version() -> str:

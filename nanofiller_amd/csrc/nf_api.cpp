@@ -1069,3 +1069,34 @@ extern "C" int nf_selftest(int which, const double* in, double* out, uint64_t n)
   if (e != hipSuccess) return fail(nullptr, NF_E_HIP, std::string("selftest: ") + hipGetErrorString(e));
   return NF_OK;
 }
+
+extern "C" int nf_selftest_pairs(const float* ra, const float* rb, const double* shift, uint64_t n, int contact,
+                                 double sep_or_gap, double soft_frac, float ew, int32_t* screen, int32_t* decided,
+                                 int32_t* within) {
+  if (n && (!ra || !rb || !shift || !screen || !decided || !within))
+    return fail(nullptr, NF_E_INVALID, "bad selftest_pairs args");
+  if (n == 0) return NF_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, NF_E_NODEVICE, "no HIP device");
+  const size_t fb = sizeof(float) * 8 * n, db = sizeof(double) * 3 * n, ib = sizeof(int32_t) * n;
+  char* buf = nullptr;
+  if (hipMalloc((void**)&buf, 2 * fb + db + 3 * ib) != hipSuccess) return fail(nullptr, NF_E_NOMEM, "hipMalloc failed");
+  float* da = (float*)buf;
+  float* dbr = (float*)(buf + fb);
+  double* dsh = (double*)(buf + 2 * fb);
+  int32_t* o = (int32_t*)(buf + 2 * fb + db);
+  hipError_t e = hipMemcpy(da, ra, fb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dbr, rb, fb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dsh, shift, db, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch_selftest_pairs(da, dbr, dsh, n, contact, sep_or_gap, soft_frac, ew, o, o + n, o + 2 * n, nullptr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(screen, o, ib, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(decided, o + n, ib, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(within, o + 2 * n, ib, hipMemcpyDeviceToHost);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(nullptr, NF_E_HIP, std::string("selftest_pairs: ") + hipGetErrorString(e));
+  return NF_OK;
+}

@@ -373,6 +373,9 @@ void launch_hist(const float4* P, const float4* D, uint32_t n, const uint32_t* r
                  const double* edges_t, int nb, int npb, int ntb, int axis, double L,
                  const double* mu, unsigned long long* out, hipStream_t s);
 void launch_selftest(int which, const double* in, double* out, uint64_t n, hipStream_t s);
+void launch_selftest_pairs(const float* ra, const float* rb, const double* sh, uint64_t n, int contact,
+                           double sep_or_gap, double soft_frac, float ew, int32_t* out_screen, int32_t* out_dec,
+                           int32_t* out_within, hipStream_t s);
 constexpr int CENT_BINS_MAX = 1024;
 void launch_cent(const float4* P, const uint32_t* X, const uint32_t* start, const uint32_t* count,
                  const int4* st, uint32_t st_n, const int32_t nc[3], const double box[3],

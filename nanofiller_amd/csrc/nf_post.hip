@@ -678,6 +678,38 @@ __global__ void k_selftest(int which, const double* __restrict__ in, double* __r
   }
 }
 
+// The FP32 screen and band of one pair per thread, as the pair passes run
+// them on the device: rods as 8 floats (centre, direction, length,
+// radius), the FP64 image shift of rod b, the cut distance as the passes
+// form it (contact: a_1 + a_2 + gap; generation: soft (a_1 + a_2) + sep).
+__global__ void k_selftest_pairs(const float* __restrict__ ra, const float* __restrict__ rb,
+                                 const double* __restrict__ sh, uint64_t n, int contact, double sep_or_gap,
+                                 double soft_frac, float ew, int32_t* __restrict__ out_screen,
+                                 int32_t* __restrict__ out_dec, int32_t* __restrict__ out_within) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const float* a = ra + 8 * i;
+    const float* b = rb + 8 * i;
+    const float4 P1 = make_float4(a[0], a[1], a[2], 0.5f * a[6]);
+    const float4 D1 = make_float4(a[3], a[4], a[5], a[7]);
+    const float4 P2 = make_float4(b[0], b[1], b[2], 0.5f * b[6]);
+    const float4 D2 = make_float4(b[3], b[4], b[5], b[7]);
+    const float R32 = contact ? ((D1.w + D2.w) + (float)sep_or_gap)
+                              : ((float)soft_frac * (D1.w + D2.w) + (float)sep_or_gap);
+    const float sx = (float)sh[3 * i], sy = (float)sh[3 * i + 1], sz = (float)sh[3 * i + 2];
+    out_screen[i] = pair_f32_screen(P1, D1, P2, D2, sx, sy, sz, R32, ew);
+    const F32Pair r = pair_f32(P1, D1, P2, D2, sx, sy, sz, R32, ew, !contact);
+    out_dec[i] = r.decided;
+    out_within[i] = r.within;
+  }
+}
+
+void launch_selftest_pairs(const float* ra, const float* rb, const double* sh, uint64_t n, int contact,
+                           double sep_or_gap, double soft_frac, float ew, int32_t* out_screen, int32_t* out_dec,
+                           int32_t* out_within, hipStream_t s) {
+  hipLaunchKernelGGL(k_selftest_pairs, dim3(grid_for(n, 256)), dim3(256), 0, s, ra, rb, sh, n, contact, sep_or_gap,
+                     soft_frac, ew, out_screen, out_dec, out_within);
+}
+
 void launch_selftest(int which, const double* in, double* out, uint64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_selftest, dim3(grid_for(n, 256)), dim3(256), 0, s, which, in, out, n);
 }
