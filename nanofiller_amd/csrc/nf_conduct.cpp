@@ -30,9 +30,10 @@ extern "C" int nf_conductance(nf_handle* h, double tol, uint32_t max_iter, nf_co
   double *cval = nullptr, *dinv = nullptr, *bsrc = nullptr, *bsnk = nullptr, *x = nullptr,
          *r = nullptr, *z = nullptr, *pv = nullptr, *q = nullptr, *part = nullptr, *vdev = nullptr;
   CgScal* scal = nullptr;
+  uint32_t *acto = nullptr, *rlo = nullptr, *idxo = nullptr, *offo = nullptr;
   auto release = [&]() {
-    void* ptrs[] = {flag, act, rl, idx, off, tmp, crow, ccol, cval, dinv, bsrc,
-                    bsnk, x,    r,  z,   pv,  q,   part, vdev, scal};
+    void* ptrs[] = {flag, act, rl,   idx, off, tmp,  crow, ccol, cval, dinv, bsrc, bsnk,
+                    x,    r,   z,    pv,  q,   part, vdev, scal, acto, rlo,  idxo, offo};
     for (void* p : ptrs) dfree(h, p);
   };
   const uint64_t n1 = (uint64_t)n + 1;
@@ -59,8 +60,28 @@ extern "C" int nf_conductance(nf_handle* h, double tol, uint32_t max_iter, nf_co
     launch_cs_flag(c.nrow, c.ncol, c.labels, n, flag, s);
     launch_cs_active(c.nrow, c.ncol, c.labels, flag, n, act, rl, s);
   }
-  scan_u32(act, idx, n1, tmp, s);
-  scan_u32(rl, off, n1, tmp, s);
+  // unknowns numbered in the cell list's (spatial) order when the packing's
+  // cell list is at hand, else in rod order
+  const bool spatial = c.cid != nullptr && c.cell_list != 0 && n > 0;
+  if (spatial) {
+    if (rc == NF_OK) rc = dalloc(h, &acto, n1);
+    if (rc == NF_OK) rc = dalloc(h, &rlo, n1);
+    if (rc == NF_OK) rc = dalloc(h, &idxo, n1);
+    if (rc == NF_OK) rc = dalloc(h, &offo, n1);
+    if (rc != NF_OK) {
+      release();
+      return rc;
+    }
+    fill_u32(acto + n, 0u, 1, s);
+    fill_u32(rlo + n, 0u, 1, s);
+    launch_cs_order(n, c.cid, act, rl, acto, rlo, nullptr, nullptr, nullptr, nullptr, 0, s);
+    scan_u32(acto, idxo, n1, tmp, s);
+    scan_u32(rlo, offo, n1, tmp, s);
+    launch_cs_order(n, c.cid, nullptr, nullptr, nullptr, nullptr, idxo, offo, idx, off, 1, s);
+  } else {
+    scan_u32(act, idx, n1, tmp, s);
+    scan_u32(rl, off, n1, tmp, s);
+  }
   uint32_t mn[2] = {0, 0};
   if (hipError_t e_ = hipMemcpyAsync(&mn[0], idx + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s); e_ != hipSuccess) {
       release();
@@ -73,6 +94,11 @@ extern "C" int nf_conductance(nf_handle* h, double tol, uint32_t max_iter, nf_co
   if (hipError_t e = sync_stream(s, "conductance set-up"); e != hipSuccess) {
     release();
     return fail(h, NF_E_HIP, std::string("conductance set-up: ") + hipGetErrorString(e));
+  }
+  {
+    void* q[] = {acto, rlo, idxo, offo};
+    for (void* p : q) dfree(h, p);
+    acto = rlo = idxo = offo = nullptr;
   }
   const uint32_t m = mn[0], nnz = mn[1];
   nf_conductance_info out{};

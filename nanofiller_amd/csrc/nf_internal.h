@@ -416,6 +416,12 @@ void launch_cs_fill(const uint32_t* nrow, const uint32_t* ncol, const double* nv
 void launch_cs_potential(uint32_t n, const uint32_t* labels, const uint32_t* flag,
                          const uint32_t* act, const uint32_t* idx, const double* x, double* v,
                          hipStream_t s);
+// Unknowns in the order of ord (n rods): stage 0 gathers act and rl into
+// acto and rlo; stage 1 scatters their scans (n + 1 entries) back to idx and
+// off per rod, the totals at n.
+void launch_cs_order(uint32_t n, const uint32_t* ord, const uint32_t* act, const uint32_t* rl, uint32_t* acto,
+                     uint32_t* rlo, const uint32_t* idxo, const uint32_t* offo, uint32_t* idx, uint32_t* off,
+                     int stage, hipStream_t s);
 void launch_cg_init(const CgArgs& a, hipStream_t s);
 void launch_cg_iteration(const CgArgs& a, hipStream_t s);
 void launch_cg_current(const CgArgs& a, hipStream_t s);

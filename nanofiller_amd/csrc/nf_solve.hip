@@ -110,6 +110,33 @@ __global__ void __launch_bounds__(NT) k_cs_fill(const uint32_t* __restrict__ nro
   }
 }
 
+// Unknown numbering in a spatial order: ord lists the rods by cell, so
+// rods that share contacts get nearby unknowns and the matrix-vector
+// product's gathers stay within cached lines.  Gathers the active flags
+// and row lengths into that order; the scans of them are scattered back to
+// per-rod unknown indices and row offsets.
+__global__ void __launch_bounds__(NT) k_cs_to_order(uint32_t n, const uint32_t* __restrict__ ord,
+                                                    const uint32_t* __restrict__ act,
+                                                    const uint32_t* __restrict__ rl, uint32_t* __restrict__ acto,
+                                                    uint32_t* __restrict__ rlo) {
+  for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) {
+    const uint32_t i = ord[k];
+    acto[k] = act[i];
+    rlo[k] = rl[i];
+  }
+}
+
+__global__ void __launch_bounds__(NT) k_cs_from_order(uint32_t n, const uint32_t* __restrict__ ord,
+                                                      const uint32_t* __restrict__ idxo,
+                                                      const uint32_t* __restrict__ offo,
+                                                      uint32_t* __restrict__ idx, uint32_t* __restrict__ off) {
+  for (uint32_t k = blockIdx.x * NT + threadIdx.x; k <= n; k += gridDim.x * NT) {
+    const uint32_t i = k < n ? ord[k] : n;  // the totals stay at n
+    idx[i] = idxo[k];
+    off[i] = offo[k];
+  }
+}
+
 // x = 0, r = b, z = D^-1 r, p = z; partials of r.z and b.b.
 __global__ void __launch_bounds__(NT) k_cg_init(uint32_t m, const double* __restrict__ b,
                                                 const double* __restrict__ dinv,
@@ -135,19 +162,35 @@ __global__ void __launch_bounds__(NT) k_cg_init(uint32_t m, const double* __rest
   }
 }
 
-// q = A p; partial of p.q.
+// q = A p; partial of p.q.  Four lanes per row: lane j of a row sums its
+// entries j, j + 4, ... (consecutive lanes read consecutive entries, so the
+// column and value loads coalesce across the rows of a wave), then the four
+// lane sums combine in a fixed butterfly, so the result is reproducible.
+constexpr int SPMV_LANES = 4;
+
 __global__ void __launch_bounds__(NT) k_cg_spmv(uint32_t m, const uint32_t* __restrict__ crow,
                                                 const uint32_t* __restrict__ ccol,
                                                 const double* __restrict__ cval,
                                                 const double* __restrict__ p, double* __restrict__ q,
                                                 double* __restrict__ part) {
   __shared__ double sh[NT / WAVE];
+  const uint32_t sub = threadIdx.x & (SPMV_LANES - 1);
+  const uint32_t rows_per_block = NT / SPMV_LANES;
   double pq = 0.0;
-  for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < m; k += gridDim.x * NT) {
+  // block-uniform loop over row groups, so the shuffles run with every lane
+  for (uint32_t k0 = blockIdx.x * rows_per_block; k0 < m; k0 += gridDim.x * rows_per_block) {
+    const uint32_t k = k0 + threadIdx.x / SPMV_LANES;
     double s = 0.0;
-    for (uint32_t e = crow[k]; e < crow[k + 1]; ++e) s += cval[e] * p[ccol[e]];
-    q[k] = s;
-    pq += p[k] * s;
+    if (k < m) {
+      const uint32_t e1 = crow[k + 1];
+      for (uint32_t e = crow[k] + sub; e < e1; e += SPMV_LANES) s += cval[e] * p[ccol[e]];
+    }
+    s += __shfl_xor(s, 1, WAVE);
+    s += __shfl_xor(s, 2, WAVE);
+    if (k < m && sub == 0u) {
+      q[k] = s;
+      pq += p[k] * s;
+    }
   }
   const double t = block_sum(pq, sh);
   if (threadIdx.x == 0) {
@@ -297,6 +340,16 @@ void launch_cs_potential(uint32_t n, const uint32_t* labels, const uint32_t* fla
                          hipStream_t s) {
   hipLaunchKernelGGL(k_cs_potential, dim3(grid_for(n, 8192)), dim3(NT), 0, s, n, labels, flag, act,
                      idx, x, v);
+}
+
+void launch_cs_order(uint32_t n, const uint32_t* ord, const uint32_t* act, const uint32_t* rl, uint32_t* acto,
+                     uint32_t* rlo, const uint32_t* idxo, const uint32_t* offo, uint32_t* idx, uint32_t* off,
+                     int stage, hipStream_t s) {
+  if (stage == 0)
+    hipLaunchKernelGGL(k_cs_to_order, dim3(grid_for(n, 8192)), dim3(NT), 0, s, n, ord, act, rl, acto, rlo);
+  else
+    hipLaunchKernelGGL(k_cs_from_order, dim3(grid_for((uint64_t)n + 1, 8192)), dim3(NT), 0, s, n, ord, idxo, offo,
+                       idx, off);
 }
 
 void launch_cg_init(const CgArgs& a, hipStream_t s) {
