@@ -433,6 +433,7 @@ struct NodeArgs {
   uint32_t n;
   // contacts: CSR over the lower id, J > row, image applies to J
   const uint32_t* rowptr;
+  const uint32_t* rowof;  // row of every contact (filled by launch_node_points)
   const uint32_t* J;
   const int32_t* img;
   const float4* P;  // rods by id
@@ -466,7 +467,8 @@ struct NodeArgs {
   uint32_t *seg_a, *seg_b;
   double *seg_L, *seg_A;
 };
-void launch_node_points(const NodeArgs& a, hipStream_t s);
+// Contact points and screening; rowof (one entry per contact) is filled first.
+void launch_node_points(const NodeArgs& a, uint32_t* rowof, hipStream_t s);
 void launch_node_lists(const NodeArgs& a, hipStream_t s);
 void launch_node_nodes(const NodeArgs& a, hipStream_t s);
 // out[4] (zeroed by the caller): kept contacts, screened contacts, source and sink points.

@@ -74,12 +74,12 @@ extern "C" int nf_node_network(nf_handle* h, nf_node_info* info) {
   a.cstart = c.cstart;
   a.ccount = c.ccount;
   uint32_t *pstart = nullptr, *pfill = nullptr, *npt = nullptr, *ngrp = nullptr, *nseg = nullptr,
-           *gstart = nullptr, *sstart = nullptr, *tmp = nullptr, *pt_grp = nullptr;
+           *gstart = nullptr, *sstart = nullptr, *tmp = nullptr, *pt_grp = nullptr, *rowof = nullptr;
   unsigned long long* cnt = nullptr;
   double *ptI = nullptr, *ptJ = nullptr, *pt_t = nullptr;
   unsigned long long* pt_key = nullptr;
   auto release = [&]() {
-    void* ptrs[] = {pstart, pfill, npt, ngrp, nseg, gstart, sstart, tmp, pt_grp, ptI, ptJ, pt_t, pt_key, cnt};
+    void* ptrs[] = {pstart, pfill, npt, ngrp, nseg, gstart, sstart, tmp, pt_grp, ptI, ptJ, pt_t, pt_key, cnt, rowof};
     for (void* q : ptrs) dfree(h, q);
   };
   const uint64_t n1 = (uint64_t)n + 1, ncs = std::max<uint64_t>(nc, 1);
@@ -89,6 +89,7 @@ extern "C" int nf_node_network(nf_handle* h, nf_node_info* info) {
   ALLOC(c.nd_pflag, ncs);
   ALLOC(ptI, ncs);
   ALLOC(ptJ, ncs);
+  ALLOC(rowof, ncs);
   ALLOC(c.nd_edge_a, ncs);
   ALLOC(c.nd_edge_b, ncs);
   ALLOC(npt, n1);
@@ -114,6 +115,7 @@ extern "C" int nf_node_network(nf_handle* h, nf_node_info* info) {
   if (rc != NF_OK) return fail_out(rc, "node network: out of device memory");
   a.pflag = c.nd_pflag;
   a.ptI = ptI;
+  a.rowof = rowof;
   a.ptJ = ptJ;
   a.edge_a = c.nd_edge_a;
   a.edge_b = c.nd_edge_b;
@@ -135,7 +137,7 @@ extern "C" int nf_node_network(nf_handle* h, nf_node_info* info) {
   fill_u32(pfill, 0u, n1, s);
   fill_u32(ngrp, 0u, n1, s);
   fill_u32(nseg, 0u, n1, s);
-  if (n > 0) launch_node_points(a, s);
+  if (n > 0) launch_node_points(a, rowof, s);
   scan_u32(npt, pstart, n1, tmp, s);
   uint32_t npts = 0;
   if (hipMemcpyAsync(&npts, pstart + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -49,24 +49,25 @@ __device__ __forceinline__ int img_c(int32_t code, int axis) {
 
 }  // namespace
 
-// Contact points and screening, one wave per contact (its row found by
-// binary search in the row pointers); the lanes span the rods of the cells
-// around the contact's virtual inclusion.
+// Row of every contact of the CSR (one thread per rod).
+__global__ void __launch_bounds__(NT) k_nd_rowof(NodeArgs a, uint32_t* __restrict__ rowof) {
+  for (uint32_t r = blockIdx.x * NT + threadIdx.x; r < a.n; r += gridDim.x * NT)
+    for (uint32_t k = a.rowptr[r]; k < a.rowptr[r + 1]; ++k) rowof[k] = r;
+}
+
+// Contact points and screening, one wave per contact; the lanes look up 64
+// cells around the contact's virtual inclusion at once, and the rods of the
+// cells within reach are packed into 64-lane batches (a lane finds its cell
+// by a binary search of the cells' prefix counts, held one per lane and read
+// by shuffles), so every batch's loads are in flight together.  All loop
+// bounds are wave-uniform.
 __global__ void __launch_bounds__(NT) k_nd_points(NodeArgs a) {
   const uint32_t nc = a.rowptr[a.n];
   const uint32_t lane = threadIdx.x & (WAVE - 1);
   const uint32_t w0 = (blockIdx.x * NT + threadIdx.x) / WAVE, nw = gridDim.x * NT / WAVE;
   // one wave per contact: every lane holds the same contact values
   for (uint32_t k = w0; k < nc; k += nw) {
-    uint32_t lo = 0, hi = a.n;  // rowptr[lo] <= k < rowptr[hi]
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (a.rowptr[mid] <= k)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    const uint32_t r = lo;
+    const uint32_t r = a.rowof[k];
     const float4 PI = a.P[r], DI = a.D[r];
     {
       const int32_t code = a.img[k];
@@ -98,7 +99,7 @@ __global__ void __launch_bounds__(NT) k_nd_points(NodeArgs a) {
       const double hs = sqrt_cr((hx * hx + hy * hy) + hz * hz);
       // cells around the midpoint that can hold a screening rod
       const double reach = a.hl_max + hs + av + a.a_max;
-      int c[3], kc[3];
+      int c[3], kc[3], span[3];
       const double mp[3] = {mx, my, mz};
       for (int d = 0; d < 3; ++d) {
         double v = mp[d] - floor(mp[d] / a.box[d]) * a.box[d];
@@ -106,65 +107,98 @@ __global__ void __launch_bounds__(NT) k_nd_points(NodeArgs a) {
         ci = ci < 0 ? 0 : (ci >= a.ncell[d] ? a.ncell[d] - 1 : ci);
         c[d] = ci;
         kc[d] = (int)floor(reach * a.cell_scale[d]) + 1;
+        span[d] = 2 * kc[d] + 1;
       }
       // the midpoint's own periodic image relative to the primary box
       const double base[3] = {floor(mx / a.box[0]) * a.box[0], floor(my / a.box[1]) * a.box[1],
                               floor(mz / a.box[2]) * a.box[2]};
+      // FP32 sieve radius ahead of the FP64 test
+      const float fr = (float)(hs + av);
+      const uint32_t ncand = (uint32_t)(span[0] * span[1] * span[2]);
       bool screened = false;
-      for (int oz = -kc[2]; oz <= kc[2] && !screened; ++oz)
-        for (int oy = -kc[1]; oy <= kc[1] && !screened; ++oy)
-          for (int ox = -kc[0]; ox <= kc[0] && !screened; ++ox) {
-            const int raw[3] = {c[0] + ox, c[1] + oy, c[2] + oz};
-            int m[3];
-            double sh[3];
-            double gap2 = 0.0;  // squared distance from the midpoint to the cell's box
-            for (int d = 0; d < 3; ++d) {
-              int mm = raw[d] % a.ncell[d];
-              if (mm < 0) mm += a.ncell[d];
-              const int im = (raw[d] - mm) / a.ncell[d];
-              m[d] = mm;
-              sh[d] = base[d] + (double)im * a.box[d];
-              const double e = a.box[d] / (double)a.ncell[d];
-              const double lo = (double)raw[d] * e + base[d], hi = lo + e;
-              const double dd = mp[d] < lo ? lo - mp[d] : (mp[d] > hi ? mp[d] - hi : 0.0);
-              gap2 += dd * dd;
-            }
-            // rods are binned by centre: none in a cell farther than reach
-            if (gap2 > reach * reach * 1.000001) continue;
+      for (uint32_t b0 = 0; b0 < ncand && !screened; b0 += WAVE) {
+        // ---- this lane's cell of the block: first rod, rod count, image
+        const uint32_t o = b0 + lane;
+        uint32_t cnt = 0u, st0 = 0u, imc = 0u;
+        if (o < ncand) {
+          const int off[3] = {(int)(o % (uint32_t)span[0]) - kc[0],
+                              (int)((o / (uint32_t)span[0]) % (uint32_t)span[1]) - kc[1],
+                              (int)(o / (uint32_t)(span[0] * span[1])) - kc[2]};
+          int m[3], im[3];
+          double gap2 = 0.0;  // squared distance from the midpoint to the cell's box
+          for (int d = 0; d < 3; ++d) {
+            const int raw = c[d] + off[d];
+            int mm = raw % a.ncell[d];
+            if (mm < 0) mm += a.ncell[d];
+            im[d] = (raw - mm) / a.ncell[d];
+            m[d] = mm;
+            const double e = a.box[d] / (double)a.ncell[d];
+            const double lo = (double)raw * e + base[d], hi = lo + e;
+            const double dd = mp[d] < lo ? lo - mp[d] : (mp[d] > hi ? mp[d] - hi : 0.0);
+            gap2 += dd * dd;
+          }
+          // rods are binned by centre: none in a cell farther than reach
+          if (gap2 <= reach * reach * 1.000001) {
             const uint32_t cell = (uint32_t)(m[0] + a.ncell[0] * (m[1] + a.ncell[1] * m[2]));
-            const uint32_t e = a.cstart[cell] + a.ccount[cell];
-            // FP32 sieve ahead of the FP64 centre test: the midpoint in the
-            // cell's frame, rounded once (error far below the 1e-4 slack)
+            st0 = a.cstart[cell];
+            cnt = a.ccount[cell];
+            imc = (uint32_t)(im[0] & 0xFF) | ((uint32_t)(im[1] & 0xFF) << 8) | ((uint32_t)(im[2] & 0xFF) << 16);
+          }
+        }
+        // the block's cells back to back: exclusive prefix of the counts
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int sft = 1; sft < WAVE; sft <<= 1) {
+          const uint32_t tv = (uint32_t)__shfl_up((int)incl, sft, WAVE);
+          incl += lane >= (uint32_t)sft ? tv : 0u;
+        }
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, WAVE - 1);
+        const uint32_t pre = incl - cnt;
+        for (uint32_t t0 = 0; t0 < total && !screened; t0 += WAVE) {
+          const uint32_t pos = t0 + lane;
+          // the last lane whose prefix is <= pos holds rod pos (a lane with
+          // no rods shares the next lane's prefix)
+          uint32_t kk = 0u;
+#pragma unroll
+          for (uint32_t step = WAVE / 2; step > 0; step >>= 1) {
+            const uint32_t pv = (uint32_t)__shfl((int)pre, (int)(kk + step), WAVE);
+            if (pv <= pos) kk += step;
+          }
+          const uint32_t cst = (uint32_t)__shfl((int)st0, (int)kk, WAVE);
+          const uint32_t cpre = (uint32_t)__shfl((int)pre, (int)kk, WAVE);
+          const uint32_t cim = (uint32_t)__shfl((int)imc, (int)kk, WAVE);
+          bool hit = false;
+          if (pos < total) {
+            const uint32_t q = cst + (pos - cpre);
+            const double sh[3] = {base[0] + (double)img_c((int32_t)cim, 0) * a.box[0],
+                                  base[1] + (double)img_c((int32_t)cim, 1) * a.box[1],
+                                  base[2] + (double)img_c((int32_t)cim, 2) * a.box[2]};
+            const uint32_t kq = a.cid[q];
+            const float4 PK = a.cP[q], DK = a.cD[q];
+            // FP32 sieve: the midpoint in the cell's frame, rounded once
+            // (error far below the 1e-4 slack)
             const float fx = (float)(mx - sh[0]), fy = (float)(my - sh[1]), fz = (float)(mz - sh[2]);
-            const float fr = (float)(hs + av);
-            // the lanes take the cell's rods 64 at a time
-            for (uint32_t q0 = a.cstart[cell]; q0 < e && !screened; q0 += WAVE) {
-              const uint32_t q = q0 + lane;
-              bool hit = false;
-              if (q < e) {
-                const uint32_t kk = a.cid[q];
-                const float4 PK = a.cP[q], DK = a.cD[q];
-                const float gx = PK.x - fx, gy = PK.y - fy, gz = PK.z - fz;
-                const float gl = (PK.w + fr + DK.w) * 1.0001f + 1e-4f;
-                const bool near = fmaf(gz, gz, fmaf(gy, gy, gx * gx)) <= gl * gl;
-                if (near && kk != r && kk != j) {
-                  const double R = av + (double)DK.w;
-                  // conservative centre test, then the exact segment distance
-                  const double cx = ((double)PK.x + sh[0]) - mx, cy = ((double)PK.y + sh[1]) - my,
-                               cz = ((double)PK.z + sh[2]) - mz;
-                  const double lim = ((double)PK.w + hs + R) * 1.000000001;
-                  if ((cx * cx + cy * cy) + cz * cz <= lim * lim) {
-                    const double d2 = seg_seg_d2(mx, my, mz, hx, hy, hz, (double)PK.x + sh[0],
-                                                 (double)PK.y + sh[1], (double)PK.z + sh[2],
-                                                 (double)PK.w * (double)DK.x, (double)PK.w * (double)DK.y,
-                                                 (double)PK.w * (double)DK.z);
-                    hit = d2 < R * R;
-                  }
-                }
+            const float gx = PK.x - fx, gy = PK.y - fy, gz = PK.z - fz;
+            const float gl = (PK.w + fr + DK.w) * 1.0001f + 1e-4f;
+            const bool near = fmaf(gz, gz, fmaf(gy, gy, gx * gx)) <= gl * gl;
+            if (near && kq != r && kq != j) {
+              const double R = av + (double)DK.w;
+              // conservative centre test, then the exact segment distance
+              const double cx = ((double)PK.x + sh[0]) - mx, cy = ((double)PK.y + sh[1]) - my,
+                           cz = ((double)PK.z + sh[2]) - mz;
+              const double lim = ((double)PK.w + hs + R) * 1.000000001;
+              if ((cx * cx + cy * cy) + cz * cz <= lim * lim) {
+                const double d2 = seg_seg_d2(mx, my, mz, hx, hy, hz, (double)PK.x + sh[0],
+                                             (double)PK.y + sh[1], (double)PK.z + sh[2],
+                                             (double)PK.w * (double)DK.x, (double)PK.w * (double)DK.y,
+                                             (double)PK.w * (double)DK.z);
+                hit = d2 < R * R;
               }
-              screened = __ballot(hit) != 0ull;
             }
           }
+          screened = __ballot(hit) != 0ull;
+        }
+      }
       if (lane == 0) {
         a.pflag[k] = screened ? NODE_SCREENED : NODE_KEPT;
         if (!screened) {
@@ -384,7 +418,8 @@ void launch_node_count(const NodeArgs& a, uint32_t nc, unsigned long long* out, 
   hipLaunchKernelGGL(k_nd_count, dim3(grid_n(nc > a.n ? nc : a.n)), dim3(NT), 0, s, a, nc, out);
 }
 
-void launch_node_points(const NodeArgs& a, hipStream_t s) {
+void launch_node_points(const NodeArgs& a, uint32_t* rowof, hipStream_t s) {
+  if (a.n) hipLaunchKernelGGL(k_nd_rowof, dim3(grid_n(a.n)), dim3(NT), 0, s, a, rowof);
   hipLaunchKernelGGL(k_nd_points, dim3(8192), dim3(NT), 0, s, a);  // waves stride over the contacts
   hipLaunchKernelGGL(k_nd_elec, dim3(grid_n(a.n)), dim3(NT), 0, s, a);
 }
