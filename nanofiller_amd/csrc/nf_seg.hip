@@ -10,9 +10,10 @@
 // 1. Gather, once per unit: walk the run's stencil (sorted by minimum
 //    distance, cells within hl_max + r) 64 offsets at a time; the rods of
 //    the block's relevant cells are packed back to back and staged 64 at a
-//    time (a lane finds its cell by a binary search of the block's prefix
-//    counts in LDS), so a batch is full however few rods a cell holds, and
-//    the next batch's loads are in flight while one is tested.  Rods whose
+//    time (a lane finds its cell from scalar splitters and one read of
+//    eight of the block's prefix counts in LDS), so a batch is full however
+//    few rods a cell holds, and the next batch's loads are in flight while
+//    one is tested.  Rods whose
 //    capsule (radius r) crosses one of the run's cells are the unit's
 //    survivors (at most SMAX, kept in LDS with their periodic image and the
 //    cells they cross).  The test reads each rod's near mask (k_seg_marks,
@@ -71,7 +72,8 @@ struct XLds {
     };
     struct {
       // gather: prefix counts, starts, images, offsets from the run's first cell
-      uint32_t gpre[WAVE], gst[WAVE], gmeta[WAVE], gwin[WAVE];
+      alignas(16) uint32_t gpre[WAVE];  // read eight at a time from a multiple of eight
+      uint32_t gst[WAVE], gmeta[WAVE], gwin[WAVE];
     };
   };
 };
@@ -186,13 +188,21 @@ __device__ __forceinline__ uint32_t x_gather(const SegCrossArgs& a, XLds& L, int
       incl += lane >= (uint32_t)o ? t : 0u;
     }
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, WAVE - 1);
-    L.gpre[lane] = incl - cnt;
+    const uint32_t pre = incl - cnt;
+    L.gpre[lane] = pre;
     L.gst[lane] = st0;
     L.gmeta[lane] = meta;
     L.gwin[lane] = win;
+    // every eighth prefix in scalar registers: the first level of the search
+    uint32_t spl[7];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) spl[m] = (uint32_t)__builtin_amdgcn_readlane((int)pre, 8 * (m + 1));
     __syncthreads();
     // rod pos of the packed list: the last lane whose exclusive prefix is
-    // <= pos holds it (lanes without rods share the next lane's prefix).
+    // <= pos holds it (lanes without rods share the next lane's prefix; the
+    // prefixes do not decrease).  Its group of eight lanes from the scalar
+    // splitters, then its place in the group from one read of the group's
+    // eight prefixes: two LDS round trips instead of a six-step search.
     // An empty cell's start can be one past the last rod, so idle lanes
     // load rod 0's mask, which exists whenever total > 0.
     uint32_t src = 0u, mk = 0u, cm = 0u, wn = 0u;
@@ -202,8 +212,11 @@ __device__ __forceinline__ uint32_t x_gather(const SegCrossArgs& a, XLds& L, int
       v = pos < total;
       uint32_t k = 0;
 #pragma unroll
-      for (uint32_t step = WAVE / 2; step > 0; step >>= 1)
-        if (L.gpre[k + step] <= pos) k += step;
+      for (int m = 0; m < 7; ++m) k += spl[m] <= pos ? 8u : 0u;
+      const uint4 g0 = *reinterpret_cast<const uint4*>(&L.gpre[k]);
+      const uint4 g1 = *reinterpret_cast<const uint4*>(&L.gpre[k + 4u]);
+      k += (uint32_t)(g0.y <= pos) + (uint32_t)(g0.z <= pos) + (uint32_t)(g0.w <= pos) + (uint32_t)(g1.x <= pos) +
+           (uint32_t)(g1.y <= pos) + (uint32_t)(g1.z <= pos) + (uint32_t)(g1.w <= pos);
       cm = L.gmeta[k];
       wn = L.gwin[k];
       src = v ? L.gst[k] + (pos - L.gpre[k]) : 0u;
