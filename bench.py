@@ -536,8 +536,50 @@ def cpu_baseline(threads: int):
     return {"value": main["accepted_rods_per_s"], "unit": "rods/s", "cores": threads, "kind": "port",
             "sample": ("oracle generation of 7,812 rods in a 5 um cube (the 500k-rod workload's laws "
                        "and rod density at 1/64 of its volume), median of 5; 1-thread and 20k-rod "
-                       "samples under 'runs'"),
-            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "runs": runs}
+                       "samples under 'runs'; the contact list under 'contacts'"),
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "runs": runs,
+            "contacts": cpu_contacts(threads)}
+
+
+def cpu_contacts(threads: int) -> dict:
+    """The oracle's contact list (FP64 C, exact capsule distance, OpenMP over
+    cells) on bounded samples of the 4M-rod contact workload (same laws and
+    rod density): 500,000 rods in a 25 um cube on `threads` threads (median
+    of 5) and 131,072 rods in a 16 um cube on 1 thread (median of 3).  The
+    samples are packed on the GPU; the GPU contact call on the same rods is
+    timed beside the oracle (median of 5) and the two lists are compared."""
+    import oracle
+
+    runs = {}
+    for t, edge, n, reps in ((threads, 25.0, 500000, 5), (1, 16.0, 131072, 3)):
+        p = configs.cfg4(nf.NF_PREC_FP32_RECHECK, box=(edge, edge, edge), n_rods=n)
+        with nf.Packing(p, configs.tile_options("4M")) as pk:
+            pk.generate()
+            rods = pk.rods()
+            pk.contacts(p.contact_gap)  # warm-up
+            gpu_s, kern = [], []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                c = pk.contacts(p.contact_gap)
+                gpu_s.append(time.perf_counter() - t0)
+                kern.append(sum(pk.contact_stats()["kernel_ms"].values()))
+            g64 = pk.contact_gap64(c.n)
+        cpu_s = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            o = oracle.contacts(p, rods, p.contact_gap, threads=t)
+            cpu_s.append(time.perf_counter() - t0)
+        same = (c.n == len(o["I"]) and np.array_equal(c.I, o["I"]) and np.array_equal(c.col, o["J"])
+                and np.array_equal(g64.view(np.uint64), o["gap64"].view(np.uint64)))
+        cpu, gpu = float(np.median(cpu_s)), float(np.median(gpu_s))
+        runs[f"{n}_rods_{t}t"] = {
+            "threads": t, "rods": int(rods.shape[0]), "box_um": edge, "contacts": int(c.n),
+            "cpu_s": cpu, "cpu_rods_per_s": rods.shape[0] / cpu,
+            "gpu_call_ms": gpu * 1e3, "gpu_kernel_ms": float(np.median(kern)),
+            "gpu_rods_per_s": rods.shape[0] / gpu, "identical_list": bool(same)}
+        log(f"CPU contacts, {rods.shape[0]} rods on {t} threads: {cpu:.3f} s "
+            f"(GPU call {gpu * 1e3:.2f} ms), {c.n} contacts, identical {same}")
+    return runs
 
 
 def main() -> int:
