@@ -14,6 +14,12 @@ checks, bit for bit, plus size-independent properties at full size.
 - Full-size 4M-rod packing: histogram totals, contact count = mutual-angle
   total, no overlapping pair, sampled contact gaps against the oracle's FP64
   pair evaluation.
+- Full-size 8M-rod network: CSR structure, exact symmetry, zero row sums,
+  components equal to those of the contact graph (scipy), the reported
+  counts, sampled conductances against their closed form.
+- Full-size 1M-rod conductance workload: the same network properties, then
+  the solve's potentials in [0, 1], Kirchhoff's current law at every
+  current-carrying rod, and source current = sink current = conductance.
 """
 import numpy as np
 import pytest
@@ -160,3 +166,149 @@ def test_contact_workload_full_size_properties():
         d2, _ = oracle.pair_d2(ra, rb, sh, float(ra[7]) + float(rb[7]) + p.contact_gap)
         gap = np.sqrt(d2) - (float(ra[7]) + float(rb[7]))
         assert abs(gap - g64[k]) <= 1e-12, (k, gap, g64[k])
+
+
+def _network_properties(p, n, I, J, image, gap64, info, net, samples=2000):
+    """What any correct rod network of n rods satisfies, whatever its size:
+    CSR rows of sorted unique columns holding their diagonal, exact symmetry,
+    zero row sums to rounding, couplings of the right sign, rod edges inside
+    one component, components equal to those of the edge graph, the reported
+    counts, and sampled single-contact conductances g0 exp(-max(gap, 0) /
+    lambda) (their closed form, to the accuracy of the library's exp)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    ax = p.elec_axis
+    row = net["row_ptr"].astype(np.int64)
+    col = net["col"].astype(np.int64)
+    val = net["val"]
+    lab = net["labels"]
+    nn = n + 2
+    assert info["n_nodes"] == nn and row.shape[0] == nn + 1
+    assert row[0] == 0 and row[-1] == info["nnz"] == col.shape[0]
+    assert np.all(np.diff(row) >= 1)  # every row holds its diagonal
+    r_of = np.repeat(np.arange(nn, dtype=np.int64), np.diff(row))
+    same = r_of[1:] == r_of[:-1]
+    assert np.all(col[1:][same] > col[:-1][same])
+    assert col.min() >= 0 and col.max() < nn
+    d = col == r_of
+    assert np.array_equal(np.bincount(r_of[d], minlength=nn), np.ones(nn, np.int64))
+    diag = val[d]
+    assert np.all(diag >= 0.0)
+    off = ~d
+    ro, co, vo = r_of[off], col[off], val[off]
+    assert np.all(vo < 0.0)
+    # the source and sink rows couple only to rods
+    assert not np.any((ro >= n) & (co >= n))
+    # exact symmetry
+    k1, k2 = ro * nn + co, co * nn + ro
+    o1, o2 = np.argsort(k1, kind="stable"), np.argsort(k2, kind="stable")
+    assert np.array_equal(k1[o1], k2[o2])
+    assert np.array_equal(vo[o1].view(np.uint64), vo[o2].view(np.uint64))
+    # zero row sums (the diagonal sums the same couplings in another order)
+    rs = np.add.reduceat(val, row[:-1])
+    assert np.all(np.abs(rs) <= 1e-13 * diag)
+    # edges: contacts whose image does not cross the electrode axis
+    shift = np.asarray(image, np.int64) >> (8 * ax) & 0xFF
+    kept = shift == 0
+    assert info["n_edges"] == int(kept.sum())
+    Ik, Jk = np.asarray(I, np.int64)[kept], np.asarray(J, np.int64)[kept]
+    rod = (ro < n) & (co < n)
+    pairs = np.unique(Ik * n + Jk)
+    assert int(rod.sum()) == 2 * pairs.size
+    # components: those of the rod graph, with the reported sizes
+    assert np.array_equal(lab[ro[rod]], lab[co[rod]])
+    ncomp, cc = connected_components(coo_matrix((np.ones(Ik.size), (Ik, Jk)), shape=(n, n)), directed=False)
+    assert info["n_components"] == ncomp
+    assert np.unique(lab).size == ncomp
+    assert np.unique(lab.astype(np.int64) * ncomp + cc).size == ncomp  # the same partition
+    assert info["largest"] == int(np.bincount(cc).max())
+    src = co[(ro == n)]
+    snk = co[(ro == n + 1)]
+    assert src.size == info["n_source"] and snk.size == info["n_sink"]
+    assert info["percolating"] == np.intersect1d(cc[src], cc[snk]).size
+    # sampled single contacts: g0 exp(-max(gap, 0) / lambda)
+    key = Ik * n + Jk
+    u, first, cnt = np.unique(key, return_index=True, return_counts=True)
+    single = first[cnt == 1]
+    g64k = np.asarray(gap64)[kept]
+    rng = np.random.default_rng(1)
+    for k in rng.choice(single, size=min(samples, single.size), replace=False):
+        a, b = int(Ik[k]), int(Jk[k])
+        s = row[a] + int(np.searchsorted(col[row[a]:row[a + 1]], b))
+        assert col[s] == b
+        g = p.tunnel_g0 * np.exp(-max(float(g64k[k]), 0.0) / p.tunnel_lambda)
+        assert abs(-val[s] - g) <= 1e-14 * g, (a, b, -val[s], g)
+
+
+def test_network_workload_full_size_properties():
+    """8,000,000 rods (16 agglomerates, vMF(+x, 3), electrodes on x, lambda =
+    0.1 nm): the full-size network satisfies the properties above."""
+    p = configs.cfg5()
+    with nf.Packing(p, configs.tile_options("8M")) as pk:
+        st = pk.generate()[0]
+        n = pk.rods().shape[0]
+        c = pk.contacts(p.contact_gap)
+        g64 = pk.contact_gap64(c.n)
+        info, net = pk.network()
+    assert n == 8000000 and nf.STATUS[st.status] == "target"
+    print("8M network:", info, flush=True)
+    _network_properties(p, n, c.I, c.col, c.image, g64, info, net)
+
+
+def test_conduct_workload_full_size_properties():
+    """1,000,000 rods (l 1.5 um, gap <= 5 nm, lambda 0.5 nm, electrodes on
+    x), the conductance workload: the network properties above, then the
+    solve pinned by what fixes it at any size: the potentials of the rods of
+    components touching both electrodes lie in [0, 1] (maximum principle),
+    Kirchhoff's current law holds at each of them to the solver's tolerance,
+    and the source current, the sink current and the reported conductance
+    agree (the source at potential 1, the sink at 0).  Tolerances follow
+    from the solver's stopping rule, ||r|| / ||b|| <= 1e-10."""
+    import scipy.sparse as sp
+
+    p = nf.normalize(nf.default_params(box=(20, 20, 20), n_rods=1000000, len=1.5, radius=0.005, seed=11,
+                                       contact_gap=0.005, elec_axis=0, tunnel_lambda=5e-4,
+                                       precision=nf.NF_PREC_FP32_RECHECK))
+    with nf.Packing(p) as pk:
+        pk.generate()
+        n = pk.rods().shape[0]
+        c = pk.contacts(p.contact_gap)
+        g64 = pk.contact_gap64(c.n)
+        info, net = pk.network()
+        ci, v = pk.conductance(tol=1e-10, max_iter=20000, potentials=True)
+    print("1M conductance:", info, ci, flush=True)
+    _network_properties(p, n, c.I, c.col, c.image, g64, info, net)
+    assert n == 1000000 and info["percolating"] >= 1
+    assert ci["converged"] == 1 and ci["unknowns"] > 0
+    row = net["row_ptr"].astype(np.int64)
+    col = net["col"].astype(np.int64)
+    A = sp.csr_matrix((net["val"], col, row), shape=(n + 2, n + 2))
+    lab = net["labels"].astype(np.int64)
+    # electrode couplings g (off-diagonals of the electrode rows, negated)
+    gsrc, gsnk = np.zeros(n), np.zeros(n)
+    for e, g in ((n, gsrc), (n + 1, gsnk)):
+        cs, vs = col[row[e]:row[e + 1]], net["val"][row[e]:row[e + 1]]
+        g[cs[cs < n]] = -vs[cs < n]
+    flag = np.zeros(n, np.int64)
+    np.bitwise_or.at(flag, lab[gsrc > 0], 1)
+    np.bitwise_or.at(flag, lab[gsnk > 0], 2)
+    act = flag[lab] == 3
+    assert int(act.sum()) == ci["unknowns"]
+    va = v[act]
+    assert not np.any(np.isnan(va))
+    # the solve stops on the global ||r|| / ||b|| <= 1e-10; a rod held by one
+    # weak tunnelling link (g ~ exp(-10) at a 5 nm gap) carries a potential
+    # error of its residual over that g, so the bound holds to 1e-5
+    assert va.min() >= -1e-5 and va.max() <= 1.0 + 1e-5
+    vf = np.zeros(n + 2)
+    vf[:n] = np.where(np.isnan(v), 0.0, v)
+    vf[n] = 1.0
+    r = (A @ vf)[:n][act]
+    b = gsrc[act]
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b)
+    i_src = float(np.sum(gsrc[act] * (1.0 - va)))
+    i_snk = float(np.sum(gsnk[act] * va))
+    assert i_src == pytest.approx(ci["conductance"], rel=1e-8)
+    assert i_snk == pytest.approx(ci["conductance"], rel=1e-8)
+    assert ci["current_sink"] == pytest.approx(i_snk, rel=1e-10)
