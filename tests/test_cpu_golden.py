@@ -47,3 +47,27 @@ def test_golden_fixtures_are_consistent():
         assert g["h_centre"].sum() == n and g["h_polar"].sum() == n
         assert g["h_mutual"].sum() == len(I)
         assert g["row_ptr"][-1] == meta["network"]["nnz"] == len(g["col"])
+
+
+def test_digest_helpers_and_full_size_fixture():
+    """The digests hash exactly the stated bytes, notice a one-bit change,
+    and agree with the small fixtures' arrays; the full-size fixture loads
+    with its written-out schedule bound and self-consistent counters."""
+    import hashlib
+
+    p, meta, g = G.load("iso_fp64")
+    rods = g["rods_bits"].view(np.float32).reshape(-1, 8)
+    assert G.rods_digest(rods) == hashlib.sha256(g["rods_bits"].tobytes()).hexdigest()
+    flipped = g["rods_bits"].copy()
+    flipped[3] ^= 1
+    assert G.rods_digest(flipped.view(np.float32).reshape(-1, 8)) != G.rods_digest(rods)
+    d = G.contacts_digest(g["I"], g["J"], g["image"], g["gap_bits"].view(np.float32),
+                          g["gap64_bits"].view(np.float64))
+    raw = b"".join(a.tobytes() for a in (g["I"].astype(np.uint32), g["J"].astype(np.uint32),
+                                         g["image"].astype(np.int32), g["gap_bits"], g["gap64_bits"]))
+    assert d == hashlib.sha256(raw).hexdigest()
+    p, meta = G.load_digest("headline_500k")
+    assert p.n_rods == meta["stats"]["n_accepted"] == 500000
+    assert p.max_batch == meta["params"]["max_batch"]
+    assert sum(meta["round_accepts"]) == 500000 and len(meta["round_accepts"]) == meta["stats"]["rounds"]
+    assert 0 < meta["contacts"] and len(meta["rods_sha256"]) == len(meta["contacts_sha256"]) == 64

@@ -46,3 +46,24 @@ def test_device_matches_golden(name):
     for k in ("labels", "row_ptr", "col"):
         np.testing.assert_array_equal(net[k], g[k])
     np.testing.assert_array_equal(net["val"].view(np.uint64), g["val_bits"])
+
+
+def test_headline_full_size_matches_oracle_digest():
+    """The 500,000-rod headline packing and its contact list at full size,
+    bit for bit against the oracle's: counters, and SHA-256 digests of the
+    rods and of the contact list (I, J, image, gap32 and gap64 bits), written
+    by tests/golden/make_full_size.py."""
+    p, meta = G.load_digest("headline_500k")
+    with nf.Packing(p) as pk:
+        st = pk.generate()[0]
+        rods = pk.rods()
+        c = pk.contacts(p.contact_gap)
+        g64 = pk.contact_gap64(c.n)
+    s = meta["stats"]
+    print("headline:", st.n_accepted, st.rounds, st.pair_tests, c.n, flush=True)
+    assert st.n_accepted == s["n_accepted"] == 500000
+    assert st.rounds == s["rounds"] and st.pair_tests == s["pair_tests"]
+    assert st.candidates == s["candidates"] and st.proposals == s["proposals"]
+    assert G.rods_digest(rods) == meta["rods_sha256"]
+    assert c.n == meta["contacts"]
+    assert G.contacts_digest(c.I, c.col, c.image, c.gap, g64) == meta["contacts_sha256"]
