@@ -66,8 +66,12 @@ def test_digest_helpers_and_full_size_fixture():
     raw = b"".join(a.tobytes() for a in (g["I"].astype(np.uint32), g["J"].astype(np.uint32),
                                          g["image"].astype(np.int32), g["gap_bits"], g["gap64_bits"]))
     assert d == hashlib.sha256(raw).hexdigest()
-    p, meta = G.load_digest("headline_500k")
-    assert p.n_rods == meta["stats"]["n_accepted"] == 500000
-    assert p.max_batch == meta["params"]["max_batch"]
-    assert sum(meta["round_accepts"]) == 500000 and len(meta["round_accepts"]) == meta["stats"]["rounds"]
-    assert 0 < meta["contacts"] and len(meta["rods_sha256"]) == len(meta["contacts_sha256"]) == 64
+    for name, n in (("headline_500k", 500000), ("contacts_4M", 4000000)):
+        p, meta = G.load_digest(name)
+        assert p.n_rods == meta["stats"]["n_accepted"] == n
+        assert p.max_batch == meta["params"]["max_batch"]
+        assert sum(meta["round_accepts"]) == n and len(meta["round_accepts"]) == meta["stats"]["rounds"]
+        assert 0 < meta["contacts"] and len(meta["rods_sha256"]) == len(meta["contacts_sha256"]) == 64
+        if "h_centre" in meta:
+            assert sum(meta["h_centre"]) == sum(meta["h_polar"]) == n
+            assert sum(meta["h_mutual"]) == meta["contacts"]

@@ -67,3 +67,27 @@ def test_headline_full_size_matches_oracle_digest():
     assert G.rods_digest(rods) == meta["rods_sha256"]
     assert c.n == meta["contacts"]
     assert G.contacts_digest(c.I, c.col, c.image, c.gap, g64) == meta["contacts_sha256"]
+
+
+def test_contact_workload_full_size_matches_oracle_digest():
+    """The 4,000,000-rod contact workload at full size, bit for bit against
+    the oracle's: counters, digests of the rods and of the contact list, and
+    the three 64-bin histograms about +x (tests/golden/make_full_size.py)."""
+    p, meta = G.load_digest("contacts_4M")
+    hs = meta["hist"]
+    with nf.Packing(p) as pk:
+        st = pk.generate()[0]
+        rods = pk.rods()
+        c = pk.contacts(p.contact_gap)
+        g64 = pk.contact_gap64(c.n)
+        h = pk.histograms(bins=hs["bins"], centre_axis=hs["centre_axis"], pol_axis=tuple(hs["pol_axis"]))
+    s = meta["stats"]
+    print("4M:", st.n_accepted, st.rounds, st.pair_tests, c.n, flush=True)
+    assert st.n_accepted == s["n_accepted"] == 4000000
+    assert st.rounds == s["rounds"] and st.pair_tests == s["pair_tests"]
+    assert st.candidates == s["candidates"] and st.proposals == s["proposals"]
+    assert G.rods_digest(rods) == meta["rods_sha256"]
+    assert c.n == meta["contacts"]
+    assert G.contacts_digest(c.I, c.col, c.image, c.gap, g64) == meta["contacts_sha256"]
+    for k in ("centre", "polar", "mutual"):
+        np.testing.assert_array_equal(h[k], np.asarray(meta["h_" + k], np.int64), err_msg=k)
