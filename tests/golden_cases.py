@@ -46,6 +46,17 @@ def contacts_digest(I, J, image, gap32, gap64) -> str:
     return h.hexdigest()
 
 
+def network_digest(labels, row_ptr, col, val) -> str:
+    """SHA-256 of a rod network: labels, row_ptr, col (u32) and the FP64
+    value bits, each array whole and in that order."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for a, dt in ((labels, np.uint32), (row_ptr, np.uint32), (col, np.uint32), (val, np.float64)):
+        h.update(np.ascontiguousarray(a, dt).tobytes())
+    return h.hexdigest()
+
+
 def load_digest(name):
     with open(os.path.join(GOLDEN, f"{name}_digest.json")) as f:
         meta = json.load(f)

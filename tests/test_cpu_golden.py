@@ -66,7 +66,7 @@ def test_digest_helpers_and_full_size_fixture():
     raw = b"".join(a.tobytes() for a in (g["I"].astype(np.uint32), g["J"].astype(np.uint32),
                                          g["image"].astype(np.int32), g["gap_bits"], g["gap64_bits"]))
     assert d == hashlib.sha256(raw).hexdigest()
-    for name, n in (("headline_500k", 500000), ("contacts_4M", 4000000)):
+    for name, n in (("headline_500k", 500000), ("contacts_4M", 4000000), ("network_8M", 8000000)):
         p, meta = G.load_digest(name)
         assert p.n_rods == meta["stats"]["n_accepted"] == n
         assert p.max_batch == meta["params"]["max_batch"]
@@ -75,3 +75,12 @@ def test_digest_helpers_and_full_size_fixture():
         if "h_centre" in meta:
             assert sum(meta["h_centre"]) == sum(meta["h_polar"]) == n
             assert sum(meta["h_mutual"]) == meta["contacts"]
+        if "network" in meta:
+            assert meta["network"]["n_nodes"] == n + 2 and meta["network"]["n_edges"] <= meta["contacts"]
+            assert len(meta["network_sha256"]) == 64
+    # the network digest hashes exactly the stated bytes
+    p, meta, g = G.load("iso_fp64")
+    raw = b"".join(a.tobytes() for a in (g["labels"].astype(np.uint32), g["row_ptr"].astype(np.uint32),
+                                         g["col"].astype(np.uint32), g["val_bits"]))
+    assert G.network_digest(g["labels"], g["row_ptr"], g["col"], g["val_bits"].view(np.float64)) == \
+        hashlib.sha256(raw).hexdigest()

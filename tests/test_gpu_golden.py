@@ -91,3 +91,27 @@ def test_contact_workload_full_size_matches_oracle_digest():
     assert G.contacts_digest(c.I, c.col, c.image, c.gap, g64) == meta["contacts_sha256"]
     for k in ("centre", "polar", "mutual"):
         np.testing.assert_array_equal(h[k], np.asarray(meta["h_" + k], np.int64), err_msg=k)
+
+
+def test_network_workload_full_size_matches_oracle_digest():
+    """The 8,000,000-rod network workload at full size, bit for bit against
+    the oracle's: counters, digests of the rods, the contact list and the
+    network (labels, CSR and FP64 value bits), and the network's counts
+    (tests/golden/make_full_size.py)."""
+    p, meta = G.load_digest("network_8M")
+    with nf.Packing(p) as pk:
+        st = pk.generate()[0]
+        rods = pk.rods()
+        c = pk.contacts(p.contact_gap)
+        g64 = pk.contact_gap64(c.n)
+        info, net = pk.network()
+    s = meta["stats"]
+    print("8M:", st.n_accepted, st.rounds, st.pair_tests, c.n, info, flush=True)
+    assert st.n_accepted == s["n_accepted"] == 8000000
+    assert st.rounds == s["rounds"] and st.pair_tests == s["pair_tests"]
+    assert st.candidates == s["candidates"] and st.proposals == s["proposals"]
+    assert G.rods_digest(rods) == meta["rods_sha256"]
+    assert c.n == meta["contacts"]
+    assert G.contacts_digest(c.I, c.col, c.image, c.gap, g64) == meta["contacts_sha256"]
+    assert info == meta["network"]
+    assert G.network_digest(net["labels"], net["row_ptr"], net["col"], net["val"]) == meta["network_sha256"]
