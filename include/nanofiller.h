@@ -481,8 +481,12 @@ enum {
   NF_OPT_PART_SHIFT = 12,    /* reject vs accepted: a dense cell is dealt over
                                 1 << shift work items in launches with few cells per
                                 wave (-1: default 1)                                    */
-  NF_OPT_SPLIT_MIN = 13      /* probes per work item of a dealt cell, at least
+  NF_OPT_SPLIT_MIN = 13,     /* probes per work item of a dealt cell, at least
                                 (0: default 32)                                         */
+  NF_OPT_TEST_ALL_PAIRS = 14 /* 1: the reject-vs-accepted pass tests every pair within
+                                reach, so that the pair-test count is the oracle's;
+                                0 (default): a proposal leaves its stencil walk once an
+                                accepted rod cuts it (same decisions, fewer tests)      */
 };
 /* Candidate pairs of nf_contacts.  Both give the same contact list.
  * FOOTPRINT: each rod binned into the fine cells its capsule (radius

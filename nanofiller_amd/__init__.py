@@ -34,6 +34,7 @@ NF_OPT_GRAPH, NF_OPT_PROFILE, NF_OPT_DIAG, NF_OPT_CONTACT_PREC = 1, 2, 3, 4
 NF_OPT_CONTACT_ENGINE, NF_OPT_FOOTPRINT_FINE, NF_OPT_FOOTPRINT_OCC, NF_OPT_POST_INSTANCE = 5, 6, 7, 8
 NF_OPT_SLOT_SCALE = 9
 NF_OPT_WAVES_OLD, NF_OPT_WAVES_NEW, NF_OPT_PART_SHIFT, NF_OPT_SPLIT_MIN = 10, 11, 12, 13
+NF_OPT_TEST_ALL_PAIRS = 14
 NF_ENGINE_FOOTPRINT, NF_ENGINE_CELLS = 0, 1
 KERNEL_CLASSES = ["gen", "bin", "reject_old", "reject_new", "commit", "control", "contact", "other"]
 

@@ -672,6 +672,10 @@ extern "C" int nf_set_option(nf_handle* h, int key, int64_t value) {
       h->gb.tiles.split_min = (uint32_t)value;
       drop_graph(h);
       break;
+    case NF_OPT_TEST_ALL_PAIRS:
+      h->gb.tiles.test_all = value ? 1u : 0u;
+      drop_graph(h);
+      break;
     case NF_OPT_SLOT_SCALE:
       if (value < 1 || value > 1000000) return fail(h, NF_E_INVALID, "slot scale out of range (1..1000000)");
       return scale_slots(h, (double)value * 1e-3);

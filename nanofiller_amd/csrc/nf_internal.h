@@ -109,6 +109,7 @@ struct PairTiles {
   uint32_t waves_new = 0;  // waves per CU, reject vs same-round proposals
   int32_t part_shift = -1; // work items per dense cell (reject vs accepted), as a shift
   uint32_t split_min = 0;  // probes per part of a split cell, at least
+  uint32_t test_all = 0;   // reject vs accepted: cut proposals stay in the walk (every pair within reach tested)
 };
 
 struct GenBuffers {
@@ -226,6 +227,7 @@ struct PairArgs {
   uint32_t ncells_total;
   uint32_t part_shift;  // work items per cell: 1 << part_shift (pair_part_shift)
   uint32_t split_min;   // probes per part of a split cell, at least (pair_split_min)
+  uint32_t test_all;    // reject vs accepted: test every pair within reach (NF_OPT_TEST_ALL_PAIRS)
   uint32_t* order;      // ncells_total: work order of the cells (filled by launch_pair; nullptr: none)
   const int4* stencil;
   uint32_t st_off, st_n;  // used when desc == nullptr (contacts)

@@ -539,7 +539,7 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
     L.hit[lane] = 0u;
     // probes taking part, as a wave-uniform mask; the prefilter reads each
     // probe from its lane's registers
-    const unsigned long long pmask = __ballot(have);
+    unsigned long long pmask = __ballot(have);
     if (pmask == 0ull) continue;  // every probe of the chunk already cut: nothing to stage
     const float chunk_hl = unif(wave_max(have ? P1.w : 0.0f));
     if (part == 0u) acc.bytes += (unsigned long long)G * 36ull;
@@ -649,6 +649,15 @@ __device__ void process_cell(const PairArgs& a, const CellGeom& g, uint32_t gc, 
         q2n = 0;
       }
       __syncthreads();
+      if (MODE == PM_GEN_OLD && !g.inv && !a.test_all) {
+        // a proposal once cut is rejected whatever else it overlaps: cut
+        // proposals leave the walk, which ends when none is left.  The
+        // stencil is sorted by distance, so near a jammed state most
+        // proposals leave after their first batches.  The decisions are
+        // unchanged; the pair-test count is then the pairs evaluated.
+        pmask = __ballot(have && L.hit[lane] == 0u);
+        if (pmask == 0ull) break;
+      }
     }
     if (MODE == PM_GEN_OLD && !g.inv) {
       // branch-free result store: lanes without a result write a private
@@ -763,11 +772,13 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
           cur_inst = inst;
           if (MODE == PM_GEN_OLD) {
             // accepted-major when the instance's accepted rods outnumber
-            // this round's proposals: fewer, denser probe chunks stage the
-            // sparser set (the pair set and every decision are the same)
+            // this round's proposals four to one: fewer, denser probe chunks
+            // stage the sparser set (the pair set and every decision are the
+            // same).  Below that ratio the proposals are the probes, so that
+            // cut ones can leave their walk early.
             const uint32_t nacc = uni(a.st[inst].n_acc);
             const uint32_t nprop = uni(a.prStart[d.cell_base + d.ncells]) - uni(a.prStart[d.cell_base]);
-            g.inv = nacc > nprop;
+            g.inv = nacc > 4u * nprop;
             g.idle = nprop == 0u || nacc == 0u;
             g.hP = g.inv ? a.nbP : a.prP;
             g.hD = g.inv ? a.nbD : a.prD;
@@ -882,6 +893,7 @@ void launch_pair(int mode, int prec, const PairArgs& args, hipStream_t s, const 
   else
     a.order = nullptr;
   a.split_min = (t && t->split_min) ? t->split_min : (uint32_t)pair_split_min(mode);
+  a.test_all = (t && mode == PM_GEN_OLD) ? t->test_all : 0u;
 #define NF_LAUNCH(M, P) hipLaunchKernelGGL((k_pair<M, P>), grid, block, 0, s, a)
   if (prec == NF_PREC_FP64) {
     switch (mode) {

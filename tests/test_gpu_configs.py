@@ -30,10 +30,13 @@ from nanofiller_amd import configs
 
 pytestmark = pytest.mark.gpu
 
+# generations whose pair-test count is compared test every pair within reach
+EXACT = {nf.NF_OPT_TEST_ALL_PAIRS: 1}
+
 
 def _gen_compare(p, threads=16):
     ro, so, _ = oracle.generate(p, threads=threads)
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, EXACT) as pk:
         st = pk.generate()[0]
         rg = pk.rods()
     assert st.n_accepted == so["n_accepted"]
@@ -44,7 +47,7 @@ def _gen_compare(p, threads=16):
 
 def test_batch_mix_matches_oracle():
     ps = configs.cfg3(n_inst=24, n_scale=1.0 / 40.0, max_rounds=30)
-    with nf.Packing(ps) as pk:
+    with nf.Packing(ps, EXACT) as pk:
         sts = pk.generate()
         rods = [pk.rods(i) for i in range(len(ps))]
     statuses = set()

@@ -14,10 +14,13 @@ from tests import cases
 
 pytestmark = pytest.mark.gpu
 
+# generations whose pair-test count is compared test every pair within reach
+EXACT = {nf.NF_OPT_TEST_ALL_PAIRS: 1}
+
 
 def _compare(p, threads=16, options=None):
     rods_o, st_o, _ = oracle.generate(p, threads=threads)
-    with nf.Packing(p, options) as pk:
+    with nf.Packing(p, {**EXACT, **(options or {})}) as pk:
         st = pk.generate()[0]
         rods_g = pk.rods()
     assert st.n_accepted == st_o["n_accepted"]
@@ -85,7 +88,7 @@ def test_batch_matches_single():
     ps = [cases.small_iso(nf.NF_PREC_FP32_RECHECK, seed=s, stream=i, n_rods=500 + 300 * i)
           for i, s in enumerate([3, 4, 5, 6])]
     ps.append(cases.small_cfg2(nf.NF_PREC_FP32_RECHECK, n_rods=2000, stream=9))
-    with nf.Packing(ps) as pk:
+    with nf.Packing(ps, EXACT) as pk:
         sts = pk.generate()
         rods = [pk.rods(i) for i in range(len(ps))]
     for i, p in enumerate(ps):
@@ -102,10 +105,10 @@ def test_cell_size_does_not_change_packing(cell_size):
     p = cases.small_cfg2(nf.NF_PREC_FP32_RECHECK, n_rods=3000)
     q = p.copy()
     q.cell_size = cell_size
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, EXACT) as pk:
         s0 = pk.generate()[0]
         r0 = pk.rods()
-    with nf.Packing(q) as pk:
+    with nf.Packing(q, EXACT) as pk:
         s1 = pk.generate()[0]
         r1 = pk.rods()
     np.testing.assert_array_equal(r0.view(np.uint32), r1.view(np.uint32))
@@ -121,10 +124,10 @@ def test_work_schedule_does_not_change_packing():
     p = cases.small_cfg2(nf.NF_PREC_FP32_RECHECK, n_rods=6000)
     big = nf.normalize(nf.default_params(box=(40, 40, 40), n_rods=1000, len=0.5, radius=0.005, seed=8,
                                          stream=3, precision=nf.NF_PREC_FP32_RECHECK))
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, EXACT) as pk:
         s0 = pk.generate()[0]
         r0 = pk.rods()
-    with nf.Packing([p, big]) as pk:
+    with nf.Packing([p, big], EXACT) as pk:
         s1 = pk.generate()[0]
         r1 = pk.rods(0)
     np.testing.assert_array_equal(r0.view(np.uint32), r1.view(np.uint32))
@@ -138,7 +141,7 @@ def test_long_rods_in_small_box_default_cells():
     p = nf.normalize(nf.default_params(box=(9.2, 9.2, 9.2), n_rods=2500, len=4.9, radius=0.005,
                                        seed=3, stream=247, precision=nf.NF_PREC_FP32_RECHECK))
     ro, so, _ = oracle.generate(p, threads=16)
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, EXACT) as pk:
         st = pk.generate()[0]
         rg = pk.rods()
     np.testing.assert_array_equal(rg.view(np.uint32), ro.view(np.uint32))
@@ -154,11 +157,11 @@ def test_slot_overflow_regenerates_the_same_packing(scale):
     ps = [cases.small_cfg2(nf.NF_PREC_FP32_RECHECK), cases.small_iso(nf.NF_PREC_FP32_RECHECK)]
     refs = []
     for p in ps:
-        with nf.Packing(p) as pk:
+        with nf.Packing(p, EXACT) as pk:
             st = pk.generate()[0]
             refs.append((st, pk.rods()))
         assert st.slot_growths == 0
-    with nf.Packing(ps[0], {nf.NF_OPT_SLOT_SCALE: scale}) as pk:
+    with nf.Packing(ps[0], {**EXACT, nf.NF_OPT_SLOT_SCALE: scale}) as pk:
         st2 = pk.generate()[0]
         rods = pk.rods()
         st3 = pk.generate()[0]
@@ -183,8 +186,43 @@ def test_tile_shape_does_not_change_packing(tiles):
     changes speed only: the same packing and counters as the oracle."""
     p = cases.small_cfg2(nf.NF_PREC_FP32_RECHECK)
     ro, so, _ = oracle.generate(p, threads=16)
-    with nf.Packing(p, tiles) as pk:
+    with nf.Packing(p, {**EXACT, **tiles}) as pk:
         st = pk.generate()[0]
         rods = pk.rods()
     assert st.n_accepted == so["n_accepted"] and st.pair_tests == so["pair_tests"]
     np.testing.assert_array_equal(rods.view(np.uint32), ro.view(np.uint32))
+
+
+@pytest.mark.parametrize("case", ["small_cfg2_fp32", "cfg1_fp64", "jamming_batch"])
+def test_cut_proposals_leave_the_walk_with_the_same_packing(case):
+    """By default a proposal leaves its stencil walk once an accepted rod
+    cuts it; with NF_OPT_TEST_ALL_PAIRS every pair within reach is tested.
+    Both give the oracle's packing and counters, except the pair-test count,
+    which the default can only lower.  The jamming batch (the 256-packing
+    mix scaled down, 30-round cap) is where most proposals are cut."""
+    from nanofiller_amd import configs
+
+    if case == "small_cfg2_fp32":
+        ps = [cases.small_cfg2(nf.NF_PREC_FP32_RECHECK)]
+    elif case == "cfg1_fp64":
+        ps = [cases.cfg1(nf.NF_PREC_FP64)]
+    else:
+        ps = configs.cfg3(n_inst=12, n_scale=1.0 / 40.0, max_rounds=30)
+    out = {}
+    for mode, opts in (("default", {}), ("exact", EXACT)):
+        with nf.Packing(ps if len(ps) > 1 else ps[0], opts) as pk:
+            sts = pk.generate()
+            out[mode] = (sts, [pk.rods(i) for i in range(len(ps))])
+    (sd, rd), (se, re_) = out["default"], out["exact"]
+    cut = 0
+    for i, p in enumerate(ps):
+        ro, so, _ = oracle.generate(p, threads=16)
+        for st, rods in ((sd[i], rd[i]), (se[i], re_[i])):
+            np.testing.assert_array_equal(rods.view(np.uint32), ro.view(np.uint32), err_msg=str(i))
+            assert nf.STATUS[st.status] == nf.STATUS[so["status"]], i
+            assert st.rounds == so["rounds"] and st.candidates == so["candidates"], i
+            assert st.rejected_old == so["rejected_old"] and st.rejected_new == so["rejected_new"], i
+        assert se[i].pair_tests == so["pair_tests"], i
+        assert 0 < sd[i].pair_tests <= so["pair_tests"], i
+        cut += so["pair_tests"] - sd[i].pair_tests
+    print(case, "pair tests skipped by the early exit:", cut, flush=True)

@@ -8,12 +8,15 @@ from tests import golden_cases as G
 
 pytestmark = pytest.mark.gpu
 
+# generations whose pair-test count is compared test every pair within reach
+EXACT = {nf.NF_OPT_TEST_ALL_PAIRS: 1}
+
 
 @pytest.mark.parametrize("name", G.NAMES)
 def test_device_matches_golden(name):
     p, meta, g = G.load(name)
     hs = meta["hist"]
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, EXACT) as pk:
         st = pk.generate()[0]
         rods = pk.rods()
         c = pk.contacts(p.contact_gap)
@@ -54,7 +57,7 @@ def test_headline_full_size_matches_oracle_digest():
     rods and of the contact list (I, J, image, gap32 and gap64 bits), written
     by tests/golden/make_full_size.py."""
     p, meta = G.load_digest("headline_500k")
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, EXACT) as pk:
         st = pk.generate()[0]
         rods = pk.rods()
         c = pk.contacts(p.contact_gap)
@@ -75,7 +78,7 @@ def test_contact_workload_full_size_matches_oracle_digest():
     the three 64-bin histograms about +x (tests/golden/make_full_size.py)."""
     p, meta = G.load_digest("contacts_4M")
     hs = meta["hist"]
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, EXACT) as pk:
         st = pk.generate()[0]
         rods = pk.rods()
         c = pk.contacts(p.contact_gap)
@@ -99,7 +102,7 @@ def test_network_workload_full_size_matches_oracle_digest():
     network (labels, CSR and FP64 value bits), and the network's counts
     (tests/golden/make_full_size.py)."""
     p, meta = G.load_digest("network_8M")
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, EXACT) as pk:
         st = pk.generate()[0]
         rods = pk.rods()
         c = pk.contacts(p.contact_gap)

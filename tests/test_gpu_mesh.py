@@ -8,6 +8,9 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
+# generations whose pair-test count is compared test every pair within reach
+EXACT = {nf.NF_OPT_TEST_ALL_PAIRS: 1}
+
 
 def _mesh(shape, seed):
     rng = np.random.default_rng(seed)
@@ -31,7 +34,7 @@ def _same(p, rods, st):
 @pytest.mark.parametrize("prec", [nf.NF_PREC_FP32_RECHECK, nf.NF_PREC_FP64])
 def test_mesh_packing_matches_oracle(prec):
     p = _p(prec)
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, EXACT) as pk:
         st = pk.generate()[0]
         rods = pk.rods()
     assert st.n_accepted == p.n_rods
@@ -45,7 +48,7 @@ def test_mesh_from_params_file(tmp_path):
                                     "INC_LEN_MED 0.5\nINC_RAD_MED 0.005\nRNG_SEED 9\n")
     p = nf.normalize(nf.load_params(str(tmp_path / "p.txt")))
     assert p.den_law == nf.NF_DEN_MESH
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, EXACT) as pk:
         st = pk.generate()[0]
         rods = pk.rods()
     _same(p, rods, st)
@@ -57,11 +60,11 @@ def test_mesh_in_mixed_batch():
                                          den_law=nf.NF_DEN_AGGLOM, n_agg=3, agg_sigma=1.0,
                                          agg_peak=2.0, stream=1)),
           _p(nf.NF_PREC_FP32_RECHECK, seed=6, n=1500, shape=(3, 7, 2))]
-    with nf.Packing(ps) as pk:
+    with nf.Packing(ps, EXACT) as pk:
         sts = pk.generate()
         batch = [pk.rods(i) for i in range(len(ps))]
     for i, p in enumerate(ps):
-        with nf.Packing(p) as pk:
+        with nf.Packing(p, EXACT) as pk:
             pk.generate()
             single = pk.rods()
         assert np.array_equal(batch[i].view(np.uint32), single.view(np.uint32)), i
