@@ -473,7 +473,7 @@ def leg_prof_main(a, out: dict) -> None:
     """One generation of the headline workload and nothing else: the
     command that rocprofv3 counter summaries of its kernels are taken on."""
     p = configs.cfg2(nf.NF_PREC_FP32_RECHECK)
-    with nf.Packing(p) as pk:
+    with nf.Packing(p, configs.tile_options("500k")) as pk:  # the timed steps' launch shape
         st = pk.generate()[0]
     out["extra"]["prof_main"] = {"workload": "one 500k-rod headline generation", "rounds": st.rounds,
                                  "rounds_launched": st.rounds_launched, "tile_bytes": st.tile_bytes,
