@@ -684,7 +684,6 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
   CellGeom g;
   Acc acc = {0u, 0u, 0u, 0.0, 0ull};
   uint32_t cur_inst = 0xFFFFFFFFu;
-  uint32_t inst_end = 0u;  // one past the current instance's last cell
   if (!GEN) {
     for (int d = 0; d < 3; ++d) {
       g.box[d] = a.box[d];
@@ -793,24 +792,9 @@ __global__ void __launch_bounds__(PAIR_THREADS, 4) k_pair(PairArgs a) {
             g.nCount = g.inv ? a.prCount : a.nbCount;
             g.nHl = g.inv ? a.prHl : a.nbHl;
           } else {
-            // the same-round pass: an instance with fewer than two
-            // survivors of the first pass has no pair
             g.inv = false;
-            g.idle = uni(a.prStart[d.cell_base + d.ncells]) - uni(a.prStart[d.cell_base]) < 2u;
+            g.idle = false;
           }
-          inst_end = d.cell_base + d.ncells;
-        }
-        if (g.idle) {
-          // an idle instance (finished, or without pairs this round) has no
-          // work; with the cells taken in index order its cells are
-          // consecutive items, and the queue moves past the rest of them at
-          // once (the counter only grows, so every wave still reaches the
-          // exit test above)
-          if (a.order == nullptr) {
-            const uint32_t e = min(inst_end << a.part_shift, qe) - qb;
-            if (l0) atomicMax(&a.qctr[q * QSTRIDE], e);
-          }
-          continue;
         }
       }
       process_cell<MODE, PREC>(a, g, gc, part, L, acc);
