@@ -16,7 +16,8 @@
 //   PM_GEN_OLD    new vs accepted rods   cuda_OldExtIsectLB (inc/intersection_kern.h:245-364)
 //                 (run accepted-major, with the roles of the two sets
 //                 swapped, once an instance's accepted rods outnumber the
-//                 round's proposals)
+//                 round's proposals four to one; otherwise a cut proposal
+//                 leaves its stencil walk)
 //   PM_GEN_NEW    new vs new, priority   cuda_InternalIsect_noLck / cuda_NewExtIsectLB_parted
 //                                        (inc/intersection_kern.h:12-241)
 //                 (over the first pass's survivors, compacted per cell;
